@@ -1,0 +1,219 @@
+/*
+ * mhh_oracle.h -- CPU FP64 restatement of the MicroHH dynamical-core hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py may load it.
+ * The product (libmicrohh_amd.so) never links, loads or calls it.
+ *
+ * Every function restates one reference loop nest, with the same expression
+ * tree and loop order, and cites the reference file:line it follows.  All
+ * 3-D arrays use the reference layout: ijk = i + j*jj + k*kk with
+ * jj = icells, kk = icells*jcells (include/grid.h:69-83).
+ *
+ * Parity pins (tests/test_oracle_*.py):
+ *   - golden vectors written by the compiled reference translation units
+ *     (oracle/_ref, built by oracle/Makefile from /root/reference sources):
+ *     advection 2i5 and 2, strain2, diff_u/v/w/c, dnmul, diff_2, N2,
+ *     buoyancy, rk3;
+ *   - closed forms / invariants / brute force for the parts whose reference
+ *     translation unit needs FFTW, NetCDF or Boost (pres_2 solve+tdma, FFT,
+ *     grid calculate, evisc, cyclic, ghost cells, randomize).
+ */
+#ifndef MHH_ORACLE_H
+#define MHH_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Loop bounds and strides shared by every kernel (include/grid.h:49-138). */
+typedef struct
+{
+    int itot, jtot, ktot;
+    int imax, jmax, kmax;
+    int igc, jgc, kgc;
+    int icells, jcells, kcells, ijcells, ncells;
+    int istart, iend, jstart, jend, kstart, kend;
+    double dx, dy, xsize, ysize, zsize;
+} mo_dims;
+
+void mo_dims_init(mo_dims* d, int itot, int jtot, int ktot,
+                  int igc, int jgc, int kgc,
+                  double xsize, double ysize, double zsize);
+
+/* Boundary types, same numbering as include/boundary.h:51. */
+enum { MO_BC_DIRICHLET = 0, MO_BC_NEUMANN = 1, MO_BC_FLUX = 2, MO_BC_USTAR = 3, MO_BC_OFF = 4 };
+
+/* ---- grid (src/grid.cxx:244-305, 2nd-order branch) ---- */
+void mo_grid_calculate(const mo_dims* d, double* z, double* zh, double* dz,
+                       double* dzh, double* dzi, double* dzhi);
+
+/* ---- fields (src/fields.cxx:994-1134) ---- */
+void mo_srand(unsigned int seed);
+int  mo_rand(void);
+void mo_randomize(const mo_dims* d, double* data, const double* z,
+                  double rndamp, double rndz, double rndexp);
+void mo_add_mean_prof(const mo_dims* d, double* data, const double* prof, double offset);
+
+/* ---- boundary (src/boundary_cyclic.cxx:369-443, src/boundary.cxx:51-91, 700-775) ---- */
+enum { MO_EDGE_EW = 0, MO_EDGE_NS = 1, MO_EDGE_BOTH = 2 };
+void mo_boundary_cyclic(const mo_dims* d, double* data, int edge);
+void mo_set_bc(const mo_dims* d, double* a, double* agrad, double* aflux,
+               int sw, double aval, double visc, double offset, int set_flux_grad);
+void mo_ghost_cells_bot_2nd(const mo_dims* d, double* a, const double* dzh, int bctype,
+                            const double* abot, const double* agradbot);
+void mo_ghost_cells_top_2nd(const mo_dims* d, double* a, const double* dzh, int bctype,
+                            double* atop, const double* agradtop);
+
+/* ---- smag2 viscosity (include/diff_kernels.h:34-142, src/diff_smag2.cxx:47-269,
+        src/thermo_dry.cxx:66-79) ---- */
+void mo_calc_strain2(const mo_dims* d, int surface_model, double* strain2,
+                     const double* u, const double* v, const double* w,
+                     const double* ugradbot, const double* vgradbot,
+                     const double* z, const double* dzi, const double* dzhi);
+void mo_calc_N2(const mo_dims* d, double* N2, const double* th, const double* dzi,
+                const double* thref);
+void mo_calc_evisc(const mo_dims* d, int surface_model, int sw_mason, double* evisc,
+                   const double* N2, const double* bgradbot, const double* z,
+                   const double* dz, const double* z0m, double cs, double tPr);
+void mo_calc_evisc_neutral(const mo_dims* d, int surface_model, int sw_mason, double* evisc,
+                           const double* u, const double* v,
+                           const double* z, const double* dz, const double* dzhi,
+                           const double* z0m, double cs, double visc);
+
+/* ---- smag2 diffusion (include/diff_kernels.h:144-511) ---- */
+void mo_diff_u(const mo_dims* d, int surface_model, double* ut,
+               const double* u, const double* v, const double* w,
+               const double* dzi, const double* dzhi, const double* evisc,
+               const double* fluxbot, const double* fluxtop,
+               const double* rhoref, const double* rhorefh, double visc);
+void mo_diff_v(const mo_dims* d, int surface_model, double* vt,
+               const double* u, const double* v, const double* w,
+               const double* dzi, const double* dzhi, const double* evisc,
+               const double* fluxbot, const double* fluxtop,
+               const double* rhoref, const double* rhorefh, double visc);
+void mo_diff_w(const mo_dims* d, double* wt,
+               const double* u, const double* v, const double* w,
+               const double* dzi, const double* dzhi, const double* evisc,
+               const double* rhoref, const double* rhorefh, double visc);
+void mo_diff_c(const mo_dims* d, int surface_model, double* at, const double* a,
+               const double* dzi, const double* dzhi, const double* evisc,
+               const double* fluxbot, const double* fluxtop,
+               const double* rhoref, const double* rhorefh, double tPr, double visc);
+double mo_calc_dnmul(const mo_dims* d, const double* evisc, const double* dzi, double tPr);
+
+/* ---- advection 2i5 (src/advec_2i5.cxx:59-728) ---- */
+double mo_advec_2i5_cfl(const mo_dims* d, const double* u, const double* v, const double* w,
+                        const double* dzi, double dt);
+void mo_advec_2i5_u(const mo_dims* d, double* ut, const double* u, const double* v,
+                    const double* w, const double* dzi, const double* rhoref, const double* rhorefh);
+void mo_advec_2i5_v(const mo_dims* d, double* vt, const double* u, const double* v,
+                    const double* w, const double* dzi, const double* rhoref, const double* rhorefh);
+void mo_advec_2i5_w(const mo_dims* d, double* wt, const double* u, const double* v,
+                    const double* w, const double* dzhi, const double* rhoref, const double* rhorefh);
+void mo_advec_2i5_s(const mo_dims* d, double* st, const double* s, const double* u,
+                    const double* v, const double* w, const double* dzi,
+                    const double* rhoref, const double* rhorefh);
+
+/* ---- advection 2 (src/advec_2.cxx:50-202) ---- */
+double mo_advec_2_cfl(const mo_dims* d, const double* u, const double* v, const double* w,
+                      const double* dzi, double dt);
+void mo_advec_2_u(const mo_dims* d, double* ut, const double* u, const double* v,
+                  const double* w, const double* dzi, const double* rhoref, const double* rhorefh);
+void mo_advec_2_v(const mo_dims* d, double* vt, const double* u, const double* v,
+                  const double* w, const double* dzi, const double* rhoref, const double* rhorefh);
+void mo_advec_2_w(const mo_dims* d, double* wt, const double* u, const double* v,
+                  const double* w, const double* dzhi, const double* rhoref, const double* rhorefh);
+void mo_advec_2_s(const mo_dims* d, double* st, const double* s, const double* u,
+                  const double* v, const double* w, const double* dzi,
+                  const double* rhoref, const double* rhorefh);
+
+/* ---- constant-viscosity diffusion (src/diff_2.cxx:39-90) ---- */
+void mo_diff_2_c(const mo_dims* d, double* at, const double* a, double visc,
+                 const double* dzi, const double* dzhi);
+void mo_diff_2_w(const mo_dims* d, double* wt, const double* w, double visc,
+                 const double* dzi, const double* dzhi);
+
+/* ---- dry thermodynamics (src/thermo_dry.cxx:163-180) ---- */
+void mo_buoyancy_tend_2nd(const mo_dims* d, double* wt, const double* th, const double* threfh);
+
+/* ---- pressure (src/pres_2.cxx:124-422, src/fft.cxx:338-452) ---- */
+void mo_pres_2_set_values(const mo_dims* d, double* bmati, double* bmatj,
+                          double* a, double* c, const double* dz, const double* dzhi,
+                          const double* rhorefh);
+void mo_pres_2_input(const mo_dims* d, double* p, const double* u, const double* v,
+                     const double* w, double* ut, double* vt, const double* wt,
+                     const double* dzi, const double* rhoref, const double* rhorefh, double dt);
+void mo_fft_r2hc(int n, const double* in, double* out);
+void mo_fft_hc2r(int n, const double* in, double* out);
+void mo_fft_forward(const mo_dims* d, double* data);
+void mo_fft_backward(const mo_dims* d, double* data, double* tmp1);
+void mo_tdma(double* a, double* b, double* c, double* p, double* work2d, double* work3d,
+             int iblock, int jblock, int kmax);
+void mo_pres_2_solve(const mo_dims* d, double* p, double* work3d, double* b,
+                     const double* dz, const double* rhoref,
+                     const double* bmati, const double* bmatj,
+                     const double* a, const double* c);
+void mo_pres_2_output(const mo_dims* d, double* ut, double* vt, double* wt,
+                      const double* p, const double* dzhi);
+double mo_pres_2_divergence(const mo_dims* d, const double* u, const double* v, const double* w,
+                            const double* dzi, const double* rhoref, const double* rhorefh);
+
+/* ---- time integration (src/timeloop.cxx:251-286, include/timeloop.h:142-146) ---- */
+void mo_rk3(const mo_dims* d, double* a, double* at, int substep, double dt);
+double mo_rk3_subdt(double dt, int substep);
+unsigned long mo_convert_to_itime(double t);
+
+/* ---- driver: one dry-CBL LES or channel case (src/model.cxx:356-545) ----
+ * The model owns all arrays; the caller reads them through mo_model_field().
+ */
+typedef struct mo_model mo_model;
+
+typedef struct
+{
+    int itot, jtot, ktot;
+    double xsize, ysize, zsize;
+    int advec;          /* 0: advec_2i5, 1: advec_2 */
+    int diff;           /* 0: smag2, 1: diff_2 */
+    int thermo;         /* 0: dry (Boussinesq, buoyancy + N2), 1: disabled */
+    double cflmax, dnmax, cs, tPr;
+    int swmason;
+    double visc, svisc;
+    double thref0;
+    /* boundary = "default" (no surface model) */
+    int mbcbot, mbctop, sbcbot, sbctop;
+    double ubot, vbot, utop, vtop, sbot, stop;
+    double utrans, vtrans;
+    /* randomizer (fields) */
+    int rndseed;
+    double rndamp_u, rndamp_v, rndamp_w, rndamp_th;
+    double rndz, rndexp;
+    /* time */
+    int adaptivestep;
+    double dt, dtmax, endtime, savetime;
+    int nthreads;       /* OpenMP threads for the timed baseline (1 = serial reference semantics) */
+} mo_case;
+
+mo_model* mo_model_create(const mo_case* c, const double* z_interior,
+                          const double* u_prof, const double* v_prof, const double* th_prof);
+void   mo_model_destroy(mo_model* m);
+double* mo_model_field(mo_model* m, const char* name);
+const mo_dims* mo_model_dims(const mo_model* m);
+/* Runs one substep of the reference order: cyclic, ghost, viscosity, dt, thermo,
+ * advection, diffusion, pressure, rk3, step_time.  Returns 0 on success. */
+int    mo_model_substep(mo_model* m);
+double mo_model_dt(const mo_model* m);
+double mo_model_time(const mo_model* m);
+int    mo_model_substep_index(const mo_model* m);
+int    mo_model_iteration(const mo_model* m);
+double mo_model_cfl(mo_model* m);
+double mo_model_dn(mo_model* m);
+double mo_model_div(mo_model* m);
+/* Tendency snapshot hooks for per-stage parity: when non-zero, the model copies
+ * ut/vt/wt/tht right before the pressure solve into the *_pre fields. */
+void   mo_model_set_capture(mo_model* m, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,350 @@
+/*
+ * mhh_oracle_model.c -- CPU FP64 restatement of the dry-CBL driver loop
+ * (src/model.cxx:356-545) for the in-scope operators, with the reference's
+ * integer time stepping (src/timeloop.cxx:138-246, include/timeloop.h:142-146).
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "mhh_oracle.h"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define NFIELDS 18
+
+struct mo_model
+{
+    mo_case c;
+    mo_dims d;
+    double *z, *zh, *dz, *dzh, *dzi, *dzhi;
+    double *rhoref, *rhorefh, *thref, *threfh;
+    double *bmati, *bmatj, *pa, *pc;
+    /* 3-D fields */
+    double *u, *v, *w, *th, *ut, *vt, *wt, *tht, *p, *evisc, *tmp1, *tmp2;
+    double *ut_pre, *vt_pre, *wt_pre, *tht_pre;
+    /* 2-D boundary arrays (ijcells): value, gradient, flux at bottom and top */
+    double *u_bot, *u_top, *u_gbot, *u_gtop, *u_fbot, *u_ftop;
+    double *v_bot, *v_top, *v_gbot, *v_gtop, *v_fbot, *v_ftop;
+    double *s_bot, *s_top, *s_gbot, *s_gtop, *s_fbot, *s_ftop;
+    /* time loop */
+    int substep, iteration, capture;
+    unsigned long itime, idt, idtmax, iendtime, isavetime, idtlim;
+    double time, dt;
+};
+
+static double* zalloc(size_t n) { return (double*)calloc(n, sizeof(double)); }
+
+const mo_dims* mo_model_dims(const mo_model* m) { return &m->d; }
+double mo_model_dt(const mo_model* m) { return m->dt; }
+double mo_model_time(const mo_model* m) { return m->time; }
+int mo_model_substep_index(const mo_model* m) { return m->substep; }
+int mo_model_iteration(const mo_model* m) { return m->iteration; }
+void mo_model_set_capture(mo_model* m, int on) { m->capture = on; }
+
+double* mo_model_field(mo_model* m, const char* n)
+{
+#define F(name) if (!strcmp(n, #name)) return m->name;
+    F(u) F(v) F(w) F(th) F(ut) F(vt) F(wt) F(tht) F(p) F(evisc) F(tmp1) F(tmp2)
+    F(ut_pre) F(vt_pre) F(wt_pre) F(tht_pre)
+    F(z) F(zh) F(dz) F(dzh) F(dzi) F(dzhi) F(rhoref) F(rhorefh) F(thref) F(threfh)
+    F(bmati) F(bmatj) F(pa) F(pc)
+    F(u_bot) F(u_top) F(v_bot) F(v_top) F(s_bot) F(s_top)
+    F(u_gbot) F(u_gtop) F(v_gbot) F(v_gtop) F(s_gbot) F(s_gtop)
+    F(u_fbot) F(u_ftop) F(v_fbot) F(v_ftop) F(s_fbot) F(s_ftop)
+#undef F
+    return NULL;
+}
+
+mo_model* mo_model_create(const mo_case* c, const double* z_interior,
+                          const double* u_prof, const double* v_prof, const double* th_prof)
+{
+    mo_model* m = (mo_model*)calloc(1, sizeof(mo_model));
+    m->c = *c;
+    /* advec_2i5 raises the horizontal ghost cells to 3 (src/advec_2i5.cxx:42-45);
+       advec_2 and the 2nd-order grid need 1. */
+    const int gc = (c->advec == 0) ? 3 : 1;
+    mo_dims_init(&m->d, c->itot, c->jtot, c->ktot, gc, gc, 1, c->xsize, c->ysize, c->zsize);
+    const mo_dims* d = &m->d;
+    const size_t n3 = (size_t)d->ncells, n2 = (size_t)d->ijcells, n1 = (size_t)d->kcells;
+
+    m->z = zalloc(n1); m->zh = zalloc(n1); m->dz = zalloc(n1); m->dzh = zalloc(n1);
+    m->dzi = zalloc(n1); m->dzhi = zalloc(n1);
+    m->rhoref = zalloc(n1); m->rhorefh = zalloc(n1); m->thref = zalloc(n1); m->threfh = zalloc(n1);
+    m->bmati = zalloc(d->itot); m->bmatj = zalloc(d->jtot); m->pa = zalloc(d->kmax); m->pc = zalloc(d->kmax);
+
+    double** f3[] = {&m->u, &m->v, &m->w, &m->th, &m->ut, &m->vt, &m->wt, &m->tht,
+                     &m->p, &m->evisc, &m->tmp1, &m->tmp2,
+                     &m->ut_pre, &m->vt_pre, &m->wt_pre, &m->tht_pre};
+    for (size_t q=0; q<sizeof(f3)/sizeof(f3[0]); ++q) *f3[q] = zalloc(n3);
+    double** f2[] = {&m->u_bot, &m->u_top, &m->u_gbot, &m->u_gtop, &m->u_fbot, &m->u_ftop,
+                     &m->v_bot, &m->v_top, &m->v_gbot, &m->v_gtop, &m->v_fbot, &m->v_ftop,
+                     &m->s_bot, &m->s_top, &m->s_gbot, &m->s_gtop, &m->s_fbot, &m->s_ftop};
+    for (size_t q=0; q<sizeof(f2)/sizeof(f2[0]); ++q) *f2[q] = zalloc(n2);
+
+    /* Grid::create + calculate (src/grid.cxx:221-305). */
+    for (int k=0; k<d->ktot; ++k) m->z[k + d->kstart] = z_interior[k];
+    mo_grid_calculate(d, m->z, m->zh, m->dz, m->dzh, m->dzi, m->dzhi);
+
+    /* Boussinesq base state (src/thermo_dry.cxx:558-572); rhoref = 1 everywhere. */
+    for (int k=0; k<d->kcells; ++k)
+    {
+        m->rhoref[k] = 1.; m->rhorefh[k] = 1.;
+        m->thref[k] = c->thref0; m->threfh[k] = c->thref0;
+    }
+
+    /* Fields::create (src/fields.cxx:994-1032): seed once, randomize u, w, v, th. */
+    mo_srand((unsigned int)(c->rndseed + 2));
+    mo_randomize(d, m->u, m->z, c->rndamp_u, c->rndz, c->rndexp);
+    mo_randomize(d, m->w, m->z, c->rndamp_w, c->rndz, c->rndexp);
+    if (d->jtot > 1)
+        mo_randomize(d, m->v, m->z, c->rndamp_v, c->rndz, c->rndexp);
+    if (c->thermo == 0)
+        mo_randomize(d, m->th, m->z, c->rndamp_th, c->rndz, c->rndexp);
+    mo_add_mean_prof(d, m->u, u_prof, c->utrans);
+    mo_add_mean_prof(d, m->v, v_prof, c->vtrans);
+    if (c->thermo == 0)
+    {
+        mo_add_mean_prof(d, m->th, th_prof, 0.);
+        for (int ij=0; ij<d->ijcells; ++ij) m->s_bot[ij] = th_prof[0];
+    }
+    for (int ij=0; ij<d->ijcells; ++ij)
+    {
+        m->w[ij + d->kstart*d->ijcells] = 0.;
+        m->w[ij + d->kend*d->ijcells] = 0.;
+    }
+
+    /* Boundary::set_values (src/boundary.cxx:599-696), default switch. */
+    mo_set_bc(d, m->u_bot, m->u_gbot, m->u_fbot, c->mbcbot, c->ubot, c->visc, c->utrans, 1);
+    mo_set_bc(d, m->v_bot, m->v_gbot, m->v_fbot, c->mbcbot, c->vbot, c->visc, c->vtrans, 1);
+    mo_set_bc(d, m->u_top, m->u_gtop, m->u_ftop, c->mbctop, c->utop, c->visc, c->utrans, 1);
+    mo_set_bc(d, m->v_top, m->v_gtop, m->v_ftop, c->mbctop, c->vtop, c->visc, c->vtrans, 1);
+    if (c->thermo == 0)
+    {
+        mo_set_bc(d, m->s_bot, m->s_gbot, m->s_fbot, c->sbcbot, c->sbot, c->svisc, 0., 1);
+        mo_set_bc(d, m->s_top, m->s_gtop, m->s_ftop, c->sbctop, c->stop, c->svisc, 0., 1);
+    }
+
+    /* Pres_2::set_values (src/pres_2.cxx:124-153). */
+    mo_pres_2_set_values(d, m->bmati, m->bmatj, m->pa, m->pc, m->dz, m->dzhi, m->rhorefh);
+
+    /* Timeloop constructor (src/timeloop.cxx:40-130). */
+    m->dt = c->dt;
+    m->idt = mo_convert_to_itime(c->dt);
+    m->idtmax = mo_convert_to_itime(c->dtmax);
+    m->iendtime = mo_convert_to_itime(c->endtime);
+    m->isavetime = mo_convert_to_itime(c->savetime);
+    m->idtlim = m->idt;
+    m->itime = 0; m->time = 0.; m->substep = 0; m->iteration = 0;
+    return m;
+}
+
+void mo_model_destroy(mo_model* m)
+{
+    if (!m) return;
+    double* all[] = {m->z, m->zh, m->dz, m->dzh, m->dzi, m->dzhi, m->rhoref, m->rhorefh,
+                     m->thref, m->threfh, m->bmati, m->bmatj, m->pa, m->pc,
+                     m->u, m->v, m->w, m->th, m->ut, m->vt, m->wt, m->tht, m->p, m->evisc,
+                     m->tmp1, m->tmp2, m->ut_pre, m->vt_pre, m->wt_pre, m->tht_pre,
+                     m->u_bot, m->u_top, m->u_gbot, m->u_gtop, m->u_fbot, m->u_ftop,
+                     m->v_bot, m->v_top, m->v_gbot, m->v_gtop, m->v_fbot, m->v_ftop,
+                     m->s_bot, m->s_top, m->s_gbot, m->s_gtop, m->s_fbot, m->s_ftop};
+    for (size_t q=0; q<sizeof(all)/sizeof(all[0]); ++q) free(all[q]);
+    free(m);
+}
+
+static void set_ghost_cells(mo_model* m)
+{
+    /* src/boundary.cxx:932-997 (2nd order); w has no ghost update in 2nd order. */
+    const mo_dims* d = &m->d;
+    const mo_case* c = &m->c;
+    mo_ghost_cells_bot_2nd(d, m->u, m->dzh, c->mbcbot, m->u_bot, m->u_gbot);
+    mo_ghost_cells_top_2nd(d, m->u, m->dzh, c->mbctop, m->u_top, m->u_gtop);
+    mo_ghost_cells_bot_2nd(d, m->v, m->dzh, c->mbcbot, m->v_bot, m->v_gbot);
+    mo_ghost_cells_top_2nd(d, m->v, m->dzh, c->mbctop, m->v_top, m->v_gtop);
+    if (c->thermo == 0)
+    {
+        mo_ghost_cells_bot_2nd(d, m->th, m->dzh, c->sbcbot, m->s_bot, m->s_gbot);
+        mo_ghost_cells_top_2nd(d, m->th, m->dzh, c->sbctop, m->s_top, m->s_gtop);
+    }
+}
+
+double mo_model_cfl(mo_model* m)
+{
+    if (m->c.advec == 0)
+        return mo_advec_2i5_cfl(&m->d, m->u, m->v, m->w, m->dzi, m->dt);
+    return mo_advec_2_cfl(&m->d, m->u, m->v, m->w, m->dzi, m->dt);
+}
+
+double mo_model_dn(mo_model* m)
+{
+    if (m->c.diff == 0)
+        return mo_calc_dnmul(&m->d, m->evisc, m->dzi, m->c.tPr) * m->dt;
+    /* Diff_2: constant viscosity (src/diff_2.cxx:114-130). */
+    const mo_dims* d = &m->d;
+    double viscmax = m->c.visc > m->c.svisc ? m->c.visc : m->c.svisc;
+    double dnmul = 0.;
+    for (int k=d->kstart; k<d->kend; ++k)
+    {
+        double v = fabs(viscmax*(1./(d->dx*d->dx) + 1./(d->dy*d->dy) + 1./(m->dz[k]*m->dz[k])));
+        if (v > dnmul) dnmul = v;
+    }
+    return dnmul * m->dt;
+}
+
+double mo_model_div(mo_model* m)
+{
+    return mo_pres_2_divergence(&m->d, m->u, m->v, m->w, m->dzi, m->rhoref, m->rhorefh);
+}
+
+static void set_time_step(mo_model* m)
+{
+    /* src/model.cxx:764-786 with src/timeloop.cxx:138-246. */
+    if (m->substep > 0) return;
+    m->idtlim = m->idtmax;
+    {
+        unsigned long lim = m->isavetime - m->itime % m->isavetime;
+        if (lim < m->idtlim) m->idtlim = lim;
+        if (m->itime < m->iendtime && m->iendtime - m->itime < m->idtlim)
+            m->idtlim = m->iendtime - m->itime;
+    }
+    /* Advec::get_time_limit (src/advec_2i5.cxx:973-992, src/advec_2.cxx:300-309) */
+    {
+        double cfl = mo_model_cfl(m);
+        if (cfl < 1.e-5) cfl = 1.e-5;
+        unsigned long lim = (unsigned long)(m->idt * m->c.cflmax / cfl);
+        if (lim < m->idtlim) m->idtlim = lim;
+    }
+    /* Diff::get_time_limit (src/diff_smag2.cxx:312-333, src/diff_2.cxx:98-102) */
+    {
+        unsigned long lim;
+        if (m->c.diff == 0)
+        {
+            double dnmul = mo_calc_dnmul(&m->d, m->evisc, m->dzi, m->c.tPr);
+            if (dnmul < 1.e-9) dnmul = 1.e-9;
+            lim = (unsigned long)(m->idt * m->c.dnmax / (m->dt * dnmul));
+        }
+        else
+        {
+            double dnmul = mo_model_dn(m) / m->dt;
+            lim = (unsigned long)(m->idt * m->c.dnmax / (m->dt * dnmul));
+        }
+        if (lim < m->idtlim) m->idtlim = lim;
+    }
+    if (m->c.adaptivestep)
+    {
+        m->idt = m->idtlim;
+        m->dt = (double)m->idt / 1e9;
+    }
+}
+
+int mo_model_substep(mo_model* m)
+{
+    const mo_dims* d = &m->d;
+    const mo_case* c = &m->c;
+#ifdef _OPENMP
+    omp_set_num_threads(c->nthreads > 0 ? c->nthreads : 1);
+#endif
+
+    /* 1. cyclic + ghost cells (src/model.cxx:368-370) */
+    mo_boundary_cyclic(d, m->u, MO_EDGE_BOTH);
+    mo_boundary_cyclic(d, m->v, MO_EDGE_BOTH);
+    mo_boundary_cyclic(d, m->w, MO_EDGE_BOTH);
+    if (c->thermo == 0) mo_boundary_cyclic(d, m->th, MO_EDGE_BOTH);
+    set_ghost_cells(m);
+
+    /* 2. viscosity (src/diff_smag2.cxx:473-607) */
+    if (c->diff == 0)
+    {
+        mo_calc_strain2(d, 0, m->evisc, m->u, m->v, m->w, NULL, NULL, m->z, m->dzi, m->dzhi);
+        if (c->thermo == 0)
+        {
+            mo_calc_N2(d, m->tmp1, m->th, m->dzi, m->thref);
+            mo_calc_evisc(d, 0, c->swmason, m->evisc, m->tmp1, NULL, m->z, m->dz, NULL, c->cs, c->tPr);
+        }
+        else
+            mo_calc_evisc_neutral(d, 0, c->swmason, m->evisc, m->u, m->v, m->z, m->dz, m->dzhi,
+                                  NULL, c->cs, c->visc);
+    }
+
+    /* 3. time step at substep 0 */
+    set_time_step(m);
+    const double subdt = mo_rk3_subdt(m->dt, m->substep);
+
+    /* 4. buoyancy (src/thermo_dry.cxx:609-643) */
+    if (c->thermo == 0)
+        mo_buoyancy_tend_2nd(d, m->wt, m->th, m->threfh);
+
+    /* 5. boundary exec is a no-op for the default switch; ghost cells again */
+    set_ghost_cells(m);
+
+    /* 6. advection */
+    if (c->advec == 0)
+    {
+        mo_advec_2i5_u(d, m->ut, m->u, m->v, m->w, m->dzi, m->rhoref, m->rhorefh);
+        mo_advec_2i5_v(d, m->vt, m->u, m->v, m->w, m->dzi, m->rhoref, m->rhorefh);
+        mo_advec_2i5_w(d, m->wt, m->u, m->v, m->w, m->dzhi, m->rhoref, m->rhorefh);
+        if (c->thermo == 0)
+            mo_advec_2i5_s(d, m->tht, m->th, m->u, m->v, m->w, m->dzi, m->rhoref, m->rhorefh);
+    }
+    else
+    {
+        mo_advec_2_u(d, m->ut, m->u, m->v, m->w, m->dzi, m->rhoref, m->rhorefh);
+        mo_advec_2_v(d, m->vt, m->u, m->v, m->w, m->dzi, m->rhoref, m->rhorefh);
+        mo_advec_2_w(d, m->wt, m->u, m->v, m->w, m->dzhi, m->rhoref, m->rhorefh);
+        if (c->thermo == 0)
+            mo_advec_2_s(d, m->tht, m->th, m->u, m->v, m->w, m->dzi, m->rhoref, m->rhorefh);
+    }
+
+    /* 7. diffusion */
+    if (c->diff == 0)
+    {
+        mo_diff_u(d, 0, m->ut, m->u, m->v, m->w, m->dzi, m->dzhi, m->evisc, m->u_fbot, m->u_ftop,
+                  m->rhoref, m->rhorefh, c->visc);
+        mo_diff_v(d, 0, m->vt, m->u, m->v, m->w, m->dzi, m->dzhi, m->evisc, m->v_fbot, m->v_ftop,
+                  m->rhoref, m->rhorefh, c->visc);
+        mo_diff_w(d, m->wt, m->u, m->v, m->w, m->dzi, m->dzhi, m->evisc, m->rhoref, m->rhorefh, c->visc);
+        if (c->thermo == 0)
+            mo_diff_c(d, 0, m->tht, m->th, m->dzi, m->dzhi, m->evisc, m->s_fbot, m->s_ftop,
+                      m->rhoref, m->rhorefh, c->tPr, c->svisc);
+    }
+    else
+    {
+        mo_diff_2_c(d, m->ut, m->u, c->visc, m->dzi, m->dzhi);
+        mo_diff_2_c(d, m->vt, m->v, c->visc, m->dzi, m->dzhi);
+        mo_diff_2_w(d, m->wt, m->w, c->visc, m->dzi, m->dzhi);
+        if (c->thermo == 0)
+            mo_diff_2_c(d, m->tht, m->th, c->svisc, m->dzi, m->dzhi);
+    }
+
+    if (m->capture)
+    {
+        const size_t nb = sizeof(double)*(size_t)d->ncells;
+        memcpy(m->ut_pre, m->ut, nb); memcpy(m->vt_pre, m->vt, nb);
+        memcpy(m->wt_pre, m->wt, nb); memcpy(m->tht_pre, m->tht, nb);
+    }
+
+    /* 8. pressure (src/pres_2.cxx:64-94) */
+    mo_pres_2_input(d, m->p, m->u, m->v, m->w, m->ut, m->vt, m->wt, m->dzi, m->rhoref, m->rhorefh, subdt);
+    mo_pres_2_solve(d, m->p, m->tmp1, m->tmp2, m->dz, m->rhoref, m->bmati, m->bmatj, m->pa, m->pc);
+    mo_pres_2_output(d, m->ut, m->vt, m->wt, m->p, m->dzhi);
+
+    /* 9. time integration (src/timeloop.cxx:358-412), fields.at in map order */
+    if (c->thermo == 0) mo_rk3(d, m->th, m->tht, m->substep, m->dt);
+    mo_rk3(d, m->u, m->ut, m->substep, m->dt);
+    mo_rk3(d, m->v, m->vt, m->substep, m->dt);
+    mo_rk3(d, m->w, m->wt, m->substep, m->dt);
+    m->substep = (m->substep + 1) % 3;
+
+    /* 10. step_time (src/timeloop.cxx:161-176) */
+    if (m->substep == 0)
+    {
+        m->time += m->dt;
+        m->itime += m->idt;
+        ++m->iteration;
+    }
+    return 0;
+}

@@ -1,0 +1,351 @@
+/*
+ * mhh_oracle_pres.c -- CPU FP64 restatement of the 2nd-order pressure
+ * projection (src/pres_2.cxx) and the serial FFT path (src/fft.cxx).
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product).
+ *
+ * FFTW3 (fftw_plan_many_r2r with FFTW_R2HC / FFTW_HC2R, src/fft.cxx:194-201)
+ * is a third-party dependency absent from this image and not vendored by the
+ * reference; CI installs Ubuntu 24.04's libfftw3-dev (3.3.10).  Its published
+ * transform definitions are restated here: R2HC is the unnormalised real DFT
+ * Y_k = sum_j x_j exp(-2 pi i j k / n) stored as r_0..r_{n/2}, i_{(n+1)/2-1}..i_1;
+ * HC2R is its unnormalised inverse.  The complex FFT underneath is a plain
+ * mixed-radix decimation-in-time recursion.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "mhh_oracle.h"
+
+typedef struct { double re, im; } cplx;
+
+static int smallest_factor(int n)
+{
+    if (n % 4 == 0 && n != 4) return 4;
+    for (int p=2; p*p<=n; ++p)
+        if (n % p == 0) return p;
+    return n;
+}
+
+/* out[0..n) = DFT_n of in[0], in[s], ..., with roots W[t*ws] = exp(-2 pi i t ws / N). */
+static void fft_rec(cplx* out, const cplx* in, int n, int s, const cplx* W, int ws, int N)
+{
+    if (n == 1) { out[0] = in[0]; return; }
+    const int p = smallest_factor(n);
+    const int m = n / p;
+    for (int r=0; r<p; ++r)
+        fft_rec(out + r*m, in + r*s, m, s*p, W, ws*p, N);
+
+    cplx t[64];
+    cplx* tt = (p <= 64) ? t : (cplx*)malloc(sizeof(cplx)*p);
+    for (int k=0; k<m; ++k)
+    {
+        for (int r=0; r<p; ++r)
+        {
+            const cplx a = out[r*m + k];
+            const cplx w = W[((long)r*k % n) * ws];
+            tt[r].re = a.re*w.re - a.im*w.im;
+            tt[r].im = a.re*w.im + a.im*w.re;
+        }
+        for (int q=0; q<p; ++q)
+        {
+            double sr = 0., si = 0.;
+            for (int r=0; r<p; ++r)
+            {
+                const cplx w = W[((long)r*q*m % n) * ws];
+                sr += tt[r].re*w.re - tt[r].im*w.im;
+                si += tt[r].re*w.im + tt[r].im*w.re;
+            }
+            out[k + q*m].re = sr;
+            out[k + q*m].im = si;
+        }
+    }
+    if (tt != t) free(tt);
+}
+
+static cplx* roots(int n)
+{
+    cplx* W = (cplx*)malloc(sizeof(cplx)*n);
+    const double pi = acos(-1.);
+    for (int t=0; t<n; ++t)
+    {
+        W[t].re =  cos(2.*pi*(double)t/(double)n);
+        W[t].im = -sin(2.*pi*(double)t/(double)n);
+    }
+    return W;
+}
+
+void mo_fft_r2hc(int n, const double* in, double* out)
+{
+    cplx* a = (cplx*)malloc(sizeof(cplx)*n);
+    cplx* y = (cplx*)malloc(sizeof(cplx)*n);
+    cplx* W = roots(n);
+    for (int j=0; j<n; ++j) { a[j].re = in[j]; a[j].im = 0.; }
+    fft_rec(y, a, n, 1, W, 1, n);
+    for (int k=0; k<=n/2; ++k) out[k] = y[k].re;
+    for (int k=1; k<(n+1)/2; ++k) out[n-k] = y[k].im;
+    free(a); free(y); free(W);
+}
+
+void mo_fft_hc2r(int n, const double* in, double* out)
+{
+    /* x_j = sum_k Y_k exp(+2 pi i j k / n): conjugate input, forward DFT, real part. */
+    cplx* a = (cplx*)malloc(sizeof(cplx)*n);
+    cplx* y = (cplx*)malloc(sizeof(cplx)*n);
+    cplx* W = roots(n);
+    a[0].re = in[0]; a[0].im = 0.;
+    for (int k=1; k<(n+1)/2; ++k)
+    {
+        a[k].re = in[k];   a[k].im = -in[n-k];     /* conj(Y_k) */
+        a[n-k].re = in[k]; a[n-k].im = in[n-k];    /* conj(Y_{n-k}) = Y_k */
+    }
+    if (n % 2 == 0) { a[n/2].re = in[n/2]; a[n/2].im = 0.; }
+    fft_rec(y, a, n, 1, W, 1, n);
+    for (int j=0; j<n; ++j) out[j] = y[j].re;
+    free(a); free(y); free(W);
+}
+
+/* --------------------------------------------------------- pres_2 */
+void mo_pres_2_set_values(const mo_dims* d, double* bmati, double* bmatj,
+                          double* a, double* c, const double* dz, const double* dzhi,
+                          const double* rhorefh)
+{
+    /* src/pres_2.cxx:124-153 */
+    const double dxidxi = 1./(d->dx*d->dx);
+    const double dyidyi = 1./(d->dy*d->dy);
+    const double pi = acos(-1.);
+    for (int j=0; j<d->jtot/2+1; ++j)
+        bmatj[j] = 2. * (cos(2.*pi*(double)j/(double)d->jtot)-1.) * dyidyi;
+    for (int j=d->jtot/2+1; j<d->jtot; ++j)
+        bmatj[j] = bmatj[d->jtot-j];
+    for (int i=0; i<d->itot/2+1; ++i)
+        bmati[i] = 2. * (cos(2.*pi*(double)i/(double)d->itot)-1.) * dxidxi;
+    for (int i=d->itot/2+1; i<d->itot; ++i)
+        bmati[i] = bmati[d->itot-i];
+    for (int k=0; k<d->kmax; ++k)
+    {
+        a[k] = dz[k+d->kgc] * rhorefh[k+d->kgc  ]*dzhi[k+d->kgc  ];
+        c[k] = dz[k+d->kgc] * rhorefh[k+d->kgc+1]*dzhi[k+d->kgc+1];
+    }
+}
+
+void mo_pres_2_input(const mo_dims* d, double* p, const double* u, const double* v,
+                     const double* w, double* ut, double* vt, const double* wt,
+                     const double* dzi, const double* rhoref, const double* rhorefh, double dt)
+{
+    /* src/pres_2.cxx:155-196 */
+    const int ii = 1, jj = d->icells, kk = d->ijcells;
+    const int jjp = d->imax, kkp = d->imax*d->jmax;
+    const double dxi = 1./d->dx, dyi = 1./d->dy, dti = 1./dt;
+    const int igc = d->igc, jgc = d->jgc, kgc = d->kgc;
+
+    mo_boundary_cyclic(d, ut, MO_EDGE_EW);
+    mo_boundary_cyclic(d, vt, MO_EDGE_NS);
+
+    for (int k=0; k<d->kmax; ++k)
+        for (int j=0; j<d->jmax; ++j)
+            for (int i=0; i<d->imax; ++i)
+            {
+                const int ijkp = i + j*jjp + k*kkp;
+                const int ijk  = i+igc + (j+jgc)*jj + (k+kgc)*kk;
+                p[ijkp] = rhoref[k+kgc] * ( (ut[ijk+ii] + u[ijk+ii] * dti) - (ut[ijk] + u[ijk] * dti) ) * dxi
+                        + rhoref[k+kgc] * ( (vt[ijk+jj] + v[ijk+jj] * dti) - (vt[ijk] + v[ijk] * dti) ) * dyi
+                        + ( rhorefh[k+kgc+1] * (wt[ijk+kk] + w[ijk+kk] * dti)
+                          - rhorefh[k+kgc  ] * (wt[ijk   ] + w[ijk   ] * dti) ) * dzi[k+kgc];
+            }
+}
+
+void mo_fft_forward(const mo_dims* d, double* data)
+{
+    /* src/fft.cxx:338-394 (serial): per k-slice, R2HC along x for jmax rows,
+       then R2HC along y for iblock columns (stride iblock). */
+    const int itot = d->itot, jtot = d->jtot, jmax = d->jmax, iblock = d->itot;
+    double* in  = (double*)malloc(sizeof(double)*(itot > jtot ? itot : jtot));
+    double* out = (double*)malloc(sizeof(double)*(itot > jtot ? itot : jtot));
+
+    int kk = itot*jmax;
+    for (int k=0; k<d->ktot; ++k)
+        for (int j=0; j<jmax; ++j)
+        {
+            double* row = data + j*itot + k*kk;
+            mo_fft_r2hc(itot, row, out);
+            memcpy(row, out, sizeof(double)*itot);
+        }
+
+    kk = iblock*jtot;
+    for (int k=0; k<d->ktot; ++k)
+        for (int i=0; i<iblock; ++i)
+        {
+            for (int j=0; j<jtot; ++j) in[j] = data[i + j*iblock + k*kk];
+            mo_fft_r2hc(jtot, in, out);
+            for (int j=0; j<jtot; ++j) data[i + j*iblock + k*kk] = out[j];
+        }
+    free(in); free(out);
+}
+
+void mo_fft_backward(const mo_dims* d, double* data, double* tmp1)
+{
+    /* src/fft.cxx:396-452 (serial): HC2R along y (/jtot) into data, then
+       HC2R along x (/itot) into tmp1. */
+    const int itot = d->itot, jtot = d->jtot, jmax = d->jmax, iblock = d->itot;
+    double* in  = (double*)malloc(sizeof(double)*(itot > jtot ? itot : jtot));
+    double* out = (double*)malloc(sizeof(double)*(itot > jtot ? itot : jtot));
+
+    int kk = iblock*jtot;
+    for (int k=0; k<d->ktot; ++k)
+        for (int i=0; i<iblock; ++i)
+        {
+            for (int j=0; j<jtot; ++j) in[j] = data[i + j*iblock + k*kk];
+            mo_fft_hc2r(jtot, in, out);
+            for (int j=0; j<jtot; ++j) data[i + j*iblock + k*kk] = out[j] / jtot;
+        }
+
+    kk = itot*jmax;
+    for (int k=0; k<d->ktot; ++k)
+        for (int j=0; j<jmax; ++j)
+        {
+            mo_fft_hc2r(itot, data + j*itot + k*kk, out);
+            for (int i=0; i<itot; ++i) tmp1[i + j*itot + k*kk] = out[i] / itot;
+        }
+    free(in); free(out);
+}
+
+void mo_tdma(double* a, double* b, double* c, double* p, double* work2d, double* work3d,
+             int iblock, int jblock, int kmax)
+{
+    /* src/pres_2.cxx:200-263 (Thomas algorithm, Numerical Recipes form). */
+    const int jj = iblock, kk = iblock*jblock;
+    for (int j=0; j<jblock; ++j)
+        for (int i=0; i<iblock; ++i)
+            work2d[i + j*jj] = b[i + j*jj];
+    for (int j=0; j<jblock; ++j)
+        for (int i=0; i<iblock; ++i)
+            p[i + j*jj] /= work2d[i + j*jj];
+
+    for (int k=1; k<kmax; ++k)
+    {
+        for (int j=0; j<jblock; ++j)
+            for (int i=0; i<iblock; ++i)
+            {
+                const int ij = i + j*jj, ijk = ij + k*kk;
+                work3d[ijk] = c[k-1] / work2d[ij];
+            }
+        for (int j=0; j<jblock; ++j)
+            for (int i=0; i<iblock; ++i)
+            {
+                const int ij = i + j*jj, ijk = ij + k*kk;
+                work2d[ij] = b[ijk] - a[k]*work3d[ijk];
+            }
+        for (int j=0; j<jblock; ++j)
+            for (int i=0; i<iblock; ++i)
+            {
+                const int ij = i + j*jj, ijk = ij + k*kk;
+                p[ijk] -= a[k]*p[ijk-kk];
+                p[ijk] /= work2d[ij];
+            }
+    }
+
+    for (int k=kmax-2; k>=0; --k)
+        for (int j=0; j<jblock; ++j)
+            for (int i=0; i<iblock; ++i)
+            {
+                const int ijk = i + j*jj + k*kk;
+                p[ijk] -= work3d[ijk+kk]*p[ijk+kk];
+            }
+}
+
+void mo_pres_2_solve(const mo_dims* d, double* p, double* work3d, double* b,
+                     const double* dz, const double* rhoref,
+                     const double* bmati, const double* bmatj,
+                     const double* a, const double* c)
+{
+    /* src/pres_2.cxx:266-362 (serial: mpicoordx = mpicoordy = 0). */
+    const int imax = d->imax, jmax = d->jmax, kmax = d->kmax;
+    const int iblock = d->itot, jblock = d->jtot, kgc = d->kgc;
+
+    mo_fft_forward(d, p);
+
+    int jj = iblock, kk = iblock*jblock;
+    for (int k=0; k<kmax; ++k)
+        for (int j=0; j<jblock; ++j)
+            for (int i=0; i<iblock; ++i)
+            {
+                const int ijk = i + j*jj + k*kk;
+                b[ijk] = dz[k+kgc]*dz[k+kgc] * rhoref[k+kgc]*(bmati[i]+bmatj[j]) - (a[k]+c[k]);
+                p[ijk] = dz[k+kgc]*dz[k+kgc] * p[ijk];
+            }
+
+    for (int j=0; j<jblock; ++j)
+        for (int i=0; i<iblock; ++i)
+        {
+            b[i + j*jj] += a[0];
+            const int ijk = i + j*jj + (kmax-1)*kk;
+            if (i == 0 && j == 0)
+                b[ijk] -= c[kmax-1];
+            else
+                b[ijk] += c[kmax-1];
+        }
+
+    double* work2d = (double*)malloc(sizeof(double)*iblock*jblock);
+    double* acopy = (double*)malloc(sizeof(double)*kmax);
+    double* ccopy = (double*)malloc(sizeof(double)*kmax);
+    memcpy(acopy, a, sizeof(double)*kmax);
+    memcpy(ccopy, c, sizeof(double)*kmax);
+    mo_tdma(acopy, b, ccopy, p, work2d, work3d, iblock, jblock, kmax);
+    free(work2d); free(acopy); free(ccopy);
+
+    mo_fft_backward(d, p, work3d);
+
+    jj = imax; kk = imax*jmax;
+    const int jjp = d->icells, kkp = d->ijcells;
+    for (int k=0; k<kmax; ++k)
+        for (int j=0; j<jmax; ++j)
+            for (int i=0; i<imax; ++i)
+                p[i+d->igc + (j+d->jgc)*jjp + (k+kgc)*kkp] = work3d[i + j*jj + k*kk];
+
+    for (int j=d->jstart; j<d->jend; ++j)
+        for (int i=d->istart; i<d->iend; ++i)
+        {
+            const int ijk = i + j*jjp + d->kstart*kkp;
+            p[ijk-kkp] = p[ijk];
+        }
+
+    mo_boundary_cyclic(d, p, MO_EDGE_BOTH);
+}
+
+void mo_pres_2_output(const mo_dims* d, double* ut, double* vt, double* wt,
+                      const double* p, const double* dzhi)
+{
+    /* src/pres_2.cxx:364-387 */
+    const int ii = 1, jj = d->icells, kk = d->ijcells;
+    const double dxi = 1./d->dx, dyi = 1./d->dy;
+    for (int k=d->kstart; k<d->kend; ++k)
+        for (int j=d->jstart; j<d->jend; ++j)
+            for (int i=d->istart; i<d->iend; ++i)
+            {
+                const int ijk = i + j*jj + k*kk;
+                ut[ijk] -= (p[ijk] - p[ijk-ii]) * dxi;
+                vt[ijk] -= (p[ijk] - p[ijk-jj]) * dyi;
+                wt[ijk] -= (p[ijk] - p[ijk-kk]) * dzhi[k];
+            }
+}
+
+double mo_pres_2_divergence(const mo_dims* d, const double* u, const double* v, const double* w,
+                            const double* dzi, const double* rhoref, const double* rhorefh)
+{
+    /* src/pres_2.cxx:389-422 */
+    const int ii = 1, jj = d->icells, kk = d->ijcells;
+    const double dxi = 1./d->dx, dyi = 1./d->dy;
+    double divmax = 0.;
+    for (int k=d->kstart; k<d->kend; ++k)
+        for (int j=d->jstart; j<d->jend; ++j)
+            for (int i=d->istart; i<d->iend; ++i)
+            {
+                const int ijk = i + j*jj + k*kk;
+                const double div = rhoref[k]*((u[ijk+ii]-u[ijk])*dxi + (v[ijk+jj]-v[ijk])*dyi)
+                                 + (rhorefh[k+1]*w[ijk+kk]-rhorefh[k]*w[ijk])*dzi[k];
+                const double a = fabs(div);
+                if (a > divmax) divmax = a;
+            }
+    return divmax;
+}

@@ -1,0 +1,251 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by the cpu_baseline leg of bench.py.  The product package never imports it.
+
+Arrays are numpy float64 in the reference layout (k, j, i) with ghost cells,
+i.e. ijk = i + j*icells + k*ijcells (include/grid.h:69-83).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+REFGEN_PATH = os.path.join(_HERE, "_ref", "mhh_refgen")
+
+DP = C.POINTER(C.c_double)
+
+
+class Dims(C.Structure):
+    _fields_ = [(n, C.c_int) for n in (
+        "itot", "jtot", "ktot", "imax", "jmax", "kmax", "igc", "jgc", "kgc",
+        "icells", "jcells", "kcells", "ijcells", "ncells",
+        "istart", "iend", "jstart", "jend", "kstart", "kend")] + \
+        [(n, C.c_double) for n in ("dx", "dy", "xsize", "ysize", "zsize")]
+
+
+class Case(C.Structure):
+    _fields_ = [
+        ("itot", C.c_int), ("jtot", C.c_int), ("ktot", C.c_int),
+        ("xsize", C.c_double), ("ysize", C.c_double), ("zsize", C.c_double),
+        ("advec", C.c_int), ("diff", C.c_int), ("thermo", C.c_int),
+        ("cflmax", C.c_double), ("dnmax", C.c_double), ("cs", C.c_double), ("tPr", C.c_double),
+        ("swmason", C.c_int),
+        ("visc", C.c_double), ("svisc", C.c_double), ("thref0", C.c_double),
+        ("mbcbot", C.c_int), ("mbctop", C.c_int), ("sbcbot", C.c_int), ("sbctop", C.c_int),
+        ("ubot", C.c_double), ("vbot", C.c_double), ("utop", C.c_double), ("vtop", C.c_double),
+        ("sbot", C.c_double), ("stop", C.c_double),
+        ("utrans", C.c_double), ("vtrans", C.c_double),
+        ("rndseed", C.c_int),
+        ("rndamp_u", C.c_double), ("rndamp_v", C.c_double), ("rndamp_w", C.c_double),
+        ("rndamp_th", C.c_double), ("rndz", C.c_double), ("rndexp", C.c_double),
+        ("adaptivestep", C.c_int),
+        ("dt", C.c_double), ("dtmax", C.c_double), ("endtime", C.c_double), ("savetime", C.c_double),
+        ("nthreads", C.c_int),
+    ]
+
+
+BC = {"dirichlet": 0, "neumann": 1, "flux": 2, "ustar": 3, "off": 4,
+      "noslip": 0, "freeslip": 1}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L):
+    P = C.POINTER(Dims)
+    i, d = C.c_int, C.c_double
+    sig = {
+        "mo_dims_init": (None, [P, i, i, i, i, i, i, d, d, d]),
+        "mo_grid_calculate": (None, [P, DP, DP, DP, DP, DP, DP]),
+        "mo_srand": (None, [C.c_uint]),
+        "mo_rand": (i, []),
+        "mo_randomize": (None, [P, DP, DP, d, d, d]),
+        "mo_add_mean_prof": (None, [P, DP, DP, d]),
+        "mo_boundary_cyclic": (None, [P, DP, i]),
+        "mo_set_bc": (None, [P, DP, DP, DP, i, d, d, d, i]),
+        "mo_ghost_cells_bot_2nd": (None, [P, DP, DP, i, DP, DP]),
+        "mo_ghost_cells_top_2nd": (None, [P, DP, DP, i, DP, DP]),
+        "mo_calc_strain2": (None, [P, i, DP, DP, DP, DP, DP, DP, DP, DP, DP]),
+        "mo_calc_N2": (None, [P, DP, DP, DP, DP]),
+        "mo_calc_evisc": (None, [P, i, i, DP, DP, DP, DP, DP, DP, d, d]),
+        "mo_calc_evisc_neutral": (None, [P, i, i, DP, DP, DP, DP, DP, DP, DP, d, d]),
+        "mo_diff_u": (None, [P, i, DP, DP, DP, DP, DP, DP, DP, DP, DP, DP, DP, d]),
+        "mo_diff_v": (None, [P, i, DP, DP, DP, DP, DP, DP, DP, DP, DP, DP, DP, d]),
+        "mo_diff_w": (None, [P, DP, DP, DP, DP, DP, DP, DP, DP, DP, d]),
+        "mo_diff_c": (None, [P, i, DP, DP, DP, DP, DP, DP, DP, DP, DP, d, d]),
+        "mo_calc_dnmul": (d, [P, DP, DP, d]),
+        "mo_advec_2i5_cfl": (d, [P, DP, DP, DP, DP, d]),
+        "mo_advec_2i5_u": (None, [P, DP, DP, DP, DP, DP, DP, DP]),
+        "mo_advec_2i5_v": (None, [P, DP, DP, DP, DP, DP, DP, DP]),
+        "mo_advec_2i5_w": (None, [P, DP, DP, DP, DP, DP, DP, DP]),
+        "mo_advec_2i5_s": (None, [P, DP, DP, DP, DP, DP, DP, DP, DP]),
+        "mo_advec_2_cfl": (d, [P, DP, DP, DP, DP, d]),
+        "mo_advec_2_u": (None, [P, DP, DP, DP, DP, DP, DP, DP]),
+        "mo_advec_2_v": (None, [P, DP, DP, DP, DP, DP, DP, DP]),
+        "mo_advec_2_w": (None, [P, DP, DP, DP, DP, DP, DP, DP]),
+        "mo_advec_2_s": (None, [P, DP, DP, DP, DP, DP, DP, DP, DP]),
+        "mo_diff_2_c": (None, [P, DP, DP, d, DP, DP]),
+        "mo_diff_2_w": (None, [P, DP, DP, d, DP, DP]),
+        "mo_buoyancy_tend_2nd": (None, [P, DP, DP, DP]),
+        "mo_pres_2_set_values": (None, [P, DP, DP, DP, DP, DP, DP, DP]),
+        "mo_pres_2_input": (None, [P, DP, DP, DP, DP, DP, DP, DP, DP, DP, DP, d]),
+        "mo_fft_r2hc": (None, [i, DP, DP]),
+        "mo_fft_hc2r": (None, [i, DP, DP]),
+        "mo_fft_forward": (None, [P, DP]),
+        "mo_fft_backward": (None, [P, DP, DP]),
+        "mo_tdma": (None, [DP, DP, DP, DP, DP, DP, i, i, i]),
+        "mo_pres_2_solve": (None, [P, DP, DP, DP, DP, DP, DP, DP, DP, DP]),
+        "mo_pres_2_output": (None, [P, DP, DP, DP, DP, DP]),
+        "mo_pres_2_divergence": (d, [P, DP, DP, DP, DP, DP, DP]),
+        "mo_rk3": (None, [P, DP, DP, i, d]),
+        "mo_rk3_subdt": (d, [d, i]),
+        "mo_convert_to_itime": (C.c_ulong, [d]),
+        "mo_model_create": (C.c_void_p, [C.POINTER(Case), DP, DP, DP, DP]),
+        "mo_model_destroy": (None, [C.c_void_p]),
+        "mo_model_field": (DP, [C.c_void_p, C.c_char_p]),
+        "mo_model_dims": (P, [C.c_void_p]),
+        "mo_model_substep": (i, [C.c_void_p]),
+        "mo_model_dt": (d, [C.c_void_p]),
+        "mo_model_time": (d, [C.c_void_p]),
+        "mo_model_substep_index": (i, [C.c_void_p]),
+        "mo_model_iteration": (i, [C.c_void_p]),
+        "mo_model_cfl": (d, [C.c_void_p]),
+        "mo_model_dn": (d, [C.c_void_p]),
+        "mo_model_div": (d, [C.c_void_p]),
+        "mo_model_set_capture": (None, [C.c_void_p, i]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def ptr(a):
+    """float64 C-contiguous numpy array -> double*; None -> NULL."""
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(DP)
+
+
+def dims(itot, jtot, ktot, igc=3, jgc=3, kgc=1, xsize=1.0, ysize=1.0, zsize=1.0):
+    d = Dims()
+    lib().mo_dims_init(C.byref(d), itot, jtot, ktot, igc, jgc, kgc, xsize, ysize, zsize)
+    return d
+
+
+def field(d, fill=0.0):
+    return np.full((d.kcells, d.jcells, d.icells), fill, dtype=np.float64)
+
+
+def plane(d, fill=0.0):
+    return np.full((d.jcells, d.icells), fill, dtype=np.float64)
+
+
+def prof(d, fill=0.0):
+    return np.full(d.kcells, fill, dtype=np.float64)
+
+
+def grid_arrays(d, z_interior):
+    """Grid::calculate (src/grid.cxx:244-305) -> dict of kcells-long arrays."""
+    z = prof(d)
+    z[d.kstart:d.kend] = z_interior
+    out = {k: prof(d) for k in ("zh", "dz", "dzh", "dzi", "dzhi")}
+    lib().mo_grid_calculate(C.byref(d), ptr(z), ptr(out["zh"]), ptr(out["dz"]),
+                            ptr(out["dzh"]), ptr(out["dzi"]), ptr(out["dzhi"]))
+    out["z"] = z
+    return out
+
+
+class Model:
+    """The oracle's dry-CBL driver (src/model.cxx:356-545 order)."""
+
+    def __init__(self, case, z_interior, u_prof, v_prof, th_prof):
+        L = lib()
+        self._z = np.ascontiguousarray(z_interior, dtype=np.float64)
+        self._u = np.ascontiguousarray(u_prof, dtype=np.float64)
+        self._v = np.ascontiguousarray(v_prof, dtype=np.float64)
+        self._t = np.ascontiguousarray(th_prof, dtype=np.float64)
+        self.case = case
+        self.h = L.mo_model_create(C.byref(case), ptr(self._z), ptr(self._u), ptr(self._v), ptr(self._t))
+        if not self.h:
+            raise RuntimeError("mo_model_create failed")
+        self.d = L.mo_model_dims(self.h).contents
+
+    def field(self, name):
+        d = self.d
+        L = lib()
+        p = L.mo_model_field(self.h, name.encode())
+        if not p:
+            raise KeyError(name)
+        shapes3 = {"u", "v", "w", "th", "ut", "vt", "wt", "tht", "p", "evisc", "tmp1", "tmp2",
+                   "ut_pre", "vt_pre", "wt_pre", "tht_pre"}
+        if name in shapes3:
+            shape = (d.kcells, d.jcells, d.icells)
+        elif name in ("bmati",):
+            shape = (d.itot,)
+        elif name in ("bmatj",):
+            shape = (d.jtot,)
+        elif name in ("pa", "pc"):
+            shape = (d.kmax,)
+        elif name.endswith(("bot", "top")):
+            shape = (d.jcells, d.icells)
+        else:
+            shape = (d.kcells,)
+        n = int(np.prod(shape))
+        return np.ctypeslib.as_array(p, shape=(n,)).reshape(shape)
+
+    def substep(self):
+        if lib().mo_model_substep(self.h) != 0:
+            raise RuntimeError("oracle substep failed")
+
+    def step(self):
+        for _ in range(3):
+            self.substep()
+
+    @property
+    def dt(self):
+        return lib().mo_model_dt(self.h)
+
+    @property
+    def time(self):
+        return lib().mo_model_time(self.h)
+
+    @property
+    def substep_index(self):
+        return lib().mo_model_substep_index(self.h)
+
+    def cfl(self):
+        return lib().mo_model_cfl(self.h)
+
+    def dn(self):
+        return lib().mo_model_dn(self.h)
+
+    def div(self):
+        return lib().mo_model_div(self.h)
+
+    def set_capture(self, on):
+        lib().mo_model_set_capture(self.h, int(on))
+
+    def close(self):
+        if self.h:
+            lib().mo_model_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,66 @@
+/*
+ * ref_advec_2.cxx -- wraps the reference's own src/advec_2.cxx kernels
+ * (anonymous-namespace templates, reached by including the translation unit).
+ * TEST INFRASTRUCTURE ONLY: builds oracle/_ref, which writes golden vectors.
+ */
+#include "advec_2.cxx"
+#include "ref_io.h"
+
+namespace
+{
+    void op_u(const Ref_run& r)
+    {
+        Ref_dims d(r); const size_t n = d.ncells, nk = d.kcells;
+        auto ut = r.load("ut", n); auto u = r.load("u", n); auto v = r.load("v", n); auto w = r.load("w", n);
+        auto dzi = r.load("dzi", nk); auto rho = r.load("rhoref", nk); auto rhoh = r.load("rhorefh", nk);
+        advec_u<double>(ut.data(), u.data(), v.data(), w.data(), dzi.data(), d.dx, d.dy, rho.data(), rhoh.data(),
+                        d.istart, d.iend, d.jstart, d.jend, d.kstart, d.kend, d.icells, d.ijcells);
+        r.save("ut", ut);
+    }
+    void op_v(const Ref_run& r)
+    {
+        Ref_dims d(r); const size_t n = d.ncells, nk = d.kcells;
+        auto vt = r.load("vt", n); auto u = r.load("u", n); auto v = r.load("v", n); auto w = r.load("w", n);
+        auto dzi = r.load("dzi", nk); auto rho = r.load("rhoref", nk); auto rhoh = r.load("rhorefh", nk);
+        advec_v<double>(vt.data(), u.data(), v.data(), w.data(), dzi.data(), d.dx, d.dy, rho.data(), rhoh.data(),
+                        d.istart, d.iend, d.jstart, d.jend, d.kstart, d.kend, d.icells, d.ijcells);
+        r.save("vt", vt);
+    }
+    void op_w(const Ref_run& r)
+    {
+        Ref_dims d(r); const size_t n = d.ncells, nk = d.kcells;
+        auto wt = r.load("wt", n); auto u = r.load("u", n); auto v = r.load("v", n); auto w = r.load("w", n);
+        auto dzhi = r.load("dzhi", nk); auto rho = r.load("rhoref", nk); auto rhoh = r.load("rhorefh", nk);
+        advec_w<double>(wt.data(), u.data(), v.data(), w.data(), dzhi.data(), d.dx, d.dy, rho.data(), rhoh.data(),
+                        d.istart, d.iend, d.jstart, d.jend, d.kstart, d.kend, d.icells, d.ijcells);
+        r.save("wt", wt);
+    }
+    void op_s(const Ref_run& r)
+    {
+        Ref_dims d(r); const size_t n = d.ncells, nk = d.kcells;
+        auto st = r.load("st", n); auto s = r.load("s", n);
+        auto u = r.load("u", n); auto v = r.load("v", n); auto w = r.load("w", n);
+        auto dzi = r.load("dzi", nk); auto rho = r.load("rhoref", nk); auto rhoh = r.load("rhorefh", nk);
+        advec_s<double>(st.data(), s.data(), u.data(), v.data(), w.data(), dzi.data(), d.dx, d.dy, rho.data(), rhoh.data(),
+                        d.istart, d.iend, d.jstart, d.jend, d.kstart, d.kend, d.icells, d.ijcells);
+        r.save("st", st);
+    }
+    void op_cfl(const Ref_run& r)
+    {
+        Ref_dims d(r); const size_t n = d.ncells, nk = d.kcells;
+        auto u = r.load("u", n); auto v = r.load("v", n); auto w = r.load("w", n); auto dzi = r.load("dzi", nk);
+        Master master;
+        const double cfl = calc_cfl<double>(u.data(), v.data(), w.data(), dzi.data(), d.dx, d.dy, r.p("dt"), master,
+                                            d.istart, d.iend, d.jstart, d.jend, d.kstart, d.kend, d.icells, d.ijcells);
+        r.save("cfl", std::vector<double>(1, cfl));
+    }
+}
+
+void ref_register_advec_2()
+{
+    ref_register("advec_2_u", op_u);
+    ref_register("advec_2_v", op_v);
+    ref_register("advec_2_w", op_w);
+    ref_register("advec_2_s", op_s);
+    ref_register("advec_2_cfl", op_cfl);
+}

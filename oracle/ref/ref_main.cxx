@@ -1,0 +1,92 @@
+/*
+ * ref_main.cxx -- dispatcher of the golden-vector generator (oracle/_ref).
+ * TEST INFRASTRUCTURE ONLY; see ref_io.h.
+ */
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <sstream>
+#include <stdexcept>
+#include "ref_io.h"
+
+static std::map<std::string, ref_op>& registry()
+{
+    static std::map<std::string, ref_op> r;
+    return r;
+}
+
+void ref_register(const char* name, ref_op op) { registry()[name] = op; }
+
+double Ref_run::p(const std::string& k) const
+{
+    auto it = par.find(k);
+    if (it == par.end())
+        throw std::runtime_error("missing parameter " + k);
+    return it->second;
+}
+
+std::vector<double> Ref_run::load(const std::string& name, size_t n) const
+{
+    std::vector<double> a(n);
+    std::ifstream f(dir + "/" + name + ".bin", std::ios::binary);
+    if (!f)
+        throw std::runtime_error("missing array " + name);
+    f.read(reinterpret_cast<char*>(a.data()), static_cast<std::streamsize>(n*sizeof(double)));
+    if (static_cast<size_t>(f.gcount()) != n*sizeof(double))
+        throw std::runtime_error("short array " + name);
+    return a;
+}
+
+void Ref_run::save(const std::string& name, const std::vector<double>& a) const
+{
+    std::ofstream f(dir + "/out_" + name + ".bin", std::ios::binary);
+    f.write(reinterpret_cast<const char*>(a.data()), static_cast<std::streamsize>(a.size()*sizeof(double)));
+}
+
+Ref_dims::Ref_dims(const Ref_run& r)
+{
+    itot = r.pi("itot"); jtot = r.pi("jtot"); ktot = r.pi("ktot");
+    const int igc = r.pi("igc"), jgc = r.pi("jgc"), kgc = r.pi("kgc");
+    icells = itot + 2*igc; jcells = jtot + 2*jgc; kcells = ktot + 2*kgc;
+    ijcells = icells*jcells; ncells = ijcells*kcells;
+    istart = igc; iend = itot + igc; jstart = jgc; jend = jtot + jgc; kstart = kgc; kend = ktot + kgc;
+    dx = r.p("dx"); dy = r.p("dy");
+}
+
+void ref_register_advec_2i5();
+void ref_register_advec_2();
+void ref_register_diff();
+void ref_register_diff_2();
+void ref_register_timeloop();
+
+int main(int argc, char** argv)
+{
+    if (argc != 3)
+    {
+        std::fprintf(stderr, "usage: %s <op> <rundir>\n", argv[0]);
+        return 2;
+    }
+    ref_register_advec_2i5();
+    ref_register_advec_2();
+    ref_register_diff();
+    ref_register_diff_2();
+    ref_register_timeloop();
+    try
+    {
+        Ref_run r;
+        r.dir = argv[2];
+        std::ifstream f(r.dir + "/params.txt");
+        std::string k; double v;
+        while (f >> k >> v) r.par[k] = v;
+        auto it = registry().find(argv[1]);
+        if (it == registry().end())
+            throw std::runtime_error(std::string("unknown op ") + argv[1]);
+        it->second(r);
+    }
+    catch (std::exception& e)
+    {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

@@ -1,0 +1,161 @@
+"""Write the golden vectors of the hot-path operators.
+
+Runs the reference's own kernels -- oracle/_ref/mhh_refgen, compiled by
+`make -C oracle ref` from the reference translation units where they lie --
+on seeded random inputs and stores inputs and outputs as compressed .npz
+fixtures next to this script.  Run it only where the reference tree exists:
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+Fixtures (one per grid): ref_ops_<grid>.npz with
+  params      : itot jtot ktot igc jgc kgc dx dy visc tPr dt substep
+  in_<name>   : input arrays (3-D fields shaped (kcells, jcells, icells))
+  out_<op>    : the reference output of operator <op>
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REFGEN = os.path.join(ROOT, "oracle", "_ref", "mhh_refgen")
+
+GRIDS = {
+    # name: itot, jtot, ktot, stretched z, variable density, dx, dy, seed
+    "g1": dict(itot=8, jtot=6, ktot=8, stretch=True, anelastic=True, dx=50.0, dy=40.0, seed=11),
+    "g2": dict(itot=5, jtot=7, ktot=7, stretch=False, anelastic=False, dx=25.0, dy=25.0, seed=12),
+}
+
+OPS = [
+    # op name, output array, inputs
+    ("advec_2i5_u", "ut", ["ut", "u", "v", "w", "dzi", "rhoref", "rhorefh"]),
+    ("advec_2i5_v", "vt", ["vt", "u", "v", "w", "dzi", "rhoref", "rhorefh"]),
+    ("advec_2i5_w", "wt", ["wt", "u", "v", "w", "dzhi", "rhoref", "rhorefh"]),
+    ("advec_2i5_s", "st", ["st", "s", "u", "v", "w", "dzi", "rhoref", "rhorefh"]),
+    ("advec_2i5_cfl", "cfl", ["u", "v", "w", "dzi"]),
+    ("advec_2_u", "ut", ["ut", "u", "v", "w", "dzi", "rhoref", "rhorefh"]),
+    ("advec_2_v", "vt", ["vt", "u", "v", "w", "dzi", "rhoref", "rhorefh"]),
+    ("advec_2_w", "wt", ["wt", "u", "v", "w", "dzhi", "rhoref", "rhorefh"]),
+    ("advec_2_s", "st", ["st", "s", "u", "v", "w", "dzi", "rhoref", "rhorefh"]),
+    ("advec_2_cfl", "cfl", ["u", "v", "w", "dzi"]),
+    ("strain2_nosfc", "strain2", ["strain2", "u", "v", "w", "ugradbot", "vgradbot", "z", "dzi", "dzhi"]),
+    ("strain2_sfc", "strain2", ["strain2", "u", "v", "w", "ugradbot", "vgradbot", "z", "dzi", "dzhi"]),
+    ("diff_u_nosfc", "at", ["at", "u", "v", "w", "evisc", "fluxbot", "fluxtop", "dzi", "dzhi", "rhoref", "rhorefh"]),
+    ("diff_u_sfc", "at", ["at", "u", "v", "w", "evisc", "fluxbot", "fluxtop", "dzi", "dzhi", "rhoref", "rhorefh"]),
+    ("diff_v_nosfc", "at", ["at", "u", "v", "w", "evisc", "fluxbot", "fluxtop", "dzi", "dzhi", "rhoref", "rhorefh"]),
+    ("diff_v_sfc", "at", ["at", "u", "v", "w", "evisc", "fluxbot", "fluxtop", "dzi", "dzhi", "rhoref", "rhorefh"]),
+    ("diff_w", "at", ["at", "u", "v", "w", "evisc", "dzi", "dzhi", "rhoref", "rhorefh"]),
+    ("diff_c_nosfc", "at", ["at", "a", "evisc", "fluxbot", "fluxtop", "dzi", "dzhi", "rhoref", "rhorefh"]),
+    ("diff_c_sfc", "at", ["at", "a", "evisc", "fluxbot", "fluxtop", "dzi", "dzhi", "rhoref", "rhorefh"]),
+    ("dnmul", "dnmul", ["evisc", "dzi"]),
+    ("diff_2_c", "at", ["at", "a", "dzi", "dzhi"]),
+    ("diff_2_w", "at", ["at", "a", "dzi", "dzhi"]),
+    ("rk3", "a", ["a", "at"]),
+    ("rk3", "at", ["a", "at"]),
+]
+
+
+def make_inputs(g):
+    rng = np.random.default_rng(g["seed"])
+    itot, jtot, ktot = g["itot"], g["jtot"], g["ktot"]
+    igc = jgc = 3
+    kgc = 1
+    ic, jc, kc = itot + 2 * igc, jtot + 2 * jgc, ktot + 2 * kgc
+    shape3, shape2 = (kc, jc, ic), (jc, ic)
+
+    # Vertical grid: 2nd-order ghost levels of src/grid.cxx:275-305, written here in
+    # numpy from the level definitions so that the fixtures do not depend on the oracle.
+    zsize = 1000.0
+    if g["stretch"]:
+        dzs = 60.0 * 1.15 ** np.arange(ktot)
+        zh_int = np.concatenate([[0.0], np.cumsum(dzs)])
+        zsize = zh_int[-1]
+        zc = 0.5 * (zh_int[1:] + zh_int[:-1])
+    else:
+        dz0 = zsize / ktot
+        zc = dz0 * (np.arange(ktot) + 0.5)
+    z = np.zeros(kc); zh = np.zeros(kc); dz = np.zeros(kc); dzh = np.zeros(kc)
+    z[kgc:kgc + ktot] = zc
+    z[kgc - 1] = -z[kgc]
+    z[kgc + ktot] = 2 * zsize - z[kgc + ktot - 1]
+    for k in range(kgc + 1, kgc + ktot):
+        zh[k] = 0.5 * (z[k - 1] + z[k])
+    zh[kgc] = 0.0
+    zh[kgc + ktot] = zsize
+    dzh[1:] = z[1:] - z[:-1]
+    dzh[kgc - 1] = dzh[kgc + 1]
+    dz[1:kc - 1] = zh[2:] - zh[1:kc - 1]
+    dz[kgc - 1] = dz[kgc]
+    dz[kgc + ktot] = dz[kgc + ktot - 1]
+    dzi = 1.0 / dz
+    dzhi = np.zeros(kc); dzhi[1:] = 1.0 / dzh[1:]; dzhi[kgc - 1] = dzhi[kgc + 1]
+
+    if g["anelastic"]:
+        rhoref = 1.2 * np.exp(-z / 8000.0)
+        rhorefh = 1.2 * np.exp(-np.maximum(zh, 0.0) / 8000.0)
+    else:
+        rhoref = np.ones(kc)
+        rhorefh = np.ones(kc)
+
+    f = lambda scale=1.0, off=0.0: off + scale * rng.uniform(-1.0, 1.0, shape3)
+    inp = dict(
+        u=f(2.0, 3.0), v=f(2.0, -1.0), w=f(0.5), s=f(1.0, 300.0), a=f(1.0, 300.0),
+        ut=f(1e-2), vt=f(1e-2), wt=f(1e-2), st=f(1e-3), at=f(1e-2),
+        strain2=f(1.0), evisc=np.abs(f(1.0)) + 0.1,
+        ugradbot=rng.uniform(-0.1, 0.1, shape2), vgradbot=rng.uniform(-0.1, 0.1, shape2),
+        fluxbot=rng.uniform(-0.1, 0.1, shape2), fluxtop=rng.uniform(-0.1, 0.1, shape2),
+        z=z, zh=zh, dz=dz, dzh=dzh, dzi=dzi, dzhi=dzhi, rhoref=rhoref, rhorefh=rhorefh,
+    )
+    params = dict(itot=itot, jtot=jtot, ktot=ktot, igc=igc, jgc=jgc, kgc=kgc,
+                  dx=g["dx"], dy=g["dy"], visc=1.5e-5, tPr=1.0 / 3.0, dt=0.75, substep=1)
+    return params, inp
+
+
+def run_op(op, params, inp, names):
+    with tempfile.TemporaryDirectory() as tmp:
+        with open(os.path.join(tmp, "params.txt"), "w") as fh:
+            for k, v in params.items():
+                fh.write(f"{k} {float(v)!r}\n")
+        for n in names:
+            np.ascontiguousarray(inp[n], dtype=np.float64).tofile(os.path.join(tmp, n + ".bin"))
+        subprocess.run([REFGEN, op, tmp], check=True)
+        outs = {}
+        for fn in os.listdir(tmp):
+            if fn.startswith("out_"):
+                outs[fn[4:-4]] = np.fromfile(os.path.join(tmp, fn), dtype=np.float64)
+        return outs
+
+
+def main():
+    if not os.path.exists(REFGEN):
+        sys.exit("oracle/_ref/mhh_refgen missing: run `make -C oracle ref` where the reference exists")
+    for gname, g in GRIDS.items():
+        params, inp = make_inputs(g)
+        store = {"params_" + k: np.array(v) for k, v in params.items()}
+        store.update({"in_" + k: v for k, v in inp.items()})
+        done = set()
+        for op, outname, names in OPS:
+            if op in done:
+                continue
+            done.add(op)
+            # rk3 is recorded for every substep
+            substeps = [0, 1, 2] if op == "rk3" else [None]
+            for sst in substeps:
+                p = dict(params)
+                if sst is not None:
+                    p["substep"] = sst
+                outs = run_op(op, p, inp, names)
+                for k, v in outs.items():
+                    key = f"out_{op}_{k}" if sst is None else f"out_{op}{sst}_{k}"
+                    shaped = v if v.size == 1 else v.reshape(inp[k].shape if k in inp else (-1,))
+                    store[key] = shaped
+        path = os.path.join(HERE, f"ref_ops_{gname}.npz")
+        np.savez_compressed(path, **store)
+        print("wrote", path, os.path.getsize(path), "bytes")
+
+
+if __name__ == "__main__":
+    main()
