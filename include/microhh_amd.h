@@ -1,0 +1,183 @@
+/*
+ * microhh_amd.h -- C ABI of the MI355X (gfx950) dynamical core.
+ *
+ * The reference has no C ABI: its boundary is the C++ operator classes
+ * Advec<TF>, Diff<TF>, Pres<TF>, Timeloop<TF> acting on Fields/Field3d
+ * (include/advec.h:45-70, include/diff.h:37-71, include/pres.h:46-58,
+ * include/timeloop.h:48-139).  Each entry point below replaces one of those
+ * methods and says which; the C++ host classes in microhh_amd/host mirror the
+ * reference classes on top of it.
+ *
+ * Conventions
+ *   - All functions return 0 on success and a non-zero MHH_E* code on error;
+ *     mhh_last_error() gives the message (thread-local).  No exception crosses
+ *     the ABI.
+ *   - 3-D arrays are device pointers in the reference layout
+ *     ijk = i + j*icells + k*icells*jcells, ghost cells included
+ *     (include/grid.h:69-83); 2-D arrays are icells*jcells planes.
+ *   - The element type is double (MHH_FP64) or float (MHH_FP32), fixed per
+ *     context.  The caller owns every field; the library borrows the pointers
+ *     bound with mhh_fields_bind() and owns only small tables and scratch.
+ *   - Work is enqueued on the context's HIP stream; functions that return a
+ *     host scalar synchronise that stream.
+ */
+#ifndef MICROHH_AMD_H
+#define MICROHH_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MHH_MAX_SCALARS 8
+
+enum { MHH_OK = 0, MHH_EINVAL = 1, MHH_EHIP = 2, MHH_ESTATE = 3, MHH_ENOMEM = 4 };
+enum { MHH_FP64 = 0, MHH_FP32 = 1 };
+/* Boundary types: include/boundary.h:51 (Boundary_type). */
+enum { MHH_BC_DIRICHLET = 0, MHH_BC_NEUMANN = 1, MHH_BC_FLUX = 2, MHH_BC_USTAR = 3, MHH_BC_OFF = 4 };
+/* Halo edges: include/boundary_cyclic.h (Edge). */
+enum { MHH_EDGE_EAST_WEST = 0, MHH_EDGE_NORTH_SOUTH = 1, MHH_EDGE_BOTH = 2 };
+
+typedef struct mhh_ctx mhh_ctx;
+
+/* Grid_data<TF> (include/grid.h:49-138) for one GPU's share.  The 1-D arrays
+ * are host arrays of kcells = ktot + 2*kgc doubles, as computed by
+ * Grid::calculate (src/grid.cxx:244-305) and Thermo_dry::create_basestate
+ * (src/thermo_dry.cxx:504-585); they are copied at creation. */
+typedef struct
+{
+    int itot, jtot, ktot;
+    int igc, jgc, kgc;
+    double dx, dy, zsize;
+    const double* z;
+    const double* zh;
+    const double* dz;
+    const double* dzh;
+    const double* dzi;
+    const double* dzhi;
+    const double* rhoref;
+    const double* rhorefh;
+} mhh_grid_desc;
+
+/* Fields<TF> maps mp/mt/sp/st/sd (include/fields.h:134-145) as device pointers. */
+typedef struct
+{
+    void* u;  void* v;  void* w;
+    void* ut; void* vt; void* wt;
+    int nscalars;
+    void* s[MHH_MAX_SCALARS];
+    void* st[MHH_MAX_SCALARS];
+    void* p;       /* sd["p"] */
+    void* evisc;   /* sd["evisc"] */
+    void* tmp1;    /* two ncells work fields (Fields::get_tmp, src/fields.cxx:631-663) */
+    void* tmp2;
+    /* Field3d 2-D boundary planes (include/field3d.h:36-83). */
+    void* u_bot; void* u_top; void* u_gradbot; void* u_gradtop; void* u_fluxbot; void* u_fluxtop;
+    void* v_bot; void* v_top; void* v_gradbot; void* v_gradtop; void* v_fluxbot; void* v_fluxtop;
+    void* s_bot[MHH_MAX_SCALARS];     void* s_top[MHH_MAX_SCALARS];
+    void* s_gradbot[MHH_MAX_SCALARS]; void* s_gradtop[MHH_MAX_SCALARS];
+    void* s_fluxbot[MHH_MAX_SCALARS]; void* s_fluxtop[MHH_MAX_SCALARS];
+    /* Surface-model inputs (Boundary::get_dudz/get_dvdz/get_dbdz/get_z0m); only
+     * read when a Surface_model::Enabled variant runs. */
+    void* dudz; void* dvdz; void* dbdz; void* z0m;
+} mhh_field_set;
+
+/* Boundary switches of Boundary::set_values (src/boundary.cxx:599-696). */
+typedef struct
+{
+    int mbcbot, mbctop;
+    int sbcbot[MHH_MAX_SCALARS], sbctop[MHH_MAX_SCALARS];
+} mhh_bc_desc;
+
+/* Diff_smag2 settings (src/diff_smag2.cxx:272-292). */
+typedef struct
+{
+    double cs, tPr;
+    double visc;                    /* fields.visc */
+    double svisc[MHH_MAX_SCALARS];  /* fields.sp[...]->visc */
+    int swmason;
+    int surface_model;              /* 0: Surface_model::Disabled ("default"), 1: Enabled */
+    int thermo;                     /* 1: N2 from scalar 0 (Thermo_dry), 0: neutral */
+} mhh_smag_params;
+
+/* ---- context (Model::prepare_device, src/model.cxx:584-604) ---- */
+int  mhh_ctx_create(const mhh_grid_desc* grid, int precision, void* hip_stream, mhh_ctx** out);
+int  mhh_ctx_destroy(mhh_ctx* ctx);
+int  mhh_fields_bind(mhh_ctx* ctx, const mhh_field_set* fields);
+int  mhh_set_base_state(mhh_ctx* ctx, const double* thref, const double* threfh);
+int  mhh_sync(mhh_ctx* ctx);
+const char* mhh_last_error(void);
+/* Bytes per element of the context (8 or 4). */
+int  mhh_element_size(const mhh_ctx* ctx);
+
+/* ---- boundary ---- */
+/* Boundary_cyclic::exec (src/boundary_cyclic.cxx:369-443) on one 3-D field. */
+int  mhh_boundary_cyclic(mhh_ctx* ctx, void* field, int edge);
+/* Boundary::set_prognostic_cyclic_bcs (src/boundary.cxx:449-464). */
+int  mhh_boundary_set_prognostic_cyclic_bcs(mhh_ctx* ctx);
+/* Boundary::set_ghost_cells, 2nd order (src/boundary.cxx:932-997, 700-775). */
+int  mhh_boundary_set_ghost_cells(mhh_ctx* ctx, const mhh_bc_desc* bc);
+
+/* ---- advection ---- */
+/* Advec_2i5::exec (src/advec_2i5.cxx:995-1063). */
+int  mhh_advec_2i5_exec(mhh_ctx* ctx);
+/* Advec_2i5::get_cfl (src/advec_2i5.cxx:954-971): max CFL times dt. */
+int  mhh_advec_2i5_cfl(mhh_ctx* ctx, double dt, double* cfl);
+/* Advec_2::exec / get_cfl (src/advec_2.cxx:312-350, 280-296). */
+int  mhh_advec_2_exec(mhh_ctx* ctx);
+int  mhh_advec_2_cfl(mhh_ctx* ctx, double dt, double* cfl);
+
+/* ---- diffusion ---- */
+/* Diff_smag2::exec_viscosity (src/diff_smag2.cxx:473-607): strain2, N2, evisc,
+ * wall mirror, cyclic. */
+int  mhh_diff_smag2_exec_viscosity(mhh_ctx* ctx, const mhh_smag_params* par);
+/* Diff_smag2::exec (src/diff_smag2.cxx:380-471). */
+int  mhh_diff_smag2_exec(mhh_ctx* ctx, const mhh_smag_params* par);
+/* Diff_smag2::get_dn without the dt factor: calc_dnmul (include/diff_kernels.h:486-511). */
+int  mhh_diff_smag2_dnmul(mhh_ctx* ctx, double tPr, double* dnmul);
+/* Diff_2::exec (src/diff_2.cxx:165-190). */
+int  mhh_diff_2_exec(mhh_ctx* ctx, double visc, const double* svisc);
+
+/* ---- thermodynamics ---- */
+/* Thermo_dry::exec, 2nd order (src/thermo_dry.cxx:609-643): buoyancy of
+ * scalar 0 on wt.  Needs mhh_set_base_state(). */
+int  mhh_thermo_dry_exec(mhh_ctx* ctx);
+
+/* ---- pressure ---- */
+/* Pres_2::exec (src/pres_2.cxx:64-94): input, FFT, tridiagonal, inverse FFT,
+ * output.  dt is the RK sub-time step.  Uses p, tmp1 and tmp2 as work. */
+int  mhh_pres_2_exec(mhh_ctx* ctx, double dt);
+/* Pres_2::check_divergence (src/pres_2.cxx:389-422). */
+int  mhh_pres_2_divergence(mhh_ctx* ctx, double* divmax);
+
+/* ---- time integration ---- */
+/* Timeloop::exec, rkorder 3 (src/timeloop.cxx:358-412, 251-286) on u, v, w
+ * and every bound scalar. */
+int  mhh_timeloop_rk3(mhh_ctx* ctx, int substep, double dt);
+
+/* ---- device buffers ----
+ * RAII device vectors of the reference (cuda_vector, include/cuda_buffer.h)
+ * and the host<->device field copies of Fields::prepare_device /
+ * forward_device / backward_device (src/fields.cu:323-601).  Allocations are
+ * zero-filled.  Copies are synchronous. */
+int  mhh_device_alloc(size_t bytes, void** ptr);
+int  mhh_device_free(void* ptr);
+int  mhh_copy_to_device(void* dst, const void* src, size_t bytes);
+int  mhh_copy_to_host(void* dst, const void* src, size_t bytes);
+int  mhh_device_synchronize(void);
+
+/* ---- kernel timing ----
+ * Per-kernel HIP-event timing on the context's stream, the measurement the
+ * reference's launcher records in its wisdom files (src/cuda_launcher.cu:276-309).
+ * While enabled, every launch of the named kernel families is bracketed by
+ * events; mhh_timing_query returns the summed milliseconds and the launch count
+ * since the last reset (it synchronises the stream). */
+int  mhh_timing_enable(mhh_ctx* ctx, int on);
+int  mhh_timing_reset(mhh_ctx* ctx);
+int  mhh_timing_query(mhh_ctx* ctx, const char* kernel, double* total_ms, int* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

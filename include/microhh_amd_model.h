@@ -1,0 +1,74 @@
+/*
+ * microhh_amd_model.h -- C ABI of the model driver built on the operator ABI.
+ *
+ * The driver is the C++ host layer in microhh_amd/host (classes Grid, Fields,
+ * Boundary, Advec_2i5/Advec_2, Diff_smag2/Diff_2, Thermo_dry, Pres_2,
+ * Timeloop, Model mirroring the reference classes).  These entry points let a
+ * non-C++ caller create a case, advance it and read fields; they replace
+ * Model<TF>::init/load/exec (src/model.cxx:108-581) for the in-scope
+ * operators.  Same conventions as microhh_amd.h: int status, 0 = OK,
+ * mhh_last_error() for the message.
+ */
+#ifndef MICROHH_AMD_MODEL_H
+#define MICROHH_AMD_MODEL_H
+
+#include "microhh_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mhh_model mhh_model;
+
+/* The .ini switches of the in-scope path (src/advec.cxx:54-96,
+ * src/diff.cxx:55-95, src/boundary.cxx:599-696, src/fields.cxx:1035-1079,
+ * src/timeloop.cxx:40-130), boundary switch "default". */
+typedef struct
+{
+    int itot, jtot, ktot;
+    double xsize, ysize, zsize;
+    int advec;          /* 0: "2i5", 1: "2" */
+    int diff;           /* 0: "smag2", 1: "2" */
+    int thermo;         /* 0: "dry" (Boussinesq, scalar th), 1: "0" (no scalar) */
+    double cflmax, dnmax, cs, tPr;
+    int swmason;
+    double visc, svisc;
+    double thref0;
+    int mbcbot, mbctop, sbcbot, sbctop;
+    double ubot, vbot, utop, vtop, sbot, stop;
+    double utrans, vtrans;
+    int rndseed;
+    double rndamp_u, rndamp_v, rndamp_w, rndamp_th;
+    double rndz, rndexp;
+    int adaptivestep;
+    double dt, dtmax, endtime, savetime;
+    int precision;      /* MHH_FP64 or MHH_FP32 */
+} mhh_case;
+
+/* z_interior, u_prof, v_prof, th_prof: ktot host values each (the "init"
+ * profiles of the case input, src/fields.cxx:1103-1134). */
+int mhh_model_create(const mhh_case* c, const double* z_interior, const double* u_prof,
+                     const double* v_prof, const double* th_prof, void* hip_stream, mhh_model** out);
+int mhh_model_destroy(mhh_model* m);
+/* One RK3 substep in the reference order (src/model.cxx:356-545). */
+int mhh_model_substep(mhh_model* m);
+/* n full time steps (3 substeps each). */
+int mhh_model_steps(mhh_model* m, int n);
+/* Device pointer of a named field ("u","v","w","th","ut","vt","wt","tht","p","evisc"). */
+int mhh_model_field_ptr(mhh_model* m, const char* name, void** ptr);
+/* Copy a named 3-D field (ghost cells included) to host float64 storage of ncells values. */
+int mhh_model_get_field(mhh_model* m, const char* name, double* host);
+int mhh_model_set_field(mhh_model* m, const char* name, const double* host);
+/* Sizes: icells, jcells, kcells, istart, jstart, kstart (6 ints). */
+int mhh_model_dims(const mhh_model* m, int* dims6);
+int mhh_model_time(const mhh_model* m, double* time, double* dt, int* substep, int* iteration);
+int mhh_model_cfl(mhh_model* m, double* cfl);
+int mhh_model_dn(mhh_model* m, double* dn);
+int mhh_model_div(mhh_model* m, double* div);
+/* The operator context the model runs on (for callers that add operators). */
+int mhh_model_ctx(mhh_model* m, mhh_ctx** ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,159 @@
+"""ctypes declarations of the C ABI in include/microhh_amd.h.
+
+This is plumbing for Python callers (tests, bench): it loads the in-tree
+libmicrohh_amd.so and fails loudly when it is missing -- there is no CPU
+fallback.  Device memory comes from the library's own allocation entry
+points, so that one HIP runtime (the one the library links) serves the process.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmicrohh_amd.so")
+
+MHH_MAX_SCALARS = 8
+MHH_FP64, MHH_FP32 = 0, 1
+BC = {"dirichlet": 0, "neumann": 1, "flux": 2, "ustar": 3, "off": 4, "noslip": 0, "freeslip": 1}
+EDGE_EW, EDGE_NS, EDGE_BOTH = 0, 1, 2
+
+DP = C.POINTER(C.c_double)
+VP = C.c_void_p
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("itot", C.c_int), ("jtot", C.c_int), ("ktot", C.c_int),
+                ("igc", C.c_int), ("jgc", C.c_int), ("kgc", C.c_int),
+                ("dx", C.c_double), ("dy", C.c_double), ("zsize", C.c_double)] + \
+               [(n, DP) for n in ("z", "zh", "dz", "dzh", "dzi", "dzhi", "rhoref", "rhorefh")]
+
+
+_S = MHH_MAX_SCALARS
+
+
+class FieldSet(C.Structure):
+    _fields_ = [("u", VP), ("v", VP), ("w", VP), ("ut", VP), ("vt", VP), ("wt", VP),
+                ("nscalars", C.c_int), ("s", VP * _S), ("st", VP * _S),
+                ("p", VP), ("evisc", VP), ("tmp1", VP), ("tmp2", VP)] + \
+               [(n, VP) for n in ("u_bot", "u_top", "u_gradbot", "u_gradtop", "u_fluxbot", "u_fluxtop",
+                                  "v_bot", "v_top", "v_gradbot", "v_gradtop", "v_fluxbot", "v_fluxtop")] + \
+               [(n, VP * _S) for n in ("s_bot", "s_top", "s_gradbot", "s_gradtop", "s_fluxbot", "s_fluxtop")] + \
+               [(n, VP) for n in ("dudz", "dvdz", "dbdz", "z0m")]
+
+
+class BcDesc(C.Structure):
+    _fields_ = [("mbcbot", C.c_int), ("mbctop", C.c_int),
+                ("sbcbot", C.c_int * _S), ("sbctop", C.c_int * _S)]
+
+
+class SmagParams(C.Structure):
+    _fields_ = [("cs", C.c_double), ("tPr", C.c_double), ("visc", C.c_double),
+                ("svisc", C.c_double * _S), ("swmason", C.c_int),
+                ("surface_model", C.c_int), ("thermo", C.c_int)]
+
+
+# name: (restype, argtypes) -- every symbol declared in include/microhh_amd.h
+SIGNATURES = {
+    "mhh_ctx_create": (C.c_int, [C.POINTER(GridDesc), C.c_int, VP, C.POINTER(VP)]),
+    "mhh_ctx_destroy": (C.c_int, [VP]),
+    "mhh_fields_bind": (C.c_int, [VP, C.POINTER(FieldSet)]),
+    "mhh_set_base_state": (C.c_int, [VP, DP, DP]),
+    "mhh_sync": (C.c_int, [VP]),
+    "mhh_last_error": (C.c_char_p, []),
+    "mhh_element_size": (C.c_int, [VP]),
+    "mhh_boundary_cyclic": (C.c_int, [VP, VP, C.c_int]),
+    "mhh_boundary_set_prognostic_cyclic_bcs": (C.c_int, [VP]),
+    "mhh_boundary_set_ghost_cells": (C.c_int, [VP, C.POINTER(BcDesc)]),
+    "mhh_advec_2i5_exec": (C.c_int, [VP]),
+    "mhh_advec_2i5_cfl": (C.c_int, [VP, C.c_double, DP]),
+    "mhh_advec_2_exec": (C.c_int, [VP]),
+    "mhh_advec_2_cfl": (C.c_int, [VP, C.c_double, DP]),
+    "mhh_diff_smag2_exec_viscosity": (C.c_int, [VP, C.POINTER(SmagParams)]),
+    "mhh_diff_smag2_exec": (C.c_int, [VP, C.POINTER(SmagParams)]),
+    "mhh_diff_smag2_dnmul": (C.c_int, [VP, C.c_double, DP]),
+    "mhh_diff_2_exec": (C.c_int, [VP, C.c_double, DP]),
+    "mhh_thermo_dry_exec": (C.c_int, [VP]),
+    "mhh_pres_2_exec": (C.c_int, [VP, C.c_double]),
+    "mhh_pres_2_divergence": (C.c_int, [VP, DP]),
+    "mhh_timeloop_rk3": (C.c_int, [VP, C.c_int, C.c_double]),
+    "mhh_device_alloc": (C.c_int, [C.c_size_t, C.POINTER(VP)]),
+    "mhh_device_free": (C.c_int, [VP]),
+    "mhh_copy_to_device": (C.c_int, [VP, VP, C.c_size_t]),
+    "mhh_copy_to_host": (C.c_int, [VP, VP, C.c_size_t]),
+    "mhh_device_synchronize": (C.c_int, []),
+    "mhh_timing_enable": (C.c_int, [VP, C.c_int]),
+    "mhh_timing_reset": (C.c_int, [VP]),
+    "mhh_timing_query": (C.c_int, [VP, C.c_char_p, DP, C.POINTER(C.c_int)]),
+}
+
+_lib = None
+
+
+class MhhError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load and declare libmicrohh_amd.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MhhError(f"{LIB_PATH} is missing: build it with `python -m microhh_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib().mhh_last_error().decode(errors="replace")
+        raise MhhError(f"{what} failed (code {rc}): {msg}")
+
+
+def header_symbols(header="microhh_amd.h"):
+    """The function names declared in include/<header>."""
+    import re
+    path = os.path.join(os.path.dirname(HERE), "include", header)
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    return sorted(set(re.findall(r"\b(mhh_[a-z0-9_]+)\s*\(", text)))
+
+
+class DeviceArray:
+    """A zero-filled device buffer holding a float64/float32 array of a given shape."""
+
+    def __init__(self, shape, dtype="float64"):
+        import numpy as np
+        self.shape = tuple(shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = VP()
+        check(lib().mhh_device_alloc(self.nbytes, C.byref(p)), "mhh_device_alloc")
+        self.ptr = p.value
+
+    def upload(self, arr):
+        import numpy as np
+        a = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert a.shape == self.shape, (a.shape, self.shape)
+        check(lib().mhh_copy_to_device(self.ptr, a.ctypes.data, self.nbytes), "mhh_copy_to_device")
+
+    def download(self):
+        import numpy as np
+        out = np.empty(self.shape, dtype=self.dtype)
+        check(lib().mhh_copy_to_host(out.ctypes.data, self.ptr, self.nbytes), "mhh_copy_to_host")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().mhh_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -1,0 +1,314 @@
+// mhh_advec.hip -- 2i5 and 2nd-order flux-form advection and their CFL numbers.
+//
+// One thread per grid point, 64x4 blocks over (i, j), one k-level per block
+// row in z so that the wall-level variant is uniform across a block.  x is the
+// contiguous direction: a wavefront reads 64 consecutive cells per row and the
+// +-3 halo comes from the same cache lines; the y/z neighbours of adjacent
+// blocks hit L2.
+#include "mhh_internal.h"
+#include "mhh_stencil.h"
+
+namespace
+{
+    using namespace mhh;
+
+    template<typename TF>
+    struct Adv_args
+    {
+        TF* __restrict__ at;
+        const TF* __restrict__ s;   // advected field (scalars only)
+        const TF* __restrict__ u;
+        const TF* __restrict__ v;
+        const TF* __restrict__ w;
+        const TF* __restrict__ dz;  // dzi (u, v, s) or dzhi (w)
+        const TF* __restrict__ rho;
+        const TF* __restrict__ rhoh;
+        TF dxi, dyi;
+    };
+
+    __device__ __forceinline__ bool point(const Gd& g, int kfirst, int& i, int& j, int& k)
+    {
+        i = blockIdx.x*BX + threadIdx.x + g.istart;
+        j = blockIdx.y*BY + threadIdx.y + g.jstart;
+        k = blockIdx.z + kfirst;
+        return i < g.iend && j < g.jend;
+    }
+
+    // Advec_2i5 advec_u (src/advec_2i5.cxx:150-300).
+    template<typename TF>
+    __global__ void __launch_bounds__(BX*BY) advec_2i5_u_g(Adv_args<TF> a, Gd g)
+    {
+        int i, j, k;
+        if (!point(g, g.kstart, i, j, k)) return;
+        const int ii = 1, jj = g.icells, kk = g.ijcells;
+        const int ijk = i + j*jj + k*kk;
+        const TF* __restrict__ u = a.u; const TF* __restrict__ v = a.v; const TF* __restrict__ w = a.w;
+        TF t = a.at[ijk];
+        t += hsum(hflux_2i5(i2(u[ijk], u[ijk+ii]), i2(u[ijk-ii], u[ijk]), u, ijk, ii, a.dxi),
+                  hflux_2i5(i2(v[ijk-ii+jj], v[ijk+jj]), i2(v[ijk-ii], v[ijk]), u, ijk, jj, a.dyi));
+        t += vflux_2i5_c(level_class_c(k, g.kstart, g.kend),
+                         i2(w[ijk-ii+kk], w[ijk+kk]), i2(w[ijk-ii], w[ijk]),
+                         a.rhoh[k+1], a.rhoh[k], a.rho[k], a.dz[k], u, ijk, kk);
+        a.at[ijk] = t;
+    }
+
+    // Advec_2i5 advec_v (src/advec_2i5.cxx:302-450).
+    template<typename TF>
+    __global__ void __launch_bounds__(BX*BY) advec_2i5_v_g(Adv_args<TF> a, Gd g)
+    {
+        int i, j, k;
+        if (!point(g, g.kstart, i, j, k)) return;
+        const int ii = 1, jj = g.icells, kk = g.ijcells;
+        const int ijk = i + j*jj + k*kk;
+        const TF* __restrict__ u = a.u; const TF* __restrict__ v = a.v; const TF* __restrict__ w = a.w;
+        TF t = a.at[ijk];
+        t += hsum(hflux_2i5(i2(u[ijk+ii-jj], u[ijk+ii]), i2(u[ijk-jj], u[ijk]), v, ijk, ii, a.dxi),
+                  hflux_2i5(i2(v[ijk], v[ijk+jj]), i2(v[ijk-jj], v[ijk]), v, ijk, jj, a.dyi));
+        t += vflux_2i5_c(level_class_c(k, g.kstart, g.kend),
+                         i2(w[ijk-jj+kk], w[ijk+kk]), i2(w[ijk-jj], w[ijk]),
+                         a.rhoh[k+1], a.rhoh[k], a.rho[k], a.dz[k], v, ijk, kk);
+        a.at[ijk] = t;
+    }
+
+    // Advec_2i5 advec_w (src/advec_2i5.cxx:452-579), half levels kstart+1..kend-1.
+    template<typename TF>
+    __global__ void __launch_bounds__(BX*BY) advec_2i5_w_g(Adv_args<TF> a, Gd g)
+    {
+        int i, j, k;
+        if (!point(g, g.kstart+1, i, j, k)) return;
+        const int ii = 1, jj = g.icells, kk = g.ijcells;
+        const int ijk = i + j*jj + k*kk;
+        const TF* __restrict__ u = a.u; const TF* __restrict__ v = a.v; const TF* __restrict__ w = a.w;
+        TF t = a.at[ijk];
+        t += hsum(hflux_2i5(i2(u[ijk+ii-kk], u[ijk+ii]), i2(u[ijk-kk], u[ijk]), w, ijk, ii, a.dxi),
+                  hflux_2i5(i2(v[ijk+jj-kk], v[ijk+jj]), i2(v[ijk-kk], v[ijk]), w, ijk, jj, a.dyi));
+        t += vflux_2i5_w(level_class_w(k, g.kstart, g.kend), a.rho[k], a.rho[k-1], a.rhoh[k], a.dz[k], w, ijk, kk);
+        a.at[ijk] = t;
+    }
+
+    // Advec_2i5 advec_s (src/advec_2i5.cxx:581-728).
+    template<typename TF>
+    __global__ void __launch_bounds__(BX*BY) advec_2i5_s_g(Adv_args<TF> a, Gd g)
+    {
+        int i, j, k;
+        if (!point(g, g.kstart, i, j, k)) return;
+        const int ii = 1, jj = g.icells, kk = g.ijcells;
+        const int ijk = i + j*jj + k*kk;
+        const TF* __restrict__ u = a.u; const TF* __restrict__ v = a.v; const TF* __restrict__ w = a.w;
+        TF t = a.at[ijk];
+        t += hsum(hflux_2i5(u[ijk+ii], u[ijk], a.s, ijk, ii, a.dxi),
+                  hflux_2i5(v[ijk+jj], v[ijk], a.s, ijk, jj, a.dyi));
+        t += vflux_2i5_c(level_class_c(k, g.kstart, g.kend), w[ijk+kk], w[ijk],
+                         a.rhoh[k+1], a.rhoh[k], a.rho[k], a.dz[k], a.s, ijk, kk);
+        a.at[ijk] = t;
+    }
+
+    // Advec_2 (src/advec_2.cxx:82-202).
+    template<typename TF>
+    __global__ void __launch_bounds__(BX*BY) advec_2_u_g(Adv_args<TF> a, Gd g)
+    {
+        int i, j, k;
+        if (!point(g, g.kstart, i, j, k)) return;
+        const int ii = 1, jj = g.icells, kk = g.ijcells;
+        const int ijk = i + j*jj + k*kk;
+        const TF* __restrict__ u = a.u; const TF* __restrict__ v = a.v; const TF* __restrict__ w = a.w;
+        a.at[ijk] +=
+                 - ( i2(u[ijk   ], u[ijk+ii]) * i2(u[ijk   ], u[ijk+ii])
+                   - i2(u[ijk-ii], u[ijk   ]) * i2(u[ijk-ii], u[ijk   ]) ) * a.dxi
+                 - ( i2(v[ijk-ii+jj], v[ijk+jj]) * i2(u[ijk   ], u[ijk+jj])
+                   - i2(v[ijk-ii   ], v[ijk   ]) * i2(u[ijk-jj], u[ijk   ]) ) * a.dyi
+                 - ( a.rhoh[k+1] * i2(w[ijk-ii+kk], w[ijk+kk]) * i2(u[ijk   ], u[ijk+kk])
+                   - a.rhoh[k  ] * i2(w[ijk-ii   ], w[ijk   ]) * i2(u[ijk-kk], u[ijk   ]) ) / a.rho[k] * a.dz[k];
+    }
+
+    template<typename TF>
+    __global__ void __launch_bounds__(BX*BY) advec_2_v_g(Adv_args<TF> a, Gd g)
+    {
+        int i, j, k;
+        if (!point(g, g.kstart, i, j, k)) return;
+        const int ii = 1, jj = g.icells, kk = g.ijcells;
+        const int ijk = i + j*jj + k*kk;
+        const TF* __restrict__ u = a.u; const TF* __restrict__ v = a.v; const TF* __restrict__ w = a.w;
+        a.at[ijk] +=
+                 - ( i2(u[ijk+ii-jj], u[ijk+ii]) * i2(v[ijk   ], v[ijk+ii])
+                   - i2(u[ijk   -jj], u[ijk   ]) * i2(v[ijk-ii], v[ijk   ]) ) * a.dxi
+                 - ( i2(v[ijk   ], v[ijk+jj]) * i2(v[ijk   ], v[ijk+jj])
+                   - i2(v[ijk-jj], v[ijk   ]) * i2(v[ijk-jj], v[ijk   ]) ) * a.dyi
+                 - ( a.rhoh[k+1] * i2(w[ijk-jj+kk], w[ijk+kk]) * i2(v[ijk   ], v[ijk+kk])
+                   - a.rhoh[k  ] * i2(w[ijk-jj   ], w[ijk   ]) * i2(v[ijk-kk], v[ijk   ]) ) / a.rho[k] * a.dz[k];
+    }
+
+    template<typename TF>
+    __global__ void __launch_bounds__(BX*BY) advec_2_w_g(Adv_args<TF> a, Gd g)
+    {
+        int i, j, k;
+        if (!point(g, g.kstart+1, i, j, k)) return;
+        const int ii = 1, jj = g.icells, kk = g.ijcells;
+        const int ijk = i + j*jj + k*kk;
+        const TF* __restrict__ u = a.u; const TF* __restrict__ v = a.v; const TF* __restrict__ w = a.w;
+        a.at[ijk] +=
+                 - ( i2(u[ijk+ii-kk], u[ijk+ii]) * i2(w[ijk   ], w[ijk+ii])
+                   - i2(u[ijk   -kk], u[ijk   ]) * i2(w[ijk-ii], w[ijk   ]) ) * a.dxi
+                 - ( i2(v[ijk+jj-kk], v[ijk+jj]) * i2(w[ijk   ], w[ijk+jj])
+                   - i2(v[ijk   -kk], v[ijk   ]) * i2(w[ijk-jj], w[ijk   ]) ) * a.dyi
+                 - ( a.rho[k  ] * i2(w[ijk   ], w[ijk+kk]) * i2(w[ijk   ], w[ijk+kk])
+                   - a.rho[k-1] * i2(w[ijk-kk], w[ijk   ]) * i2(w[ijk-kk], w[ijk   ]) ) / a.rhoh[k] * a.dz[k];
+    }
+
+    template<typename TF>
+    __global__ void __launch_bounds__(BX*BY) advec_2_s_g(Adv_args<TF> a, Gd g)
+    {
+        int i, j, k;
+        if (!point(g, g.kstart, i, j, k)) return;
+        const int ii = 1, jj = g.icells, kk = g.ijcells;
+        const int ijk = i + j*jj + k*kk;
+        const TF* __restrict__ u = a.u; const TF* __restrict__ v = a.v; const TF* __restrict__ w = a.w;
+        const TF* __restrict__ s = a.s;
+        a.at[ijk] +=
+                 - ( u[ijk+ii] * i2(s[ijk   ], s[ijk+ii])
+                   - u[ijk   ] * i2(s[ijk-ii], s[ijk   ]) ) * a.dxi
+                 - ( v[ijk+jj] * i2(s[ijk   ], s[ijk+jj])
+                   - v[ijk   ] * i2(s[ijk-jj], s[ijk   ]) ) * a.dyi
+                 - ( a.rhoh[k+1] * w[ijk+kk] * i2(s[ijk   ], s[ijk+kk])
+                   - a.rhoh[k  ] * w[ijk   ] * i2(s[ijk-kk], s[ijk   ]) ) / a.rho[k] * a.dz[k];
+    }
+
+    // Pointwise CFL with per-block maxima into red[1 + block] (src/advec_2i5.cxx:59-148,
+    // src/advec_2.cxx:50-80); order 0 = 2i5, 1 = 2nd order.
+    template<typename TF, int order>
+    __global__ void __launch_bounds__(BX*BY) cfl_g(const TF* __restrict__ u, const TF* __restrict__ v,
+                                                   const TF* __restrict__ w, const TF* __restrict__ dzi,
+                                                   TF dxi, TF dyi, Gd g, double* __restrict__ red)
+    {
+        __shared__ double sm[BX*BY];
+        const int tid = threadIdx.x + threadIdx.y*BX;
+        int i, j, k;
+        double c = 0.;
+        if (point(g, g.kstart, i, j, k))
+        {
+            const int ii = 1, jj = g.icells, kk = g.ijcells;
+            const int ijk = i + j*jj + k*kk;
+            TF val;
+            if (order == 0)
+            {
+                TF wpart;
+                if (k == g.kstart || k == g.kend-1)
+                    wpart = fabs(i2(w[ijk], w[ijk+kk]))*dzi[k];
+                else if (k == g.kstart+1 || k == g.kend-2)
+                    wpart = fabs(i4ws(w[ijk-kk], w[ijk], w[ijk+kk], w[ijk+2*kk]))*dzi[k];
+                else
+                    wpart = fabs(i6ws(w[ijk-2*kk], w[ijk-kk], w[ijk], w[ijk+kk], w[ijk+2*kk], w[ijk+3*kk]))*dzi[k];
+                val = fabs(i6ws(u[ijk-2*ii], u[ijk-ii], u[ijk], u[ijk+ii], u[ijk+2*ii], u[ijk+3*ii]))*dxi
+                    + fabs(i6ws(v[ijk-2*jj], v[ijk-jj], v[ijk], v[ijk+jj], v[ijk+2*jj], v[ijk+3*jj]))*dyi
+                    + wpart;
+            }
+            else
+                val = fabs(i2(u[ijk], u[ijk+ii]))*dxi + fabs(i2(v[ijk], v[ijk+jj]))*dyi
+                    + fabs(i2(w[ijk], w[ijk+kk]))*dzi[k];
+            c = static_cast<double>(val);
+        }
+        sm[tid] = c;
+        __syncthreads();
+        for (int s = BX*BY/2; s > 0; s >>= 1)
+        {
+            if (tid < s) sm[tid] = fmax(sm[tid], sm[tid + s]);
+            __syncthreads();
+        }
+        if (tid == 0)
+            red[1 + blockIdx.x + gridDim.x*(blockIdx.y + gridDim.y*blockIdx.z)] = sm[0];
+    }
+
+    template<typename TF>
+    Adv_args<TF> args(mhh_ctx* c, const Prof<TF>& p, void* at, void* s, bool w_levels)
+    {
+        Adv_args<TF> a;
+        a.at = static_cast<TF*>(at);
+        a.s = static_cast<const TF*>(s);
+        a.u = static_cast<const TF*>(c->f.u);
+        a.v = static_cast<const TF*>(c->f.v);
+        a.w = static_cast<const TF*>(c->f.w);
+        a.dz = w_levels ? p.dzhi : p.dzi;
+        a.rho = p.rhoref;
+        a.rhoh = p.rhorefh;
+        a.dxi = TF(1.)/TF(c->gd.dx);
+        a.dyi = TF(1.)/TF(c->gd.dy);
+        return a;
+    }
+
+    template<typename TF>
+    int exec_2i5(mhh_ctx* c, const Prof<TF>& p)
+    {
+        const Gd& g = c->gd;
+        const dim3 blk(BX, BY);
+        { mhh::Timer _t(c, "advec_2i5_u"); advec_2i5_u_g<TF><<<grid3(g, g.kstart, g.kend), blk, 0, c->stream>>>(args<TF>(c, p, c->f.ut, nullptr, false), g); }
+        { mhh::Timer _t(c, "advec_2i5_v"); advec_2i5_v_g<TF><<<grid3(g, g.kstart, g.kend), blk, 0, c->stream>>>(args<TF>(c, p, c->f.vt, nullptr, false), g); }
+        { mhh::Timer _t(c, "advec_2i5_w"); advec_2i5_w_g<TF><<<grid3(g, g.kstart+1, g.kend), blk, 0, c->stream>>>(args<TF>(c, p, c->f.wt, nullptr, true), g); }
+        for (int n = 0; n < c->f.nscalars; ++n)
+            { mhh::Timer _t(c, "advec_2i5_s"); advec_2i5_s_g<TF><<<grid3(g, g.kstart, g.kend), blk, 0, c->stream>>>(args<TF>(c, p, c->f.st[n], c->f.s[n], false), g); }
+        return check_launch("advec_2i5");
+    }
+
+    template<typename TF>
+    int exec_2(mhh_ctx* c, const Prof<TF>& p)
+    {
+        const Gd& g = c->gd;
+        const dim3 blk(BX, BY);
+        { mhh::Timer _t(c, "advec_2_u"); advec_2_u_g<TF><<<grid3(g, g.kstart, g.kend), blk, 0, c->stream>>>(args<TF>(c, p, c->f.ut, nullptr, false), g); }
+        { mhh::Timer _t(c, "advec_2_v"); advec_2_v_g<TF><<<grid3(g, g.kstart, g.kend), blk, 0, c->stream>>>(args<TF>(c, p, c->f.vt, nullptr, false), g); }
+        { mhh::Timer _t(c, "advec_2_w"); advec_2_w_g<TF><<<grid3(g, g.kstart+1, g.kend), blk, 0, c->stream>>>(args<TF>(c, p, c->f.wt, nullptr, true), g); }
+        for (int n = 0; n < c->f.nscalars; ++n)
+            { mhh::Timer _t(c, "advec_2_s"); advec_2_s_g<TF><<<grid3(g, g.kstart, g.kend), blk, 0, c->stream>>>(args<TF>(c, p, c->f.st[n], c->f.s[n], false), g); }
+        return check_launch("advec_2");
+    }
+
+    template<typename TF, int order>
+    int cfl(mhh_ctx* c, const Prof<TF>& p, double dt, double* out)
+    {
+        const Gd& g = c->gd;
+        const dim3 grd = grid3(g, g.kstart, g.kend);
+        const int nb = static_cast<int>(grd.x*grd.y*grd.z);
+        MHH_CHECK(ensure_reduction(c, nb));
+        { mhh::Timer _t(c, "cfl"); cfl_g<TF, order><<<grd, dim3(BX, BY), 0, c->stream>>>(
+                static_cast<const TF*>(c->f.u), static_cast<const TF*>(c->f.v), static_cast<const TF*>(c->f.w),
+                p.dzi, TF(1.)/TF(g.dx), TF(1.)/TF(g.dy), g, c->red); }
+        MHH_CHECK(check_launch("cfl"));
+        double m;
+        MHH_CHECK(reduce_max_partials(c, nb, &m));
+        *out = m*dt;
+        return 0;
+    }
+}
+
+extern "C" {
+
+int mhh_advec_2i5_exec(mhh_ctx* c)
+{
+    MHH_REQUIRE_BOUND(c);
+    if (c->gd.igc < 3 || c->gd.jgc < 3)
+        return mhh::fail(MHH_EINVAL, "advec_2i5 needs igc = jgc >= 3 (src/advec_2i5.cxx:42-45)");
+    return c->precision == MHH_FP64 ? exec_2i5<double>(c, c->pd) : exec_2i5<float>(c, c->pf);
+}
+
+int mhh_advec_2_exec(mhh_ctx* c)
+{
+    MHH_REQUIRE_BOUND(c);
+    return c->precision == MHH_FP64 ? exec_2<double>(c, c->pd) : exec_2<float>(c, c->pf);
+}
+
+int mhh_advec_2i5_cfl(mhh_ctx* c, double dt, double* out)
+{
+    MHH_REQUIRE_BOUND(c);
+    if (!out) return mhh::fail(MHH_EINVAL, "null output");
+    if (c->gd.igc < 3 || c->gd.jgc < 3)
+        return mhh::fail(MHH_EINVAL, "advec_2i5 needs igc = jgc >= 3");
+    return c->precision == MHH_FP64 ? cfl<double, 0>(c, c->pd, dt, out) : cfl<float, 0>(c, c->pf, dt, out);
+}
+
+int mhh_advec_2_cfl(mhh_ctx* c, double dt, double* out)
+{
+    MHH_REQUIRE_BOUND(c);
+    if (!out) return mhh::fail(MHH_EINVAL, "null output");
+    return c->precision == MHH_FP64 ? cfl<double, 1>(c, c->pd, dt, out) : cfl<float, 1>(c, c->pf, dt, out);
+}
+
+} // extern "C"

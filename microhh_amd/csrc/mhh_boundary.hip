@@ -1,0 +1,252 @@
+// mhh_boundary.hip -- periodic halos, 2nd-order z ghost cells, dry buoyancy
+// and the low-storage RK3 update.
+#include "mhh_internal.h"
+#include "mhh_stencil.h"
+
+namespace
+{
+    using namespace mhh;
+
+    constexpr double grav = 9.81;   // include/constants.h:29
+
+    // East-west halo copy over every (j, k) of the ghosted field
+    // (src/boundary_cyclic.cxx:378-398): column i < igc takes iend-igc+i,
+    // column iend+i takes istart+i.
+    template<typename TF>
+    __global__ void cyclic_ew_g(TF* __restrict__ a, Gd g)
+    {
+        const int t = blockIdx.x*blockDim.x + threadIdx.x;   // over 2*igc*jcells
+        const int k = blockIdx.y;
+        const int per = 2*g.igc;
+        if (t >= per*g.jcells) return;
+        const int j = t / per, c = t % per;
+        const int base = j*g.icells + k*g.ijcells;
+        if (c < g.igc)
+            a[base + c] = a[base + g.iend - g.igc + c];
+        else
+        {
+            const int i = c - g.igc;
+            a[base + g.iend + i] = a[base + g.istart + i];
+        }
+    }
+
+    // North-south halo copy over every (i, k) (src/boundary_cyclic.cxx:402-423);
+    // for jtot == 1 the ghost rows take the only row, interior levels only (:425-439).
+    template<typename TF>
+    __global__ void cyclic_ns_g(TF* __restrict__ a, Gd g)
+    {
+        const int t = blockIdx.x*blockDim.x + threadIdx.x;   // over 2*jgc*icells
+        const int k = blockIdx.y;
+        if (t >= 2*g.jgc*g.icells) return;
+        const int r = t / g.icells, i = t % g.icells;
+        const int kk = k*g.ijcells, jj = g.icells;
+        if (g.jtot > 1)
+        {
+            if (r < g.jgc)
+                a[i + r*jj + kk] = a[i + (g.jend - g.jgc + r)*jj + kk];
+            else
+            {
+                const int jr = r - g.jgc;
+                a[i + (g.jend + jr)*jj + kk] = a[i + (g.jstart + jr)*jj + kk];
+            }
+        }
+        else if (k >= g.kstart && k < g.kend)
+        {
+            const TF ref = a[i + g.jstart*jj + kk];
+            if (r < g.jgc) a[i + r*jj + kk] = ref;
+            else a[i + (g.jend + r - g.jgc)*jj + kk] = ref;
+        }
+    }
+
+    // calc_ghost_cells_bot_2nd / top_2nd (src/boundary.cxx:700-775) over the full plane.
+    template<typename TF>
+    __global__ void ghost_2nd_g(TF* __restrict__ a, const TF* __restrict__ abot, const TF* __restrict__ agradbot,
+                                TF* __restrict__ atop, const TF* __restrict__ agradtop,
+                                TF dzh_bot, TF dzh_top, int botype, int toptype, Gd g)
+    {
+        const int ij = blockIdx.x*blockDim.x + threadIdx.x;
+        if (ij >= g.ijcells) return;
+        const int kk = g.ijcells, ks = g.kstart, ke = g.kend;
+        if (botype == MHH_BC_DIRICHLET)
+            a[ij + (ks-1)*kk] = TF(2.)*abot[ij] - a[ij + ks*kk];
+        else if (botype == MHH_BC_NEUMANN || botype == MHH_BC_FLUX)
+            a[ij + (ks-1)*kk] = -agradbot[ij]*dzh_bot + a[ij + ks*kk];
+
+        if (toptype == MHH_BC_DIRICHLET || toptype == MHH_BC_OFF)
+        {
+            if (toptype == MHH_BC_OFF)
+                atop[ij] = TF(3./2.)*a[ij + (ke-1)*kk] - TF(1./2.)*a[ij + (ke-2)*kk];
+            a[ij + ke*kk] = TF(2.)*atop[ij] - a[ij + (ke-1)*kk];
+        }
+        else if (toptype == MHH_BC_NEUMANN || toptype == MHH_BC_FLUX)
+            a[ij + ke*kk] = agradtop[ij]*dzh_top + a[ij + (ke-1)*kk];
+    }
+
+    // calc_buoyancy_tend_2nd (src/thermo_dry.cxx:163-180).
+    template<typename TF>
+    __global__ void __launch_bounds__(BX*BY) buoyancy_g(TF* __restrict__ wt, const TF* __restrict__ th,
+                                                        const TF* __restrict__ threfh, Gd g)
+    {
+        const int i = blockIdx.x*BX + threadIdx.x + g.istart;
+        const int j = blockIdx.y*BY + threadIdx.y + g.jstart;
+        const int k = blockIdx.z + g.kstart + 1;
+        if (i >= g.iend || j >= g.jend) return;
+        const int ijk = i + j*g.icells + k*g.ijcells;
+        wt[ijk] += TF(grav)/threfh[k] * (i2(th[ijk-g.ijcells], th[ijk]) - threfh[k]);
+    }
+
+    // rk3 (src/timeloop.cxx:251-286): a += cB[s]*dt*at, then at *= cA[s+1]
+    // (interior), or at = 0 over the whole field after the last substep.
+    template<typename TF>
+    __global__ void __launch_bounds__(BX*BY) rk3_g(TF* __restrict__ a, TF* __restrict__ at, TF cBdt, TF cAn,
+                                                   int zero, Gd g)
+    {
+        const int i = blockIdx.x*BX + threadIdx.x + g.istart;
+        const int j = blockIdx.y*BY + threadIdx.y + g.jstart;
+        const int k = blockIdx.z + g.kstart;
+        if (i >= g.iend || j >= g.jend) return;
+        const int ijk = i + j*g.icells + k*g.ijcells;
+        const TF t = at[ijk];
+        a[ijk] += cBdt*t;
+        at[ijk] = zero ? TF(0) : t*cAn;
+    }
+
+    // Zero the ghost cells of a tendency (the whole-field reset of the last substep).
+    template<typename TF>
+    __global__ void zero_ghosts_g(TF* __restrict__ at, Gd g)
+    {
+        const int n = blockIdx.x*blockDim.x + threadIdx.x;
+        if (n >= g.ncells) return;
+        const int k = n / g.ijcells, r = n % g.ijcells, j = r / g.icells, i = r % g.icells;
+        if (k < g.kstart || k >= g.kend || j < g.jstart || j >= g.jend || i < g.istart || i >= g.iend)
+            at[n] = TF(0);
+    }
+
+    template<typename TF>
+    int cyclic(mhh_ctx* c, TF* a, int edge)
+    {
+        const Gd& g = c->gd;
+        if (edge == MHH_EDGE_EAST_WEST || edge == MHH_EDGE_BOTH)
+        {
+            const int n = 2*g.igc*g.jcells;
+            { mhh::Timer _t(c, "cyclic"); cyclic_ew_g<TF><<<dim3((n + 255)/256, g.kcells), 256, 0, c->stream>>>(a, g); }
+        }
+        if (edge == MHH_EDGE_NORTH_SOUTH || edge == MHH_EDGE_BOTH)
+        {
+            const int n = 2*g.jgc*g.icells;
+            { mhh::Timer _t(c, "cyclic"); cyclic_ns_g<TF><<<dim3((n + 255)/256, g.kcells), 256, 0, c->stream>>>(a, g); }
+        }
+        return check_launch("boundary_cyclic");
+    }
+
+    template<typename TF>
+    int ghost(mhh_ctx* c, const Prof<TF>& p, void* a, void* bot, void* gbot, void* top, void* gtop,
+              int botype, int toptype, const char* name)
+    {
+        const Gd& g = c->gd;
+        const bool need_bot = botype == MHH_BC_DIRICHLET || botype == MHH_BC_NEUMANN || botype == MHH_BC_FLUX;
+        const bool need_top = toptype == MHH_BC_DIRICHLET || toptype == MHH_BC_OFF ||
+                              toptype == MHH_BC_NEUMANN || toptype == MHH_BC_FLUX;
+        if (need_bot && !(botype == MHH_BC_DIRICHLET ? bot : gbot))
+            return fail(MHH_ESTATE, std::string("ghost cells: bottom boundary plane of ") + name + " not bound");
+        if (need_top && !((toptype == MHH_BC_DIRICHLET || toptype == MHH_BC_OFF) ? top : gtop))
+            return fail(MHH_ESTATE, std::string("ghost cells: top boundary plane of ") + name + " not bound");
+        { mhh::Timer _t(c, "ghost_cells"); ghost_2nd_g<TF><<<(g.ijcells + 255)/256, 256, 0, c->stream>>>(
+                static_cast<TF*>(a), static_cast<const TF*>(bot), static_cast<const TF*>(gbot),
+                static_cast<TF*>(top), static_cast<const TF*>(gtop),
+                TF(c->h_dzh[g.kstart]), TF(c->h_dzh[g.kend]), botype, toptype, g); }
+        return check_launch("ghost_cells");
+    }
+
+    template<typename TF>
+    int set_ghost_cells(mhh_ctx* c, const Prof<TF>& p, const mhh_bc_desc& bc)
+    {
+        const mhh_field_set& f = c->f;
+        MHH_CHECK(ghost<TF>(c, p, f.u, f.u_bot, f.u_gradbot, f.u_top, f.u_gradtop, bc.mbcbot, bc.mbctop, "u"));
+        MHH_CHECK(ghost<TF>(c, p, f.v, f.v_bot, f.v_gradbot, f.v_top, f.v_gradtop, bc.mbcbot, bc.mbctop, "v"));
+        for (int n = 0; n < f.nscalars; ++n)
+            MHH_CHECK(ghost<TF>(c, p, f.s[n], f.s_bot[n], f.s_gradbot[n], f.s_top[n], f.s_gradtop[n],
+                                bc.sbcbot[n], bc.sbctop[n], "scalar"));
+        return 0;
+    }
+
+    template<typename TF>
+    int buoyancy(mhh_ctx* c, const Prof<TF>& p)
+    {
+        const Gd& g = c->gd;
+        { mhh::Timer _t(c, "buoyancy"); buoyancy_g<TF><<<grid3(g, g.kstart+1, g.kend), dim3(BX, BY), 0, c->stream>>>(
+                static_cast<TF*>(c->f.wt), static_cast<const TF*>(c->f.s[0]), p.threfh, g); }
+        return check_launch("buoyancy");
+    }
+
+    template<typename TF>
+    int rk3(mhh_ctx* c, int substep, double dt)
+    {
+        // Williamson low-storage coefficients (src/timeloop.cxx:255-256).
+        const TF cA[] = {TF(0.), TF(-5./9.), TF(-153./128.)};
+        const TF cB[] = {TF(1./3.), TF(15./16.), TF(8./15.)};
+        const Gd& g = c->gd;
+        const int substepn = (substep + 1) % 3;
+        const TF cBdt = cB[substep]*TF(dt);
+        const int zero = substepn == 0;
+        void* fld[3 + MHH_MAX_SCALARS]; void* ten[3 + MHH_MAX_SCALARS];
+        int nf = 0;
+        for (int n = 0; n < c->f.nscalars; ++n) { fld[nf] = c->f.s[n]; ten[nf++] = c->f.st[n]; }
+        fld[nf] = c->f.u; ten[nf++] = c->f.ut;
+        fld[nf] = c->f.v; ten[nf++] = c->f.vt;
+        fld[nf] = c->f.w; ten[nf++] = c->f.wt;
+        for (int q = 0; q < nf; ++q)
+        {
+            { mhh::Timer _t(c, "rk3"); rk3_g<TF><<<grid3(g, g.kstart, g.kend), dim3(BX, BY), 0, c->stream>>>(
+                    static_cast<TF*>(fld[q]), static_cast<TF*>(ten[q]), cBdt, cA[substepn], zero, g); }
+            if (zero)
+                { mhh::Timer _t(c, "rk3_zero_ghosts"); zero_ghosts_g<TF><<<(g.ncells + 255)/256, 256, 0, c->stream>>>(static_cast<TF*>(ten[q]), g); }
+        }
+        return check_launch("rk3");
+    }
+}
+
+extern "C" {
+
+int mhh_boundary_cyclic(mhh_ctx* c, void* field, int edge)
+{
+    if (!c || !field) return mhh::fail(MHH_EINVAL, "mhh_boundary_cyclic: null argument");
+    if (edge < MHH_EDGE_EAST_WEST || edge > MHH_EDGE_BOTH) return mhh::fail(MHH_EINVAL, "bad edge");
+    return c->precision == MHH_FP64 ? cyclic<double>(c, static_cast<double*>(field), edge)
+                                    : cyclic<float>(c, static_cast<float*>(field), edge);
+}
+
+int mhh_boundary_set_prognostic_cyclic_bcs(mhh_ctx* c)
+{
+    MHH_REQUIRE_BOUND(c);
+    MHH_CHECK(mhh_boundary_cyclic(c, c->f.u, MHH_EDGE_BOTH));
+    MHH_CHECK(mhh_boundary_cyclic(c, c->f.v, MHH_EDGE_BOTH));
+    MHH_CHECK(mhh_boundary_cyclic(c, c->f.w, MHH_EDGE_BOTH));
+    for (int n = 0; n < c->f.nscalars; ++n)
+        MHH_CHECK(mhh_boundary_cyclic(c, c->f.s[n], MHH_EDGE_BOTH));
+    return 0;
+}
+
+int mhh_boundary_set_ghost_cells(mhh_ctx* c, const mhh_bc_desc* bc)
+{
+    MHH_REQUIRE_BOUND(c);
+    if (!bc) return mhh::fail(MHH_EINVAL, "null bc descriptor");
+    return c->precision == MHH_FP64 ? set_ghost_cells<double>(c, c->pd, *bc) : set_ghost_cells<float>(c, c->pf, *bc);
+}
+
+int mhh_thermo_dry_exec(mhh_ctx* c)
+{
+    MHH_REQUIRE_BOUND(c);
+    if (c->f.nscalars < 1) return mhh::fail(MHH_ESTATE, "thermo_dry needs scalar 0 (th)");
+    if (!c->have_base_state) return mhh::fail(MHH_ESTATE, "thermo_dry needs mhh_set_base_state()");
+    return c->precision == MHH_FP64 ? buoyancy<double>(c, c->pd) : buoyancy<float>(c, c->pf);
+}
+
+int mhh_timeloop_rk3(mhh_ctx* c, int substep, double dt)
+{
+    MHH_REQUIRE_BOUND(c);
+    if (substep < 0 || substep > 2) return mhh::fail(MHH_EINVAL, "rk3 substep must be 0, 1 or 2");
+    return c->precision == MHH_FP64 ? rk3<double>(c, substep, dt) : rk3<float>(c, substep, dt);
+}
+
+} // extern "C"

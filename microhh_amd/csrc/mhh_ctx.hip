@@ -1,0 +1,321 @@
+// mhh_ctx.hip -- context lifetime, field binding, errors and max-reductions.
+#include <cstdio>
+#include <cstring>
+#include <limits>
+
+#include "mhh_internal.h"
+
+namespace
+{
+    thread_local std::string last_error;
+
+    template<typename TF>
+    int upload(TF** dst, const std::vector<double>& src)
+    {
+        if (src.empty()) return 0;
+        std::vector<TF> tmp(src.begin(), src.end());
+        MHH_CHECK(mhh::check_hip(hipMalloc(dst, sizeof(TF)*tmp.size()), "hipMalloc(profile)"));
+        return mhh::check_hip(hipMemcpy(*dst, tmp.data(), sizeof(TF)*tmp.size(), hipMemcpyHostToDevice),
+                              "hipMemcpy(profile)");
+    }
+
+    template<typename TF>
+    void free_prof(mhh::Prof<TF>& p)
+    {
+        TF* all[] = {p.z, p.zh, p.dz, p.dzh, p.dzi, p.dzhi, p.rhoref, p.rhorefh, p.thref, p.threfh};
+        for (TF* a : all) if (a) (void)hipFree(a);
+        p = mhh::Prof<TF>();
+    }
+
+    template<typename TF>
+    int upload_profiles(mhh_ctx* c, mhh::Prof<TF>& p)
+    {
+        MHH_CHECK(upload(&p.z, c->h_z));
+        MHH_CHECK(upload(&p.zh, c->h_zh));
+        MHH_CHECK(upload(&p.dz, c->h_dz));
+        MHH_CHECK(upload(&p.dzh, c->h_dzh));
+        MHH_CHECK(upload(&p.dzi, c->h_dzi));
+        MHH_CHECK(upload(&p.dzhi, c->h_dzhi));
+        MHH_CHECK(upload(&p.rhoref, c->h_rhoref));
+        MHH_CHECK(upload(&p.rhorefh, c->h_rhorefh));
+        return 0;
+    }
+
+    // Second stage of a max-reduction: one block folds the partials into red[0].
+    __global__ void __launch_bounds__(256) fold_max(double* __restrict__ red, int n)
+    {
+        __shared__ double s[256];
+        double m = 0.;
+        for (int i = threadIdx.x; i < n; i += blockDim.x)
+            m = fmax(m, red[1 + i]);
+        s[threadIdx.x] = m;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1)
+        {
+            if (threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) red[0] = s[0];
+    }
+}
+
+namespace mhh
+{
+    void set_error(const std::string& msg) { last_error = msg; }
+
+    int fail(int code, const std::string& msg)
+    {
+        last_error = msg;
+        return code;
+    }
+
+    int check_hip(hipError_t e, const char* what)
+    {
+        if (e == hipSuccess) return 0;
+        return fail(MHH_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+    }
+
+    int check_launch(const char* what)
+    {
+        return check_hip(hipGetLastError(), what);
+    }
+
+    bool fits_int(long long v) { return v >= 0 && v < std::numeric_limits<int>::max(); }
+
+    int ensure_reduction(mhh_ctx* ctx, int nblocks)
+    {
+        if (nblocks + 1 <= ctx->red_capacity) return 0;
+        if (ctx->red) (void)hipFree(ctx->red);
+        ctx->red = nullptr;
+        const int cap = nblocks + 1;
+        MHH_CHECK(check_hip(hipMalloc(&ctx->red, sizeof(double)*cap), "hipMalloc(reduction)"));
+        ctx->red_capacity = cap;
+        return 0;
+    }
+
+    Timer::Timer(mhh_ctx* ctx, const char* name) : c(ctx)
+    {
+        if (!c || !c->timing) return;
+        auto& pool = c->timers[name];
+        if (pool.used == static_cast<int>(pool.ev.size()))
+        {
+            hipEvent_t a, b;
+            if (hipEventCreate(&a) != hipSuccess) return;
+            if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return; }
+            pool.ev.emplace_back(a, b);
+        }
+        auto& pr = pool.ev[pool.used++];
+        (void)hipEventRecord(pr.first, c->stream);
+        stop = pr.second;
+    }
+
+    Timer::~Timer()
+    {
+        if (stop) (void)hipEventRecord(stop, c->stream);
+    }
+
+    int reduce_max_partials(mhh_ctx* ctx, int npartials, double* out)
+    {
+        fold_max<<<1, 256, 0, ctx->stream>>>(ctx->red, npartials);
+        MHH_CHECK(check_launch("fold_max"));
+        MHH_CHECK(check_hip(hipMemcpyAsync(ctx->h_red, ctx->red, sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+                            "hipMemcpyAsync(reduction)"));
+        MHH_CHECK(check_hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize"));
+        *out = *ctx->h_red;
+        return 0;
+    }
+}
+
+extern "C" {
+
+const char* mhh_last_error(void) { return last_error.c_str(); }
+
+int mhh_element_size(const mhh_ctx* ctx)
+{
+    return ctx ? (ctx->precision == MHH_FP64 ? 8 : 4) : 0;
+}
+
+int mhh_ctx_create(const mhh_grid_desc* g, int precision, void* hip_stream, mhh_ctx** out)
+{
+    using mhh::fail;
+    if (!g || !out) return fail(MHH_EINVAL, "mhh_ctx_create: null argument");
+    *out = nullptr;
+    if (precision != MHH_FP64 && precision != MHH_FP32)
+        return fail(MHH_EINVAL, "mhh_ctx_create: precision must be MHH_FP64 or MHH_FP32");
+    if (g->itot < 1 || g->jtot < 1 || g->ktot < 6)
+        return fail(MHH_EINVAL, "mhh_ctx_create: need itot, jtot >= 1 and ktot >= 6 "
+                                "(the 2i5 wall-level variants of src/advec_2i5.cxx:219-299 must not overlap)");
+    if (g->igc < 1 || g->jgc < 1 || g->kgc < 1 || g->igc > g->itot || g->jgc > g->jtot)
+        return fail(MHH_EINVAL, "mhh_ctx_create: ghost cells must satisfy 1 <= gc <= size");
+    if (!(g->dx > 0) || !(g->dy > 0))
+        return fail(MHH_EINVAL, "mhh_ctx_create: dx and dy must be positive");
+    const double* req[] = {g->z, g->zh, g->dz, g->dzh, g->dzi, g->dzhi, g->rhoref, g->rhorefh};
+    for (const double* r : req)
+        if (!r) return fail(MHH_EINVAL, "mhh_ctx_create: every 1-D grid array is required");
+
+    const long long icells = g->itot + 2LL*g->igc, jcells = g->jtot + 2LL*g->jgc, kcells = g->ktot + 2LL*g->kgc;
+    if (!mhh::fits_int(icells*jcells*kcells))
+        return fail(MHH_EINVAL, "mhh_ctx_create: ncells exceeds the 32-bit index range");
+
+    mhh_ctx* c = new mhh_ctx();
+    c->precision = precision;
+    c->stream = static_cast<hipStream_t>(hip_stream);
+    (void)hipGetDevice(&c->device);
+
+    mhh::Gd& d = c->gd;
+    d.itot = g->itot; d.jtot = g->jtot; d.ktot = g->ktot;
+    d.igc = g->igc; d.jgc = g->jgc; d.kgc = g->kgc;
+    d.istart = g->igc; d.iend = g->itot + g->igc;
+    d.jstart = g->jgc; d.jend = g->jtot + g->jgc;
+    d.kstart = g->kgc; d.kend = g->ktot + g->kgc;
+    d.icells = static_cast<int>(icells); d.jcells = static_cast<int>(jcells); d.kcells = static_cast<int>(kcells);
+    d.ijcells = d.icells*d.jcells; d.ncells = d.ijcells*d.kcells;
+    d.dx = g->dx; d.dy = g->dy; d.dxi = 1./g->dx; d.dyi = 1./g->dy;
+
+    const size_t nk = static_cast<size_t>(kcells);
+    c->h_z.assign(g->z, g->z + nk);         c->h_zh.assign(g->zh, g->zh + nk);
+    c->h_dz.assign(g->dz, g->dz + nk);      c->h_dzh.assign(g->dzh, g->dzh + nk);
+    c->h_dzi.assign(g->dzi, g->dzi + nk);   c->h_dzhi.assign(g->dzhi, g->dzhi + nk);
+    c->h_rhoref.assign(g->rhoref, g->rhoref + nk); c->h_rhorefh.assign(g->rhorefh, g->rhorefh + nk);
+
+    int rc = (precision == MHH_FP64) ? upload_profiles(c, c->pd) : upload_profiles(c, c->pf);
+    if (!rc) rc = mhh::check_hip(hipHostMalloc(reinterpret_cast<void**>(&c->h_red), sizeof(double)), "hipHostMalloc");
+    if (!rc) rc = mhh::ensure_reduction(c, 1024);
+    if (!rc) rc = mhh::pres_init(c);
+    if (rc)
+    {
+        std::string msg = last_error;
+        mhh_ctx_destroy(c);
+        last_error = msg;
+        return rc;
+    }
+    *out = c;
+    return 0;
+}
+
+int mhh_ctx_destroy(mhh_ctx* c)
+{
+    if (!c) return 0;
+    if (c->stream) (void)hipStreamSynchronize(c->stream); else (void)hipDeviceSynchronize();
+    free_prof(c->pd);
+    free_prof(c->pf);
+    mhh::pres_free(c);
+    if (c->mlen_d) (void)hipFree(c->mlen_d);
+    if (c->mlen_f) (void)hipFree(c->mlen_f);
+    if (c->red) (void)hipFree(c->red);
+    if (c->h_red) (void)hipHostFree(c->h_red);
+    for (auto& it : c->timers)
+        for (auto& pr : it.second.ev) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+    delete c;
+    return 0;
+}
+
+int mhh_fields_bind(mhh_ctx* c, const mhh_field_set* f)
+{
+    using mhh::fail;
+    if (!c || !f) return fail(MHH_EINVAL, "mhh_fields_bind: null argument");
+    if (!f->u || !f->v || !f->w || !f->ut || !f->vt || !f->wt)
+        return fail(MHH_EINVAL, "mhh_fields_bind: u, v, w, ut, vt, wt are required");
+    if (f->nscalars < 0 || f->nscalars > MHH_MAX_SCALARS)
+        return fail(MHH_EINVAL, "mhh_fields_bind: nscalars out of range");
+    for (int n = 0; n < f->nscalars; ++n)
+        if (!f->s[n] || !f->st[n]) return fail(MHH_EINVAL, "mhh_fields_bind: scalar or its tendency missing");
+    c->f = *f;
+    c->bound = true;
+    return 0;
+}
+
+int mhh_set_base_state(mhh_ctx* c, const double* thref, const double* threfh)
+{
+    if (!c || !thref || !threfh) return mhh::fail(MHH_EINVAL, "mhh_set_base_state: null argument");
+    const size_t nk = static_cast<size_t>(c->gd.kcells);
+    c->h_thref.assign(thref, thref + nk);
+    c->h_threfh.assign(threfh, threfh + nk);
+    if (c->precision == MHH_FP64)
+    {
+        if (c->pd.thref) { (void)hipFree(c->pd.thref); (void)hipFree(c->pd.threfh); }
+        MHH_CHECK(upload(&c->pd.thref, c->h_thref));
+        MHH_CHECK(upload(&c->pd.threfh, c->h_threfh));
+    }
+    else
+    {
+        if (c->pf.thref) { (void)hipFree(c->pf.thref); (void)hipFree(c->pf.threfh); }
+        MHH_CHECK(upload(&c->pf.thref, c->h_thref));
+        MHH_CHECK(upload(&c->pf.threfh, c->h_threfh));
+    }
+    c->have_base_state = true;
+    return 0;
+}
+
+int mhh_sync(mhh_ctx* c)
+{
+    if (!c) return mhh::fail(MHH_EINVAL, "mhh_sync: null context");
+    return mhh::check_hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+int mhh_device_alloc(size_t bytes, void** ptr)
+{
+    if (!ptr) return mhh::fail(MHH_EINVAL, "mhh_device_alloc: null output");
+    *ptr = nullptr;
+    if (bytes == 0) return 0;
+    MHH_CHECK(mhh::check_hip(hipMalloc(ptr, bytes), "hipMalloc"));
+    return mhh::check_hip(hipMemset(*ptr, 0, bytes), "hipMemset");
+}
+
+int mhh_device_free(void* ptr)
+{
+    if (!ptr) return 0;
+    return mhh::check_hip(hipFree(ptr), "hipFree");
+}
+
+int mhh_copy_to_device(void* dst, const void* src, size_t bytes)
+{
+    if (bytes && (!dst || !src)) return mhh::fail(MHH_EINVAL, "mhh_copy_to_device: null pointer");
+    return mhh::check_hip(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+
+int mhh_copy_to_host(void* dst, const void* src, size_t bytes)
+{
+    if (bytes && (!dst || !src)) return mhh::fail(MHH_EINVAL, "mhh_copy_to_host: null pointer");
+    return mhh::check_hip(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+}
+
+int mhh_device_synchronize(void)
+{
+    return mhh::check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+}
+
+int mhh_timing_enable(mhh_ctx* c, int on)
+{
+    if (!c) return mhh::fail(MHH_EINVAL, "null context");
+    c->timing = on != 0;
+    return 0;
+}
+
+int mhh_timing_reset(mhh_ctx* c)
+{
+    if (!c) return mhh::fail(MHH_EINVAL, "null context");
+    MHH_CHECK(mhh::check_hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
+    for (auto& it : c->timers) it.second.used = 0;
+    return 0;
+}
+
+int mhh_timing_query(mhh_ctx* c, const char* kernel, double* total_ms, int* count)
+{
+    if (!c || !kernel || !total_ms || !count) return mhh::fail(MHH_EINVAL, "mhh_timing_query: null argument");
+    MHH_CHECK(mhh::check_hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
+    *total_ms = 0.; *count = 0;
+    auto it = c->timers.find(kernel);
+    if (it == c->timers.end()) return 0;
+    for (int n = 0; n < it->second.used; ++n)
+    {
+        float ms = 0.f;
+        MHH_CHECK(mhh::check_hip(hipEventElapsedTime(&ms, it->second.ev[n].first, it->second.ev[n].second),
+                                 "hipEventElapsedTime"));
+        *total_ms += ms;
+    }
+    *count = it->second.used;
+    return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,587 @@
+// mhh_diff.hip -- Smagorinsky (smag2) eddy viscosity and stress divergence,
+// and constant-viscosity diffusion (diff_2).
+//
+// The viscosity kernel fuses strain2 (include/diff_kernels.h:34-142), N2
+// (src/thermo_dry.cxx:66-79) and the evisc closure (src/diff_smag2.cxx:47-269)
+// per point: S^2 and N^2 never go to memory, only evisc does.
+#include <cmath>
+
+#include "mhh_internal.h"
+#include "mhh_stencil.h"
+
+namespace
+{
+    using namespace mhh;
+
+    constexpr double dsmall = 1.e-9;   // include/constants.h:97
+    constexpr double kappa = 0.4;      // include/constants.h:28
+    constexpr double grav = 9.81;      // include/constants.h:29
+
+    __device__ __forceinline__ bool point(const Gd& g, int kfirst, int& i, int& j, int& k)
+    {
+        i = blockIdx.x*BX + threadIdx.x + g.istart;
+        j = blockIdx.y*BY + threadIdx.y + g.jstart;
+        k = blockIdx.z + kfirst;
+        return i < g.iend && j < g.jend;
+    }
+
+    template<typename TF>
+    struct Visc_args
+    {
+        TF* __restrict__ evisc;
+        const TF* __restrict__ u;
+        const TF* __restrict__ v;
+        const TF* __restrict__ w;
+        const TF* __restrict__ th;
+        const TF* __restrict__ dudz;
+        const TF* __restrict__ dvdz;
+        const TF* __restrict__ dbdz;
+        const TF* __restrict__ z0m;
+        const TF* __restrict__ z;
+        const TF* __restrict__ dzi;
+        const TF* __restrict__ dzhi;
+        const TF* __restrict__ thref;
+        const TF* __restrict__ mlen0;
+        TF dxi, dyi, tPr, visc, zsize;
+    };
+
+    // S^2 in the interior form (include/diff_kernels.h:103-136), dsmall added.
+    template<typename TF>
+    __device__ __forceinline__ TF strain2_interior(const TF* __restrict__ u, const TF* __restrict__ v,
+                                                   const TF* __restrict__ w, int ijk, int ii, int jj, int kk,
+                                                   TF dxi, TF dyi, TF dzik, TF dzhik, TF dzhikp)
+    {
+        TF s2 = TF(2.)*(
+               + sq((u[ijk+ii]-u[ijk])*dxi)
+               + sq((v[ijk+jj]-v[ijk])*dyi)
+               + sq((w[ijk+kk]-w[ijk])*dzik)
+               + TF(0.125)*sq((u[ijk      ]-u[ijk   -jj])*dyi  + (v[ijk      ]-v[ijk-ii   ])*dxi)
+               + TF(0.125)*sq((u[ijk+ii   ]-u[ijk+ii-jj])*dyi  + (v[ijk+ii   ]-v[ijk      ])*dxi)
+               + TF(0.125)*sq((u[ijk   +jj]-u[ijk      ])*dyi  + (v[ijk   +jj]-v[ijk-ii+jj])*dxi)
+               + TF(0.125)*sq((u[ijk+ii+jj]-u[ijk+ii   ])*dyi  + (v[ijk+ii+jj]-v[ijk   +jj])*dxi)
+               + TF(0.125)*sq((u[ijk      ]-u[ijk   -kk])*dzhik  + (w[ijk      ]-w[ijk-ii   ])*dxi)
+               + TF(0.125)*sq((u[ijk+ii   ]-u[ijk+ii-kk])*dzhik  + (w[ijk+ii   ]-w[ijk      ])*dxi)
+               + TF(0.125)*sq((u[ijk   +kk]-u[ijk      ])*dzhikp + (w[ijk   +kk]-w[ijk-ii+kk])*dxi)
+               + TF(0.125)*sq((u[ijk+ii+kk]-u[ijk+ii   ])*dzhikp + (w[ijk+ii+kk]-w[ijk   +kk])*dxi)
+               + TF(0.125)*sq((v[ijk      ]-v[ijk   -kk])*dzhik  + (w[ijk      ]-w[ijk-jj   ])*dyi)
+               + TF(0.125)*sq((v[ijk+jj   ]-v[ijk+jj-kk])*dzhik  + (w[ijk+jj   ]-w[ijk      ])*dyi)
+               + TF(0.125)*sq((v[ijk   +kk]-v[ijk      ])*dzhikp + (w[ijk   +kk]-w[ijk-jj+kk])*dyi)
+               + TF(0.125)*sq((v[ijk+jj+kk]-v[ijk+jj   ])*dzhikp + (w[ijk+jj+kk]-w[ijk   +kk])*dyi) );
+        s2 += TF(dsmall);
+        return s2;
+    }
+
+    // S^2 at kstart with the surface-model gradients (include/diff_kernels.h:57-98).
+    template<typename TF>
+    __device__ __forceinline__ TF strain2_surface(const TF* __restrict__ u, const TF* __restrict__ v,
+                                                  const TF* __restrict__ w, TF ugrad, TF vgrad,
+                                                  int ijk, int ii, int jj, int kk, TF dxi, TF dyi, TF dzik)
+    {
+        TF s2 = TF(2.)*(
+               + sq((u[ijk+ii]-u[ijk])*dxi)
+               + sq((v[ijk+jj]-v[ijk])*dyi)
+               + sq((w[ijk+kk]-w[ijk])*dzik)
+               + TF(0.125)*sq((u[ijk      ]-u[ijk   -jj])*dyi  + (v[ijk      ]-v[ijk-ii   ])*dxi)
+               + TF(0.125)*sq((u[ijk+ii   ]-u[ijk+ii-jj])*dyi  + (v[ijk+ii   ]-v[ijk      ])*dxi)
+               + TF(0.125)*sq((u[ijk   +jj]-u[ijk      ])*dyi  + (v[ijk   +jj]-v[ijk-ii+jj])*dxi)
+               + TF(0.125)*sq((u[ijk+ii+jj]-u[ijk+ii   ])*dyi  + (v[ijk+ii+jj]-v[ijk   +jj])*dxi)
+               + TF(0.5)*sq(ugrad)
+               + TF(0.125)*sq((w[ijk      ]-w[ijk-ii   ])*dxi)
+               + TF(0.125)*sq((w[ijk+ii   ]-w[ijk      ])*dxi)
+               + TF(0.125)*sq((w[ijk   +kk]-w[ijk-ii+kk])*dxi)
+               + TF(0.125)*sq((w[ijk+ii+kk]-w[ijk   +kk])*dxi)
+               + TF(0.5)*sq(vgrad)
+               + TF(0.125)*sq((w[ijk      ]-w[ijk-jj   ])*dyi)
+               + TF(0.125)*sq((w[ijk+jj   ]-w[ijk      ])*dyi)
+               + TF(0.125)*sq((w[ijk   +kk]-w[ijk-jj+kk])*dyi)
+               + TF(0.125)*sq((w[ijk+jj+kk]-w[ijk   +kk])*dyi) );
+        s2 += TF(dsmall);
+        return s2;
+    }
+
+    // Mason wall damping of the mixing length (src/diff_smag2.cxx:131-134, 229-232).
+    template<typename TF, int n_mason>
+    __device__ __forceinline__ TF mason(TF mlen0, TF zk, TF z0)
+    {
+        const TF n = TF(n_mason);
+        return pow(TF(1.)/(TF(1.)/pow(mlen0, n) + TF(1.)/(pow(TF(kappa)*(zk + z0), n))), TF(1.)/n);
+    }
+
+    // evisc for the buoyant case (calc_evisc, src/diff_smag2.cxx:148-269) or the
+    // neutral case (calc_evisc_neutral, :47-146), strain2 and N2 fused.
+    template<typename TF, bool sfc, bool thermo, bool sw_mason>
+    __global__ void __launch_bounds__(BX*BY) evisc_g(Visc_args<TF> a, Gd g)
+    {
+        int i, j, k;
+        if (!point(g, g.kstart, i, j, k)) return;
+        const int ii = 1, jj = g.icells, kk = g.ijcells;
+        const int ij = i + j*jj;
+        const int ijk = ij + k*kk;
+        const TF* __restrict__ u = a.u; const TF* __restrict__ v = a.v; const TF* __restrict__ w = a.w;
+
+        const bool bot = sfc && (k == g.kstart);
+        const TF s2 = bot ? strain2_surface(u, v, w, a.dudz[ij], a.dvdz[ij], ijk, ii, jj, kk, a.dxi, a.dyi, a.dzi[k])
+                          : strain2_interior(u, v, w, ijk, ii, jj, kk, a.dxi, a.dyi, a.dzi[k], a.dzhi[k], a.dzhi[k+1]);
+        TF ev;
+        if (thermo)
+        {
+            TF num;
+            if (bot)
+                num = a.dbdz[ij];
+            else
+                num = TF(grav)/a.thref[k]*TF(0.5)*(a.th[ijk+kk] - a.th[ijk-kk])*a.dzi[k];
+            TF RitPrratio = num / s2 / a.tPr;
+            RitPrratio = fmin(RitPrratio, TF(1.-dsmall));
+            TF mlen = a.mlen0[k];
+            if (sfc && sw_mason)
+                mlen = mason<TF, 2>(a.mlen0[k], a.z[k], a.z0m[ij]);
+            ev = (sfc ? sq(mlen) : mlen*mlen) * sqrt(s2) * sqrt(TF(1.)-RitPrratio);
+        }
+        else if (sfc)
+        {
+            TF mlen = a.mlen0[k];
+            if (sw_mason)
+                mlen = mason<TF, 1>(a.mlen0[k], a.z[k], a.z0m[ij]);
+            ev = sq(mlen) * sqrt(s2);
+        }
+        else
+        {
+            // van Driest damping from both walls (src/diff_smag2.cxx:77-98).
+            const TF A_vandriest = TF(26.);
+            const int kb = ij + g.kstart*kk, kt = ij + g.kend*kk;
+            const TF u_tau_bot = pow(sq(a.visc*(u[kb] - u[kb-kk])*a.dzhi[g.kstart])
+                                   + sq(a.visc*(v[kb] - v[kb-kk])*a.dzhi[g.kstart]), TF(0.25));
+            const TF u_tau_top = pow(sq(a.visc*(u[kt] - u[kt-kk])*a.dzhi[g.kend])
+                                   + sq(a.visc*(v[kt] - v[kt-kk])*a.dzhi[g.kend]), TF(0.25));
+            const TF fac_bot = TF(1.) - exp(-(        a.z[k] *u_tau_bot) / (A_vandriest*a.visc));
+            const TF fac_top = TF(1.) - exp(-((a.zsize-a.z[k])*u_tau_top) / (A_vandriest*a.visc));
+            const TF fac = fmin(fac_bot, fac_top);
+            ev = sq(fac * a.mlen0[k]) * sqrt(s2);
+        }
+        a.evisc[ijk] = ev;
+    }
+
+    // Mirror evisc over both walls (src/diff_smag2.cxx:193-205) for the interior
+    // columns; the cyclic update that follows fills the horizontal ghosts.
+    template<typename TF>
+    __global__ void mirror_walls_g(TF* __restrict__ evisc, Gd g)
+    {
+        const int i = blockIdx.x*blockDim.x + threadIdx.x + g.istart;
+        const int j = blockIdx.y*blockDim.y + threadIdx.y + g.jstart;
+        if (i >= g.iend || j >= g.jend) return;
+        const int ij = i + j*g.icells, kk = g.ijcells;
+        evisc[ij + (g.kstart-1)*kk] = evisc[ij + g.kstart*kk];
+        evisc[ij + g.kend*kk] = evisc[ij + (g.kend-1)*kk];
+    }
+
+    template<typename TF>
+    struct Diff_args
+    {
+        TF* __restrict__ at;
+        const TF* __restrict__ a;      // diffused scalar (diff_c)
+        const TF* __restrict__ u;
+        const TF* __restrict__ v;
+        const TF* __restrict__ w;
+        const TF* __restrict__ evisc;
+        const TF* __restrict__ fluxbot;
+        const TF* __restrict__ fluxtop;
+        const TF* __restrict__ dzi;
+        const TF* __restrict__ dzhi;
+        const TF* __restrict__ rho;
+        const TF* __restrict__ rhoh;
+        TF dxi, dyi, visc, tPr;
+    };
+
+    // diff_u (include/diff_kernels.h:144-240).
+    template<typename TF, bool sfc>
+    __global__ void __launch_bounds__(BX*BY) diff_u_g(Diff_args<TF> a, Gd g)
+    {
+        int i, j, k;
+        if (!point(g, g.kstart, i, j, k)) return;
+        const int ii = 1, jj = g.icells, kk = g.ijcells;
+        const int ij = i + j*jj, ijk = ij + k*kk;
+        const TF* __restrict__ u = a.u; const TF* __restrict__ v = a.v; const TF* __restrict__ w = a.w;
+        const TF* __restrict__ e = a.evisc;
+        const TF visc = a.visc, dxi = a.dxi, dyi = a.dyi;
+        const TF evisce = e[ijk   ] + visc;
+        const TF eviscw = e[ijk-ii] + visc;
+        const TF eviscn = TF(0.25)*(e[ijk-ii   ] + e[ijk   ] + e[ijk-ii+jj] + e[ijk+jj]) + visc;
+        const TF eviscs = TF(0.25)*(e[ijk-ii-jj] + e[ijk-jj] + e[ijk-ii   ] + e[ijk   ]) + visc;
+        const TF hor =
+                 + ( evisce*(u[ijk+ii]-u[ijk   ])*dxi
+                   - eviscw*(u[ijk   ]-u[ijk-ii])*dxi ) * TF(2.)*dxi
+                 + ( eviscn*((u[ijk+jj]-u[ijk   ])*dyi  + (v[ijk+jj]-v[ijk-ii+jj])*dxi)
+                   - eviscs*((u[ijk   ]-u[ijk-jj])*dyi  + (v[ijk   ]-v[ijk-ii   ])*dxi) ) * dyi;
+        TF ver;
+        if (sfc && k == g.kstart)
+        {
+            const TF evisct = TF(0.25)*(e[ijk-ii   ] + e[ijk   ] + e[ijk-ii+kk] + e[ijk+kk]) + visc;
+            ver = ( a.rhoh[k+1] * evisct*((u[ijk+kk]-u[ijk   ])* a.dzhi[k+1] + (w[ijk+kk]-w[ijk-ii+kk])*dxi)
+                  + a.rhoh[k  ] * a.fluxbot[ij] ) / a.rho[k] * a.dzi[k];
+        }
+        else if (sfc && k == g.kend-1)
+        {
+            const TF eviscb = TF(0.25)*(e[ijk-ii-kk] + e[ijk-kk] + e[ijk-ii   ] + e[ijk   ]) + visc;
+            ver = (- a.rhoh[k+1] * a.fluxtop[ij]
+                   - a.rhoh[k  ] * eviscb*((u[ijk   ]-u[ijk-kk])* a.dzhi[k] + (w[ijk   ]-w[ijk-ii   ])*dxi) ) / a.rho[k] * a.dzi[k];
+        }
+        else
+        {
+            const TF evisct = TF(0.25)*(e[ijk-ii   ] + e[ijk   ] + e[ijk-ii+kk] + e[ijk+kk]) + visc;
+            const TF eviscb = TF(0.25)*(e[ijk-ii-kk] + e[ijk-kk] + e[ijk-ii   ] + e[ijk   ]) + visc;
+            ver = ( a.rhoh[k+1] * evisct*((u[ijk+kk]-u[ijk   ])* a.dzhi[k+1] + (w[ijk+kk]-w[ijk-ii+kk])*dxi)
+                  - a.rhoh[k  ] * eviscb*((u[ijk   ]-u[ijk-kk])* a.dzhi[k  ] + (w[ijk   ]-w[ijk-ii   ])*dxi) ) / a.rho[k] * a.dzi[k];
+        }
+        a.at[ijk] += hor + ver;
+    }
+
+    // diff_v (include/diff_kernels.h:242-340).
+    template<typename TF, bool sfc>
+    __global__ void __launch_bounds__(BX*BY) diff_v_g(Diff_args<TF> a, Gd g)
+    {
+        int i, j, k;
+        if (!point(g, g.kstart, i, j, k)) return;
+        const int ii = 1, jj = g.icells, kk = g.ijcells;
+        const int ij = i + j*jj, ijk = ij + k*kk;
+        const TF* __restrict__ u = a.u; const TF* __restrict__ v = a.v; const TF* __restrict__ w = a.w;
+        const TF* __restrict__ e = a.evisc;
+        const TF visc = a.visc, dxi = a.dxi, dyi = a.dyi;
+        const TF evisce = TF(0.25)*(e[ijk   -jj] + e[ijk   ] + e[ijk+ii-jj] + e[ijk+ii]) + visc;
+        const TF eviscw = TF(0.25)*(e[ijk-ii-jj] + e[ijk-ii] + e[ijk   -jj] + e[ijk   ]) + visc;
+        const TF eviscn = e[ijk   ] + visc;
+        const TF eviscs = e[ijk-jj] + visc;
+        const TF hor =
+                 + ( evisce*((v[ijk+ii]-v[ijk   ])*dxi + (u[ijk+ii]-u[ijk+ii-jj])*dyi)
+                   - eviscw*((v[ijk   ]-v[ijk-ii])*dxi + (u[ijk   ]-u[ijk   -jj])*dyi) ) * dxi
+                 + ( eviscn*(v[ijk+jj]-v[ijk   ])*dyi
+                   - eviscs*(v[ijk   ]-v[ijk-jj])*dyi ) * TF(2.)*dyi;
+        TF ver;
+        if (sfc && k == g.kstart)
+        {
+            const TF evisct = TF(0.25)*(e[ijk   -jj] + e[ijk   ] + e[ijk+kk-jj] + e[ijk+kk]) + visc;
+            ver = ( a.rhoh[k+1] * evisct*((v[ijk+kk]-v[ijk   ])*a.dzhi[k+1] + (w[ijk+kk]-w[ijk-jj+kk])*dyi)
+                  + a.rhoh[k  ] * a.fluxbot[ij] ) / a.rho[k] * a.dzi[k];
+        }
+        else if (sfc && k == g.kend-1)
+        {
+            const TF eviscb = TF(0.25)*(e[ijk-kk-jj] + e[ijk-kk] + e[ijk   -jj] + e[ijk   ]) + visc;
+            ver = (- a.rhoh[k+1] * a.fluxtop[ij]
+                   - a.rhoh[k  ] * eviscb*((v[ijk   ]-v[ijk-kk])*a.dzhi[k] + (w[ijk   ]-w[ijk-jj   ])*dyi) ) / a.rho[k] * a.dzi[k];
+        }
+        else
+        {
+            const TF evisct = TF(0.25)*(e[ijk   -jj] + e[ijk   ] + e[ijk+kk-jj] + e[ijk+kk]) + visc;
+            const TF eviscb = TF(0.25)*(e[ijk-kk-jj] + e[ijk-kk] + e[ijk   -jj] + e[ijk   ]) + visc;
+            ver = ( a.rhoh[k+1] * evisct*((v[ijk+kk]-v[ijk   ])*a.dzhi[k+1] + (w[ijk+kk]-w[ijk-jj+kk])*dyi)
+                  - a.rhoh[k  ] * eviscb*((v[ijk   ]-v[ijk-kk])*a.dzhi[k  ] + (w[ijk   ]-w[ijk-jj   ])*dyi) ) / a.rho[k] * a.dzi[k];
+        }
+        a.at[ijk] += hor + ver;
+    }
+
+    // diff_w (include/diff_kernels.h:342-390), half levels kstart+1..kend-1.
+    template<typename TF>
+    __global__ void __launch_bounds__(BX*BY) diff_w_g(Diff_args<TF> a, Gd g)
+    {
+        int i, j, k;
+        if (!point(g, g.kstart+1, i, j, k)) return;
+        const int ii = 1, jj = g.icells, kk = g.ijcells;
+        const int ijk = i + j*jj + k*kk;
+        const TF* __restrict__ u = a.u; const TF* __restrict__ v = a.v; const TF* __restrict__ w = a.w;
+        const TF* __restrict__ e = a.evisc;
+        const TF visc = a.visc, dxi = a.dxi, dyi = a.dyi, dzhik = a.dzhi[k];
+        const TF evisce = TF(0.25)*(e[ijk   -kk] + e[ijk   ] + e[ijk+ii-kk] + e[ijk+ii]) + visc;
+        const TF eviscw = TF(0.25)*(e[ijk-ii-kk] + e[ijk-ii] + e[ijk   -kk] + e[ijk   ]) + visc;
+        const TF eviscn = TF(0.25)*(e[ijk   -kk] + e[ijk   ] + e[ijk+jj-kk] + e[ijk+jj]) + visc;
+        const TF eviscs = TF(0.25)*(e[ijk-jj-kk] + e[ijk-jj] + e[ijk   -kk] + e[ijk   ]) + visc;
+        const TF evisct = e[ijk   ] + visc;
+        const TF eviscb = e[ijk-kk] + visc;
+        a.at[ijk] +=
+                 + ( evisce*((w[ijk+ii]-w[ijk   ])*dxi + (u[ijk+ii]-u[ijk+ii-kk])*dzhik)
+                   - eviscw*((w[ijk   ]-w[ijk-ii])*dxi + (u[ijk   ]-u[ijk   -kk])*dzhik) ) * dxi
+                 + ( eviscn*((w[ijk+jj]-w[ijk   ])*dyi + (v[ijk+jj]-v[ijk+jj-kk])*dzhik)
+                   - eviscs*((w[ijk   ]-w[ijk-jj])*dyi + (v[ijk   ]-v[ijk   -kk])*dzhik) ) * dyi
+                 + ( a.rho[k  ] * evisct*(w[ijk+kk]-w[ijk   ])*a.dzi[k  ]
+                   - a.rho[k-1] * eviscb*(w[ijk   ]-w[ijk-kk])*a.dzi[k-1] ) / a.rhoh[k] * TF(2.)*dzhik;
+    }
+
+    // diff_c (include/diff_kernels.h:392-484).
+    template<typename TF, bool sfc>
+    __global__ void __launch_bounds__(BX*BY) diff_c_g(Diff_args<TF> a, Gd g)
+    {
+        int i, j, k;
+        if (!point(g, g.kstart, i, j, k)) return;
+        const int ii = 1, jj = g.icells, kk = g.ijcells;
+        const int ij = i + j*jj, ijk = ij + k*kk;
+        const TF* __restrict__ s = a.a;
+        const TF* __restrict__ e = a.evisc;
+        const TF visc = a.visc, tPr_i = TF(1)/a.tPr;
+        const TF dxidxi = a.dxi*a.dxi, dyidyi = a.dyi*a.dyi;
+        const TF evisce = TF(0.5)*(e[ijk   ]+e[ijk+ii]) * tPr_i + visc;
+        const TF eviscw = TF(0.5)*(e[ijk-ii]+e[ijk   ]) * tPr_i + visc;
+        const TF eviscn = TF(0.5)*(e[ijk   ]+e[ijk+jj]) * tPr_i + visc;
+        const TF eviscs = TF(0.5)*(e[ijk-jj]+e[ijk   ]) * tPr_i + visc;
+        const TF hor =
+                 + ( evisce*(s[ijk+ii]-s[ijk   ])
+                   - eviscw*(s[ijk   ]-s[ijk-ii]) ) * dxidxi
+                 + ( eviscn*(s[ijk+jj]-s[ijk   ])
+                   - eviscs*(s[ijk   ]-s[ijk-jj]) ) * dyidyi;
+        TF ver;
+        if (sfc && k == g.kstart)
+        {
+            const TF evisct = TF(0.5)*(e[ijk   ]+e[ijk+kk]) * tPr_i + visc;
+            ver = ( a.rhoh[k+1] * evisct*(s[ijk+kk]-s[ijk   ])*a.dzhi[k+1]
+                  + a.rhoh[k  ] * a.fluxbot[ij] ) / a.rho[k] * a.dzi[k];
+        }
+        else if (sfc && k == g.kend-1)
+        {
+            const TF eviscb = TF(0.5)*(e[ijk-kk]+e[ijk   ]) * tPr_i + visc;
+            ver = (-a.rhoh[k+1] * a.fluxtop[ij]
+                   - a.rhoh[k  ] * eviscb*(s[ijk   ]-s[ijk-kk])*a.dzhi[k] ) / a.rho[k] * a.dzi[k];
+        }
+        else
+        {
+            const TF evisct = TF(0.5)*(e[ijk   ]+e[ijk+kk]) * tPr_i + visc;
+            const TF eviscb = TF(0.5)*(e[ijk-kk]+e[ijk   ]) * tPr_i + visc;
+            ver = ( a.rhoh[k+1] * evisct*(s[ijk+kk]-s[ijk   ])*a.dzhi[k+1]
+                  - a.rhoh[k  ] * eviscb*(s[ijk   ]-s[ijk-kk])*a.dzhi[k]  ) / a.rho[k] * a.dzi[k];
+        }
+        a.at[ijk] += hor + ver;
+    }
+
+    // calc_dnmul (include/diff_kernels.h:486-511), per-block maxima.
+    template<typename TF>
+    __global__ void __launch_bounds__(BX*BY) dnmul_g(const TF* __restrict__ evisc, const TF* __restrict__ dzi,
+                                                     TF dxidxi, TF dyidyi, TF tPrfac_i, Gd g, double* __restrict__ red)
+    {
+        __shared__ double sm[BX*BY];
+        const int tid = threadIdx.x + threadIdx.y*BX;
+        int i, j, k;
+        double m = 0.;
+        if (point(g, g.kstart, i, j, k))
+        {
+            const int ijk = i + j*g.icells + k*g.ijcells;
+            m = static_cast<double>(fabs(evisc[ijk]*tPrfac_i*(dxidxi + dyidyi + dzi[k]*dzi[k])));
+        }
+        sm[tid] = m;
+        __syncthreads();
+        for (int s = BX*BY/2; s > 0; s >>= 1)
+        {
+            if (tid < s) sm[tid] = fmax(sm[tid], sm[tid + s]);
+            __syncthreads();
+        }
+        if (tid == 0)
+            red[1 + blockIdx.x + gridDim.x*(blockIdx.y + gridDim.y*blockIdx.z)] = sm[0];
+    }
+
+    // Diff_2 (src/diff_2.cxx:39-85): constant viscosity Laplacian.
+    template<typename TF>
+    __global__ void __launch_bounds__(BX*BY) diff_2_c_g(TF* __restrict__ at, const TF* __restrict__ a, TF visc,
+                                                        const TF* __restrict__ dzi, const TF* __restrict__ dzhi,
+                                                        TF dxidxi, TF dyidyi, Gd g)
+    {
+        int i, j, k;
+        if (!point(g, g.kstart, i, j, k)) return;
+        const int ii = 1, jj = g.icells, kk = g.ijcells;
+        const int ijk = i + j*jj + k*kk;
+        at[ijk] += visc * (
+                + ( (a[ijk+ii] - a[ijk   ])
+                  - (a[ijk   ] - a[ijk-ii]) ) * dxidxi
+                + ( (a[ijk+jj] - a[ijk   ])
+                  - (a[ijk   ] - a[ijk-jj]) ) * dyidyi
+                + ( (a[ijk+kk] - a[ijk   ]) * dzhi[k+1]
+                  - (a[ijk   ] - a[ijk-kk]) * dzhi[k]   ) * dzi[k] );
+    }
+
+    template<typename TF>
+    __global__ void __launch_bounds__(BX*BY) diff_2_w_g(TF* __restrict__ wt, const TF* __restrict__ w, TF visc,
+                                                        const TF* __restrict__ dzi, const TF* __restrict__ dzhi,
+                                                        TF dxidxi, TF dyidyi, Gd g)
+    {
+        int i, j, k;
+        if (!point(g, g.kstart+1, i, j, k)) return;
+        const int ii = 1, jj = g.icells, kk = g.ijcells;
+        const int ijk = i + j*jj + k*kk;
+        wt[ijk] += visc * (
+                + ( (w[ijk+ii] - w[ijk   ])
+                  - (w[ijk   ] - w[ijk-ii]) ) * dxidxi
+                + ( (w[ijk+jj] - w[ijk   ])
+                  - (w[ijk   ] - w[ijk-jj]) ) * dyidyi
+                + ( (w[ijk+kk] - w[ijk   ]) * dzi[k]
+                  - (w[ijk   ] - w[ijk-kk]) * dzi[k-1] ) * dzhi[k] );
+    }
+
+    template<typename TF> TF*& mlen_ptr(mhh_ctx* c);
+    template<> double*& mlen_ptr<double>(mhh_ctx* c) { return c->mlen_d; }
+    template<> float*& mlen_ptr<float>(mhh_ctx* c) { return c->mlen_f; }
+
+    template<typename TF>
+    int ensure_mlen(mhh_ctx* c, double cs)
+    {
+        TF*& dev = mlen_ptr<TF>(c);
+        if (dev && c->mlen_cs == cs) return 0;
+        const Gd& g = c->gd;
+        // mlen0 = cs*(dx*dy*dz)^(1/3) in the element type (src/diff_smag2.cxx:163, 214, 240).
+        std::vector<TF> h(g.kcells, TF(0));
+        for (int k = g.kstart; k < g.kend; ++k)
+            h[k] = TF(cs)*std::pow(TF(g.dx)*TF(g.dy)*TF(c->h_dz[k]), TF(1./3.));
+        if (!dev)
+            MHH_CHECK(check_hip(hipMalloc(&dev, sizeof(TF)*g.kcells), "hipMalloc(mlen)"));
+        MHH_CHECK(check_hip(hipMemcpy(dev, h.data(), sizeof(TF)*g.kcells, hipMemcpyHostToDevice), "hipMemcpy(mlen)"));
+        c->mlen_cs = cs;
+        return 0;
+    }
+
+    template<typename TF>
+    const TF* dptr(void* p) { return static_cast<const TF*>(p); }
+
+    template<typename TF>
+    int viscosity(mhh_ctx* c, const Prof<TF>& p, const mhh_smag_params& par)
+    {
+        const Gd& g = c->gd;
+        if (par.thermo && (c->f.nscalars < 1 || !p.thref))
+            return fail(MHH_ESTATE, "smag2 with thermo needs scalar 0 (th) and mhh_set_base_state()");
+        if (par.surface_model && (!c->f.z0m || !c->f.dudz || !c->f.dvdz || (par.thermo && !c->f.dbdz)))
+            return fail(MHH_ESTATE, "smag2 with a surface model needs dudz, dvdz, dbdz and z0m bound");
+        MHH_CHECK(ensure_mlen<TF>(c, par.cs));
+
+        Visc_args<TF> a;
+        a.evisc = static_cast<TF*>(c->f.evisc);
+        a.u = dptr<TF>(c->f.u); a.v = dptr<TF>(c->f.v); a.w = dptr<TF>(c->f.w);
+        a.th = par.thermo ? dptr<TF>(c->f.s[0]) : nullptr;
+        a.dudz = dptr<TF>(c->f.dudz); a.dvdz = dptr<TF>(c->f.dvdz);
+        a.dbdz = dptr<TF>(c->f.dbdz); a.z0m = dptr<TF>(c->f.z0m);
+        a.z = p.z; a.dzi = p.dzi; a.dzhi = p.dzhi; a.thref = p.thref;
+        a.mlen0 = mlen_ptr<TF>(c);
+        a.dxi = TF(1.)/TF(g.dx); a.dyi = TF(1.)/TF(g.dy);
+        a.tPr = TF(par.tPr); a.visc = TF(par.visc);
+        a.zsize = TF(c->h_zh[g.kend]);
+
+        const dim3 grd = grid3(g, g.kstart, g.kend), blk(BX, BY);
+        const bool sfc = par.surface_model != 0, th = par.thermo != 0, ms = par.swmason != 0;
+        if (!sfc && th)        { mhh::Timer _t(c, "evisc"); evisc_g<TF, false, true,  false><<<grd, blk, 0, c->stream>>>(a, g); }
+        else if (!sfc && !th)  { mhh::Timer _t(c, "evisc"); evisc_g<TF, false, false, false><<<grd, blk, 0, c->stream>>>(a, g); }
+        else if (th && ms)     { mhh::Timer _t(c, "evisc"); evisc_g<TF, true,  true,  true ><<<grd, blk, 0, c->stream>>>(a, g); }
+        else if (th && !ms)    { mhh::Timer _t(c, "evisc"); evisc_g<TF, true,  true,  false><<<grd, blk, 0, c->stream>>>(a, g); }
+        else if (ms)           { mhh::Timer _t(c, "evisc"); evisc_g<TF, true,  false, true ><<<grd, blk, 0, c->stream>>>(a, g); }
+        else                   { mhh::Timer _t(c, "evisc"); evisc_g<TF, true,  false, false><<<grd, blk, 0, c->stream>>>(a, g); }
+        MHH_CHECK(check_launch("evisc"));
+
+        if (!sfc)
+        {
+            const dim3 b2(64, 4), g2((g.itot + 63)/64, (g.jtot + 3)/4);
+            { mhh::Timer _t(c, "mirror_walls"); mirror_walls_g<TF><<<g2, b2, 0, c->stream>>>(a.evisc, g); }
+            MHH_CHECK(check_launch("mirror_walls"));
+        }
+        return mhh_boundary_cyclic(c, c->f.evisc, MHH_EDGE_BOTH);
+    }
+
+    template<typename TF>
+    int diffuse(mhh_ctx* c, const Prof<TF>& p, const mhh_smag_params& par)
+    {
+        const Gd& g = c->gd;
+        const bool sfc = par.surface_model != 0;
+        if (sfc)
+        {
+            if (!c->f.u_fluxbot || !c->f.u_fluxtop || !c->f.v_fluxbot || !c->f.v_fluxtop)
+                return fail(MHH_ESTATE, "smag2 with a surface model needs the u and v flux planes bound");
+            for (int n = 0; n < c->f.nscalars; ++n)
+                if (!c->f.s_fluxbot[n] || !c->f.s_fluxtop[n])
+                    return fail(MHH_ESTATE, "smag2 with a surface model needs the scalar flux planes bound");
+        }
+        Diff_args<TF> a{};
+        a.u = dptr<TF>(c->f.u); a.v = dptr<TF>(c->f.v); a.w = dptr<TF>(c->f.w);
+        a.evisc = dptr<TF>(c->f.evisc);
+        a.dzi = p.dzi; a.dzhi = p.dzhi; a.rho = p.rhoref; a.rhoh = p.rhorefh;
+        a.dxi = TF(1.)/TF(g.dx); a.dyi = TF(1.)/TF(g.dy);
+        a.visc = TF(par.visc); a.tPr = TF(par.tPr);
+        const dim3 blk(BX, BY);
+
+        a.at = static_cast<TF*>(c->f.ut); a.fluxbot = dptr<TF>(c->f.u_fluxbot); a.fluxtop = dptr<TF>(c->f.u_fluxtop);
+        if (sfc) { mhh::Timer _t(c, "diff_u"); diff_u_g<TF, true><<<grid3(g, g.kstart, g.kend), blk, 0, c->stream>>>(a, g); }
+        else     { mhh::Timer _t(c, "diff_u"); diff_u_g<TF, false><<<grid3(g, g.kstart, g.kend), blk, 0, c->stream>>>(a, g); }
+
+        a.at = static_cast<TF*>(c->f.vt); a.fluxbot = dptr<TF>(c->f.v_fluxbot); a.fluxtop = dptr<TF>(c->f.v_fluxtop);
+        if (sfc) { mhh::Timer _t(c, "diff_v"); diff_v_g<TF, true><<<grid3(g, g.kstart, g.kend), blk, 0, c->stream>>>(a, g); }
+        else     { mhh::Timer _t(c, "diff_v"); diff_v_g<TF, false><<<grid3(g, g.kstart, g.kend), blk, 0, c->stream>>>(a, g); }
+
+        a.at = static_cast<TF*>(c->f.wt);
+        { mhh::Timer _t(c, "diff_w"); diff_w_g<TF><<<grid3(g, g.kstart+1, g.kend), blk, 0, c->stream>>>(a, g); }
+
+        for (int n = 0; n < c->f.nscalars; ++n)
+        {
+            a.at = static_cast<TF*>(c->f.st[n]); a.a = dptr<TF>(c->f.s[n]);
+            a.fluxbot = dptr<TF>(c->f.s_fluxbot[n]); a.fluxtop = dptr<TF>(c->f.s_fluxtop[n]);
+            a.visc = TF(par.svisc[n]);
+            if (sfc) { mhh::Timer _t(c, "diff_c"); diff_c_g<TF, true><<<grid3(g, g.kstart, g.kend), blk, 0, c->stream>>>(a, g); }
+            else     { mhh::Timer _t(c, "diff_c"); diff_c_g<TF, false><<<grid3(g, g.kstart, g.kend), blk, 0, c->stream>>>(a, g); }
+        }
+        return check_launch("diff_smag2");
+    }
+
+    template<typename TF>
+    int dnmul(mhh_ctx* c, const Prof<TF>& p, double tPr, double* out)
+    {
+        const Gd& g = c->gd;
+        const dim3 grd = grid3(g, g.kstart, g.kend);
+        const int nb = static_cast<int>(grd.x*grd.y*grd.z);
+        MHH_CHECK(ensure_reduction(c, nb));
+        const TF dxi = TF(1.)/TF(g.dx), dyi = TF(1.)/TF(g.dy);
+        { mhh::Timer _t(c, "dnmul"); dnmul_g<TF><<<grd, dim3(BX, BY), 0, c->stream>>>(dptr<TF>(c->f.evisc), p.dzi,
+                TF(1.)/(TF(g.dx)*TF(g.dx)), TF(1.)/(TF(g.dy)*TF(g.dy)), TF(1)/std::min(TF(1.), TF(tPr)), g, c->red); }
+        (void)dxi; (void)dyi;
+        MHH_CHECK(check_launch("dnmul"));
+        return reduce_max_partials(c, nb, out);
+    }
+
+    template<typename TF>
+    int diff_2(mhh_ctx* c, const Prof<TF>& p, double visc, const double* svisc)
+    {
+        const Gd& g = c->gd;
+        const TF dxidxi = TF(1/(g.dx*g.dx)), dyidyi = TF(1/(g.dy*g.dy));
+        const dim3 blk(BX, BY);
+        { mhh::Timer _t(c, "diff_2"); diff_2_c_g<TF><<<grid3(g, g.kstart, g.kend), blk, 0, c->stream>>>(static_cast<TF*>(c->f.ut), dptr<TF>(c->f.u),
+                TF(visc), p.dzi, p.dzhi, dxidxi, dyidyi, g); }
+        { mhh::Timer _t(c, "diff_2"); diff_2_c_g<TF><<<grid3(g, g.kstart, g.kend), blk, 0, c->stream>>>(static_cast<TF*>(c->f.vt), dptr<TF>(c->f.v),
+                TF(visc), p.dzi, p.dzhi, dxidxi, dyidyi, g); }
+        { mhh::Timer _t(c, "diff_2"); diff_2_w_g<TF><<<grid3(g, g.kstart+1, g.kend), blk, 0, c->stream>>>(static_cast<TF*>(c->f.wt), dptr<TF>(c->f.w),
+                TF(visc), p.dzi, p.dzhi, dxidxi, dyidyi, g); }
+        for (int n = 0; n < c->f.nscalars; ++n)
+            { mhh::Timer _t(c, "diff_2"); diff_2_c_g<TF><<<grid3(g, g.kstart, g.kend), blk, 0, c->stream>>>(static_cast<TF*>(c->f.st[n]),
+                    dptr<TF>(c->f.s[n]), TF(svisc ? svisc[n] : visc), p.dzi, p.dzhi, dxidxi, dyidyi, g); }
+        return check_launch("diff_2");
+    }
+}
+
+extern "C" {
+
+int mhh_diff_smag2_exec_viscosity(mhh_ctx* c, const mhh_smag_params* par)
+{
+    MHH_REQUIRE_BOUND(c);
+    if (!par) return mhh::fail(MHH_EINVAL, "null smag params");
+    if (!c->f.evisc) return mhh::fail(MHH_ESTATE, "evisc field is not bound");
+    return c->precision == MHH_FP64 ? viscosity<double>(c, c->pd, *par) : viscosity<float>(c, c->pf, *par);
+}
+
+int mhh_diff_smag2_exec(mhh_ctx* c, const mhh_smag_params* par)
+{
+    MHH_REQUIRE_BOUND(c);
+    if (!par) return mhh::fail(MHH_EINVAL, "null smag params");
+    if (!c->f.evisc) return mhh::fail(MHH_ESTATE, "evisc field is not bound");
+    return c->precision == MHH_FP64 ? diffuse<double>(c, c->pd, *par) : diffuse<float>(c, c->pf, *par);
+}
+
+int mhh_diff_smag2_dnmul(mhh_ctx* c, double tPr, double* out)
+{
+    MHH_REQUIRE_BOUND(c);
+    if (!out) return mhh::fail(MHH_EINVAL, "null output");
+    if (!c->f.evisc) return mhh::fail(MHH_ESTATE, "evisc field is not bound");
+    return c->precision == MHH_FP64 ? dnmul<double>(c, c->pd, tPr, out) : dnmul<float>(c, c->pf, tPr, out);
+}
+
+int mhh_diff_2_exec(mhh_ctx* c, double visc, const double* svisc)
+{
+    MHH_REQUIRE_BOUND(c);
+    return c->precision == MHH_FP64 ? diff_2<double>(c, c->pd, visc, svisc) : diff_2<float>(c, c->pf, visc, svisc);
+}
+
+} // extern "C"

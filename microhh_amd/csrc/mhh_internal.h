@@ -1,0 +1,120 @@
+// mhh_internal.h -- shared definitions of the gfx950 kernels and the C ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "microhh_amd.h"
+
+namespace mhh
+{
+    // Loop bounds and strides passed by value to every kernel (Grid_data,
+    // include/grid.h:49-138, serial share: imax = itot, jmax = jtot).
+    struct Gd
+    {
+        int itot, jtot, ktot;
+        int igc, jgc, kgc;
+        int istart, iend, jstart, jend, kstart, kend;
+        int icells, jcells, kcells, ijcells, ncells;
+        double dx, dy, dxi, dyi;
+    };
+
+    // Device copies of the 1-D grid and base-state profiles in the element type.
+    template<typename TF>
+    struct Prof
+    {
+        TF* z = nullptr; TF* zh = nullptr; TF* dz = nullptr; TF* dzh = nullptr;
+        TF* dzi = nullptr; TF* dzhi = nullptr; TF* rhoref = nullptr; TF* rhorefh = nullptr;
+        TF* thref = nullptr; TF* threfh = nullptr;
+    };
+
+    // Pressure-solver tables (Pres_2::set_values, src/pres_2.cxx:124-153) and
+    // FFT twiddles.
+    template<typename TF>
+    struct Pres_tables
+    {
+        TF* bmati = nullptr;    // itot
+        TF* bmatj = nullptr;    // jtot
+        TF* a = nullptr;        // kmax
+        TF* c = nullptr;        // kmax
+        double* twx = nullptr;  // itot complex roots exp(-2 pi i t / itot), (re, im) pairs
+        double* twy = nullptr;  // jtot complex roots
+    };
+}
+
+struct mhh_ctx
+{
+    int precision = MHH_FP64;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    mhh::Gd gd{};
+    std::vector<double> h_z, h_zh, h_dz, h_dzh, h_dzi, h_dzhi, h_rhoref, h_rhorefh, h_thref, h_threfh;
+    mhh::Prof<double> pd;
+    mhh::Prof<float> pf;
+    mhh::Pres_tables<double> td;
+    mhh::Pres_tables<float> tf;
+    bool have_base_state = false;
+    mhh_field_set f{};
+    bool bound = false;
+    // Reduction scratch: per-block partial maxima and one result slot.
+    double* red = nullptr;
+    int red_capacity = 0;
+    double* h_red = nullptr;  // pinned host slot for the reduction result
+    // Smagorinsky length per level, cs*(dx*dy*dz[k])^(1/3), computed on the host
+    // as the reference does (src/diff_smag2.cxx:163) and cached per cs.
+    double mlen_cs = -1.;
+    double* mlen_d = nullptr;
+    float* mlen_f = nullptr;
+    // Kernel timing: per kernel name, a pool of (start, stop) event pairs and
+    // the number recorded since the last reset.
+    bool timing = false;
+    struct Event_pool { std::vector<std::pair<hipEvent_t, hipEvent_t>> ev; int used = 0; };
+    std::map<std::string, Event_pool> timers;
+};
+
+namespace mhh
+{
+    void set_error(const std::string& msg);
+    int fail(int code, const std::string& msg);
+    int check_launch(const char* what);
+    int check_hip(hipError_t e, const char* what);
+    bool fits_int(long long v);
+
+    // Rows of blocks in k for the 3-D point kernels.
+    constexpr int BX = 64;
+    constexpr int BY = 4;
+
+    inline dim3 grid3(const Gd& g, int kfirst, int klast)
+    {
+        return dim3((g.itot + BX - 1)/BX, (g.jtot + BY - 1)/BY, klast > kfirst ? klast - kfirst : 0);
+    }
+
+    // Maximum over a device array of per-block partial maxima, returned on the host.
+    int reduce_max_partials(mhh_ctx* ctx, int npartials, double* out);
+    int ensure_reduction(mhh_ctx* ctx, int nblocks);
+
+    // Brackets the launches issued during its lifetime with HIP events on the
+    // context stream when timing is enabled.
+    class Timer
+    {
+        public:
+            Timer(mhh_ctx* c, const char* name);
+            ~Timer();
+        private:
+            mhh_ctx* c;
+            hipEvent_t stop = nullptr;
+    };
+
+    // Pressure solver setup (mhh_pres.hip).
+    int pres_init(mhh_ctx* ctx);
+    void pres_free(mhh_ctx* ctx);
+}
+
+#define MHH_CHECK(call) do { int _rc = (call); if (_rc) return _rc; } while (0)
+#define MHH_REQUIRE_BOUND(ctx) do { \
+    if (!(ctx)) return mhh::fail(MHH_EINVAL, "null context"); \
+    if (!(ctx)->bound) return mhh::fail(MHH_ESTATE, "fields are not bound (call mhh_fields_bind)"); } while (0)

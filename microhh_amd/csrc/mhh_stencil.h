@@ -1,0 +1,179 @@
+// mhh_stencil.h -- interpolation and flux building blocks of the 2nd-order grid
+// and of the 2i5 scheme (coefficients of include/finite_difference.h:37-156).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace mhh
+{
+    template<typename TF> __device__ __forceinline__ TF i2(TF a, TF b) { return TF(0.5)*(a + b); }
+
+    // 4th-order interpolation and 3rd-order dissipation (finite_difference.h:107-121).
+    template<typename TF> __device__ __forceinline__ TF i4ws(TF a, TF b, TF c, TF d)
+    { return TF(7./12.)*(b + c) - TF(1./12.)*(a + d); }
+    template<typename TF> __device__ __forceinline__ TF i3ws(TF a, TF b, TF c, TF d)
+    { return TF(3./12.)*(c - b) - TF(1./12.)*(d - a); }
+
+    // 6th-order interpolation and 5th-order dissipation (finite_difference.h:131-154).
+    template<typename TF> __device__ __forceinline__ TF i6ws(TF a, TF b, TF c, TF d, TF e, TF f)
+    { return TF(37./60.)*(c + d) - TF(8./60.)*(b + e) + TF(1./60.)*(a + f); }
+    template<typename TF> __device__ __forceinline__ TF i5ws(TF a, TF b, TF c, TF d, TF e, TF f)
+    { return TF(10./60.)*(d - c) - TF(5./60.)*(e - b) + TF(1./60.)*(f - a); }
+
+    template<typename TF> __device__ __forceinline__ TF sq(TF a) { return a*a; }
+
+    // The two parts (centred, upwind) of a 2i5 horizontal flux divergence of s
+    // along a direction with stride st, advected by face velocities fp (upper)
+    // and fm (lower).  Callers add x then y terms left to right.
+    template<typename TF>
+    struct Hterms { TF c, u; };
+
+    template<typename TF>
+    __device__ __forceinline__ Hterms<TF> hflux_2i5(TF fp, TF fm, const TF* __restrict__ s, int ijk, int st, TF di)
+    {
+        const TF sm3 = s[ijk-3*st], sm2 = s[ijk-2*st], sm1 = s[ijk-st], s0 = s[ijk], sp1 = s[ijk+st],
+                 sp2 = s[ijk+2*st], sp3 = s[ijk+3*st];
+        Hterms<TF> t;
+        t.c = - ( fp * i6ws(sm2, sm1, s0, sp1, sp2, sp3)
+                - fm * i6ws(sm3, sm2, sm1, s0, sp1, sp2) ) * di;
+        t.u =   ( fabs(fp) * i5ws(sm2, sm1, s0, sp1, sp2, sp3)
+                - fabs(fm) * i5ws(sm3, sm2, sm1, s0, sp1, sp2) ) * di;
+        return t;
+    }
+
+    template<typename TF>
+    __device__ __forceinline__ TF hsum(Hterms<TF> x, Hterms<TF> y) { return ((x.c + x.u) + y.c) + y.u; }
+
+    // Level classes of the 2i5 vertical stencil for cell-centred fields:
+    // 0=kstart, 1=kstart+1, 2=kstart+2, 3=interior, 4=kend-3, 5=kend-2, 6=kend-1
+    // (src/advec_2i5.cxx:215-299).
+    __device__ __forceinline__ int level_class_c(int k, int ks, int ke)
+    {
+        if (k == ks) return 0;
+        if (k == ks+1) return 1;
+        if (k == ks+2) return 2;
+        if (k == ke-1) return 6;
+        if (k == ke-2) return 5;
+        if (k == ke-3) return 4;
+        return 3;
+    }
+
+    // Vertical 2i5 flux divergence of a cell-centred field s: wp/wm are the
+    // face velocities, rp/rm the face densities, rc the cell density and dzc the
+    // inverse cell height.
+    template<typename TF>
+    __device__ __forceinline__ TF vflux_2i5_c(int lvl, TF wp, TF wm, TF rp, TF rm, TF rc, TF dzc,
+                                              const TF* __restrict__ s, int ijk, int kk)
+    {
+        switch (lvl)
+        {
+            case 0:
+                return - ( rp * wp * i2(s[ijk], s[ijk+kk]) ) / rc * dzc;
+            case 1:
+            {
+                const TF a = s[ijk-kk], b = s[ijk], c = s[ijk+kk], d = s[ijk+2*kk];
+                return - ( rp * wp * i4ws(a, b, c, d)
+                         - rm * wm * i2(a, b) ) / rc * dzc
+                       + ( rp * fabs(wp) * i3ws(a, b, c, d) ) / rc * dzc;
+            }
+            case 2:
+            {
+                const TF m2 = s[ijk-2*kk], m1 = s[ijk-kk], s0 = s[ijk], p1 = s[ijk+kk], p2 = s[ijk+2*kk], p3 = s[ijk+3*kk];
+                return - ( rp * wp * i6ws(m2, m1, s0, p1, p2, p3)
+                         - rm * wm * i4ws(m2, m1, s0, p1) ) / rc * dzc
+                       + ( rp * fabs(wp) * i5ws(m2, m1, s0, p1, p2, p3)
+                         - rm * fabs(wm) * i3ws(m2, m1, s0, p1) ) / rc * dzc;
+            }
+            case 3:
+            {
+                const TF m3 = s[ijk-3*kk], m2 = s[ijk-2*kk], m1 = s[ijk-kk], s0 = s[ijk], p1 = s[ijk+kk],
+                         p2 = s[ijk+2*kk], p3 = s[ijk+3*kk];
+                return - ( rp * wp * i6ws(m2, m1, s0, p1, p2, p3)
+                         - rm * wm * i6ws(m3, m2, m1, s0, p1, p2) ) / rc * dzc
+                       + ( rp * fabs(wp) * i5ws(m2, m1, s0, p1, p2, p3)
+                         - rm * fabs(wm) * i5ws(m3, m2, m1, s0, p1, p2) ) / rc * dzc;
+            }
+            case 4:
+            {
+                const TF m3 = s[ijk-3*kk], m2 = s[ijk-2*kk], m1 = s[ijk-kk], s0 = s[ijk], p1 = s[ijk+kk], p2 = s[ijk+2*kk];
+                return - ( rp * wp * i4ws(m1, s0, p1, p2)
+                         - rm * wm * i6ws(m3, m2, m1, s0, p1, p2) ) / rc * dzc
+                       + ( rp * fabs(wp) * i3ws(m1, s0, p1, p2)
+                         - rm * fabs(wm) * i5ws(m3, m2, m1, s0, p1, p2) ) / rc * dzc;
+            }
+            case 5:
+            {
+                const TF m2 = s[ijk-2*kk], m1 = s[ijk-kk], s0 = s[ijk], p1 = s[ijk+kk];
+                return - ( rp * wp * i2(s0, p1)
+                         - rm * wm * i4ws(m2, m1, s0, p1) ) / rc * dzc
+                       - ( rm * fabs(wm) * i3ws(m2, m1, s0, p1) ) / rc * dzc;
+            }
+            default:
+                return - ( -rm * wm * i2(s[ijk-kk], s[ijk]) ) / rc * dzc;
+        }
+    }
+
+    // Vertical 2i5 flux divergence of w at half level k (src/advec_2i5.cxx:501-578):
+    // 0=kstart+1, 1=kstart+2, 2=interior, 3=kend-2, 4=kend-1.
+    __device__ __forceinline__ int level_class_w(int k, int ks, int ke)
+    {
+        if (k == ks+1) return 0;
+        if (k == ks+2) return 1;
+        if (k == ke-1) return 4;
+        if (k == ke-2) return 3;
+        return 2;
+    }
+
+    template<typename TF>
+    __device__ __forceinline__ TF vflux_2i5_w(int lvl, TF rp, TF rm, TF rc, TF dzc,
+                                              const TF* __restrict__ w, int ijk, int kk)
+    {
+        switch (lvl)
+        {
+            case 0:
+            {
+                const TF a = w[ijk-kk], b = w[ijk], c = w[ijk+kk], d = w[ijk+2*kk];
+                const TF wp = i2(b, c), wm = i2(a, b);
+                return - ( rp * wp * i4ws(a, b, c, d)
+                         - rm * wm * i2(a, b) ) / rc * dzc
+                       + ( rp * fabs(wp) * i3ws(a, b, c, d) ) / rc * dzc;
+            }
+            case 1:
+            {
+                const TF m2 = w[ijk-2*kk], m1 = w[ijk-kk], s0 = w[ijk], p1 = w[ijk+kk], p2 = w[ijk+2*kk], p3 = w[ijk+3*kk];
+                const TF wp = i2(s0, p1), wm = i2(m1, s0);
+                return - ( rp * wp * i6ws(m2, m1, s0, p1, p2, p3)
+                         - rm * wm * i4ws(m2, m1, s0, p1) ) / rc * dzc
+                       + ( rp * fabs(wp) * i5ws(m2, m1, s0, p1, p2, p3)
+                         - rm * fabs(wm) * i3ws(m2, m1, s0, p1) ) / rc * dzc;
+            }
+            case 2:
+            {
+                const TF m3 = w[ijk-3*kk], m2 = w[ijk-2*kk], m1 = w[ijk-kk], s0 = w[ijk], p1 = w[ijk+kk],
+                         p2 = w[ijk+2*kk], p3 = w[ijk+3*kk];
+                const TF wp = i2(s0, p1), wm = i2(m1, s0);
+                return - ( rp * wp * i6ws(m2, m1, s0, p1, p2, p3)
+                         - rm * wm * i6ws(m3, m2, m1, s0, p1, p2) ) / rc * dzc
+                       + ( rp * fabs(wp) * i5ws(m2, m1, s0, p1, p2, p3)
+                         - rm * fabs(wm) * i5ws(m3, m2, m1, s0, p1, p2) ) / rc * dzc;
+            }
+            case 3:
+            {
+                const TF m3 = w[ijk-3*kk], m2 = w[ijk-2*kk], m1 = w[ijk-kk], s0 = w[ijk], p1 = w[ijk+kk], p2 = w[ijk+2*kk];
+                const TF wp = i2(s0, p1), wm = i2(m1, s0);
+                return - ( rp * wp * i4ws(m1, s0, p1, p2)
+                         - rm * wm * i6ws(m3, m2, m1, s0, p1, p2) ) / rc * dzc
+                       + ( rp * fabs(wp) * i3ws(m1, s0, p1, p2)
+                         - rm * fabs(wm) * i5ws(m3, m2, m1, s0, p1, p2) ) / rc * dzc;
+            }
+            default:
+            {
+                const TF m2 = w[ijk-2*kk], m1 = w[ijk-kk], s0 = w[ijk], p1 = w[ijk+kk];
+                const TF wp = i2(s0, p1), wm = i2(m1, s0);
+                return - ( rp * wp * i2(s0, p1)
+                         - rm * wm * i4ws(m2, m1, s0, p1) ) / rc * dzc
+                       - ( rm * fabs(wm) * i3ws(m2, m1, s0, p1) ) / rc * dzc;
+            }
+        }
+    }
+}

@@ -1,0 +1,856 @@
+// mhh_host.cpp -- implementation of the C++ host layer and of the model C ABI
+// (include/microhh_amd_model.h).  See mhh_host.h for the reference classes
+// each type mirrors.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+#include "mhh_host.h"
+
+namespace mhh_host
+{
+    namespace
+    {
+        void hip_check(hipError_t e, const char* what)
+        {
+            if (e != hipSuccess)
+                throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+        }
+
+        template<typename TF>
+        void upload(TF* dst, const std::vector<double>& src)
+        {
+            std::vector<TF> tmp(src.begin(), src.end());
+            hip_check(hipMemcpy(dst, tmp.data(), sizeof(TF)*tmp.size(), hipMemcpyHostToDevice), "hipMemcpy H2D");
+        }
+
+        template<typename TF>
+        void download(std::vector<double>& dst, const TF* src, size_t n)
+        {
+            std::vector<TF> tmp(n);
+            hip_check(hipMemcpy(tmp.data(), src, sizeof(TF)*n, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+            dst.assign(tmp.begin(), tmp.end());
+        }
+    }
+
+    // ------------------------------------------------------------------ Grid
+    template<typename TF>
+    Grid<TF>::Grid(int itot, int jtot, int ktot, double xsize, double ysize, double zsize)
+    {
+        gd.itot = itot; gd.jtot = jtot; gd.ktot = ktot;
+        gd.xsize = xsize; gd.ysize = ysize; gd.zsize = zsize;
+        gd.igc = 1; gd.jgc = 1; gd.kgc = 1;
+    }
+
+    template<typename TF>
+    void Grid<TF>::set_minimum_ghost_cells(int igc, int jgc, int kgc)
+    {
+        gd.igc = std::max(gd.igc, igc);
+        gd.jgc = std::max(gd.jgc, jgc);
+        gd.kgc = std::max(gd.kgc, kgc);
+    }
+
+    template<typename TF>
+    void Grid<TF>::init()
+    {
+        // src/grid.cxx:106-193 on one GPU (npx = npy = 1)
+        gd.imax = gd.itot; gd.jmax = gd.jtot; gd.kmax = gd.ktot;
+        gd.icells = gd.imax + 2*gd.igc;
+        gd.jcells = gd.jmax + 2*gd.jgc;
+        gd.kcells = gd.kmax + 2*gd.kgc;
+        gd.ijcells = gd.icells*gd.jcells;
+        gd.ncells = gd.ijcells*gd.kcells;
+        gd.istart = gd.igc; gd.iend = gd.imax + gd.igc;
+        gd.jstart = gd.jgc; gd.jend = gd.jmax + gd.jgc;
+        gd.kstart = gd.kgc; gd.kend = gd.kmax + gd.kgc;
+        for (auto* v : {&gd.z, &gd.zh, &gd.dz, &gd.dzh, &gd.dzi, &gd.dzhi})
+            v->assign(gd.kcells, 0.);
+    }
+
+    template<typename TF>
+    void Grid<TF>::create(const double* z_interior)
+    {
+        for (int k = 0; k < gd.ktot; ++k)
+            gd.z[k + gd.kstart] = z_interior[k];
+        if (gd.z[gd.kend-1] > gd.zsize)
+            throw std::runtime_error("Highest grid point is above prescribed zsize");
+        calculate();
+    }
+
+    template<typename TF>
+    void Grid<TF>::calculate()
+    {
+        // src/grid.cxx:244-305, 2nd-order branch
+        gd.dx = gd.xsize / gd.itot;
+        gd.dy = gd.ysize / gd.jtot;
+        gd.dxi = 1./gd.dx;
+        gd.dyi = 1./gd.dy;
+
+        auto& z = gd.z; auto& zh = gd.zh;
+        const int ks = gd.kstart, ke = gd.kend, kc = gd.kcells;
+        z[ks-1] = -z[ks];
+        z[ke] = 2.*gd.zsize - z[ke-1];
+        for (int k = ks+1; k < ke; ++k)
+            zh[k] = 0.5*(z[k-1] + z[k]);
+        zh[ks] = 0.;
+        zh[ke] = gd.zsize;
+        for (int k = 1; k < kc; ++k)
+        {
+            gd.dzh[k] = z[k] - z[k-1];
+            gd.dzhi[k] = 1./gd.dzh[k];
+        }
+        gd.dzh[ks-1] = gd.dzh[ks+1];
+        gd.dzhi[ks-1] = gd.dzhi[ks+1];
+        for (int k = 1; k < kc-1; ++k)
+        {
+            gd.dz[k] = zh[k+1] - zh[k];
+            gd.dzi[k] = 1./gd.dz[k];
+        }
+        gd.dz[ks-1] = gd.dz[ks];
+        gd.dzi[ks-1] = gd.dzi[ks];
+        gd.dz[ke] = gd.dz[ke-1];
+        gd.dzi[ke] = gd.dzi[ke-1];
+    }
+
+    // ---------------------------------------------------------------- Fields
+    template<typename TF>
+    Fields<TF>::~Fields()
+    {
+        for (void* p : allocations)
+            (void)hipFree(p);
+    }
+
+    template<typename TF>
+    void Fields<TF>::allocate(Field3d<TF>& f, bool with_planes)
+    {
+        const auto& gd = grid.get_grid_data();
+        auto alloc = [&](TF** p, size_t n) {
+            hip_check(hipMalloc(reinterpret_cast<void**>(p), sizeof(TF)*n), "hipMalloc(field)");
+            hip_check(hipMemset(*p, 0, sizeof(TF)*n), "hipMemset(field)");
+            allocations.push_back(*p);
+        };
+        alloc(&f.fld, static_cast<size_t>(gd.ncells));
+        if (with_planes)
+            for (TF** pl : {&f.fld_bot, &f.fld_top, &f.grad_bot, &f.grad_top, &f.flux_bot, &f.flux_top})
+                alloc(pl, static_cast<size_t>(gd.ijcells));
+    }
+
+    template<typename TF>
+    void Fields<TF>::init_momentum_field(const std::string& name)
+    {
+        auto f = std::make_shared<Field3d<TF>>(); f->name = name; f->visc = visc;
+        auto t = std::make_shared<Field3d<TF>>(); t->name = name + "t";
+        mp[name] = f; mt[name] = t;
+    }
+
+    template<typename TF>
+    void Fields<TF>::init_prognostic_field(const std::string& name, double v)
+    {
+        auto f = std::make_shared<Field3d<TF>>(); f->name = name; f->visc = v;
+        auto t = std::make_shared<Field3d<TF>>(); t->name = name + "t";
+        sp[name] = f; st[name] = t;
+    }
+
+    template<typename TF>
+    void Fields<TF>::init_diagnostic_field(const std::string& name)
+    {
+        auto f = std::make_shared<Field3d<TF>>(); f->name = name;
+        sd[name] = f;
+    }
+
+    template<typename TF>
+    void Fields<TF>::init()
+    {
+        // Fields::init (src/fields.cxx): allocate every registered field; the
+        // momentum components and the scalars carry their boundary planes.
+        for (auto& it : mp) allocate(*it.second, true);
+        for (auto& it : mt) allocate(*it.second, false);
+        for (auto& it : sp) allocate(*it.second, true);
+        for (auto& it : st) allocate(*it.second, false);
+        for (auto& it : sd) allocate(*it.second, false);
+        tmp1 = std::make_shared<Field3d<TF>>(); tmp1->name = "tmp1"; allocate(*tmp1, false);
+        tmp2 = std::make_shared<Field3d<TF>>(); tmp2->name = "tmp2"; allocate(*tmp2, false);
+        const auto& gd = grid.get_grid_data();
+        rhoref.assign(gd.kcells, 1.);
+        rhorefh.assign(gd.kcells, 1.);
+    }
+
+    template<typename TF>
+    void Fields<TF>::randomize(std::vector<double>& data, double rndamp, double rndz, double rndexp)
+    {
+        // src/fields.cxx:1035-1079: glibc rand() in (k, j, i) order below rndz.
+        const auto& gd = grid.get_grid_data();
+        if (rndz > gd.zsize)
+            throw std::runtime_error("randomizer height rndz higher than domain top");
+        int kendrnd = gd.kstart;
+        while (gd.z[kendrnd] < rndz)
+            ++kendrnd;
+        for (int k = gd.kstart; k < kendrnd; ++k)
+        {
+            const double rndfac = std::pow((rndz - gd.z[k])/rndz, rndexp);
+            for (int j = gd.jstart; j < gd.jend; ++j)
+                for (int i = gd.istart; i < gd.iend; ++i)
+                    data[i + j*gd.icells + k*gd.ijcells] =
+                        rndfac * rndamp * (static_cast<double>(std::rand()) / static_cast<double>(RAND_MAX) - 0.5);
+        }
+    }
+
+    template<typename TF>
+    void Fields<TF>::create(const mhh_case& c, const double* u_prof, const double* v_prof, const double* th_prof)
+    {
+        // Fields::create (src/fields.cxx:994-1032) on the host, then one upload.
+        // The seed is set at every model creation (the reference seeds once per
+        // process); two models created with the same case get the same fields.
+        const auto& gd = grid.get_grid_data();
+        std::vector<double> u(gd.ncells, 0.), v(gd.ncells, 0.), w(gd.ncells, 0.), th;
+        std::srand(static_cast<unsigned int>(c.rndseed + 2));
+        randomize(u, c.rndamp_u, c.rndz, c.rndexp);
+        randomize(w, c.rndamp_w, c.rndz, c.rndexp);
+        if (gd.jtot > 1)
+            randomize(v, c.rndamp_v, c.rndz, c.rndexp);
+        const bool has_th = sp.count("th") > 0;
+        if (has_th)
+        {
+            th.assign(gd.ncells, 0.);
+            randomize(th, c.rndamp_th, c.rndz, c.rndexp);
+        }
+        auto add_prof = [&](std::vector<double>& a, const double* prof, double offset) {
+            for (int k = gd.kstart; k < gd.kend; ++k)
+                for (int j = gd.jstart; j < gd.jend; ++j)
+                    for (int i = gd.istart; i < gd.iend; ++i)
+                        a[i + j*gd.icells + k*gd.ijcells] += prof[k - gd.kstart] - offset;
+        };
+        add_prof(u, u_prof, c.utrans);
+        add_prof(v, v_prof, c.vtrans);
+        if (has_th)
+            add_prof(th, th_prof, 0.);
+        for (int l = 0; l < gd.ijcells; ++l)
+        {
+            w[gd.kstart*gd.ijcells + l] = 0.;
+            w[gd.kend*gd.ijcells + l] = 0.;
+        }
+        upload(mp.at("u")->fld, u);
+        upload(mp.at("v")->fld, v);
+        upload(mp.at("w")->fld, w);
+        if (has_th)
+        {
+            upload(sp.at("th")->fld, th);
+            upload(sp.at("th")->fld_bot, std::vector<double>(gd.ijcells, th_prof[0]));
+        }
+    }
+
+    template<typename TF>
+    mhh_field_set Fields<TF>::field_set() const
+    {
+        mhh_field_set f{};
+        f.u = mp.at("u")->fld; f.v = mp.at("v")->fld; f.w = mp.at("w")->fld;
+        f.ut = mt.at("u")->fld; f.vt = mt.at("v")->fld; f.wt = mt.at("w")->fld;
+        f.u_bot = mp.at("u")->fld_bot; f.u_top = mp.at("u")->fld_top;
+        f.u_gradbot = mp.at("u")->grad_bot; f.u_gradtop = mp.at("u")->grad_top;
+        f.u_fluxbot = mp.at("u")->flux_bot; f.u_fluxtop = mp.at("u")->flux_top;
+        f.v_bot = mp.at("v")->fld_bot; f.v_top = mp.at("v")->fld_top;
+        f.v_gradbot = mp.at("v")->grad_bot; f.v_gradtop = mp.at("v")->grad_top;
+        f.v_fluxbot = mp.at("v")->flux_bot; f.v_fluxtop = mp.at("v")->flux_top;
+        int n = 0;
+        for (auto& it : sp)
+        {
+            f.s[n] = it.second->fld; f.st[n] = st.at(it.first)->fld;
+            f.s_bot[n] = it.second->fld_bot; f.s_top[n] = it.second->fld_top;
+            f.s_gradbot[n] = it.second->grad_bot; f.s_gradtop[n] = it.second->grad_top;
+            f.s_fluxbot[n] = it.second->flux_bot; f.s_fluxtop[n] = it.second->flux_top;
+            ++n;
+        }
+        f.nscalars = n;
+        f.p = sd.at("p")->fld;
+        f.evisc = sd.count("evisc") ? sd.at("evisc")->fld : nullptr;
+        f.tmp1 = tmp1->fld; f.tmp2 = tmp2->fld;
+        return f;
+    }
+
+    template<typename TF>
+    Field3d<TF>& Fields<TF>::any(const std::string& name) const
+    {
+        if (mp.count(name)) return *mp.at(name);
+        if (sp.count(name)) return *sp.at(name);
+        if (sd.count(name)) return *sd.at(name);
+        if (name.size() > 1 && name.back() == 't')
+        {
+            const std::string b = name.substr(0, name.size()-1);
+            if (mt.count(b)) return *mt.at(b);
+            if (st.count(b)) return *st.at(b);
+        }
+        if (name == "tmp1") return *tmp1;
+        if (name == "tmp2") return *tmp2;
+        throw std::runtime_error("unknown field " + name);
+    }
+
+    template<typename TF>
+    void Fields<TF>::set_host(const std::string& name, const double* data)
+    {
+        const auto& gd = grid.get_grid_data();
+        upload(any(name).fld, std::vector<double>(data, data + gd.ncells));
+    }
+
+    template<typename TF>
+    void Fields<TF>::get_host(const std::string& name, double* data) const
+    {
+        const auto& gd = grid.get_grid_data();
+        std::vector<double> h;
+        download(h, any(name).fld, static_cast<size_t>(gd.ncells));
+        std::memcpy(data, h.data(), sizeof(double)*h.size());
+    }
+
+    // -------------------------------------------------------------- Boundary
+    template<typename TF>
+    Boundary<TF>::Boundary(mhh_ctx* ctx, const Grid<TF>& grid, Fields<TF>& fields, const mhh_case& c) :
+        ctx(ctx), grid(grid), fields(fields),
+        mbcbot(c.mbcbot), mbctop(c.mbctop), sbcbot(c.sbcbot), sbctop(c.sbctop),
+        ubot(c.ubot), vbot(c.vbot), utop(c.utop), vtop(c.vtop), sbot(c.sbot), stop(c.stop),
+        utrans(c.utrans), vtrans(c.vtrans)
+    {
+        for (int t : {mbcbot, mbctop, sbcbot, sbctop})
+            if (t < MHH_BC_DIRICHLET || t > MHH_BC_OFF || t == MHH_BC_USTAR)
+                throw std::runtime_error("boundary type not supported by the default boundary switch");
+    }
+
+    template<typename TF>
+    void Boundary<TF>::set_bc(TF* a, TF* agrad, TF* aflux, int sw, double aval, double visc, double offset)
+    {
+        // set_bc<TF, set_flux_grad=true> (src/boundary.cxx:51-91)
+        const int n = grid.get_grid_data().ijcells;
+        if (sw == MHH_BC_DIRICHLET)
+            upload(a, std::vector<double>(n, aval - offset));
+        else if (sw == MHH_BC_NEUMANN)
+        {
+            upload(agrad, std::vector<double>(n, aval));
+            upload(aflux, std::vector<double>(n, -aval*visc));
+        }
+        else if (sw == MHH_BC_FLUX)
+        {
+            upload(aflux, std::vector<double>(n, aval));
+            upload(agrad, std::vector<double>(n, -aval/visc));
+        }
+    }
+
+    template<typename TF>
+    void Boundary<TF>::set_values()
+    {
+        // src/boundary.cxx:599-696, boundary switch "default"
+        auto& u = *fields.mp.at("u"); auto& v = *fields.mp.at("v");
+        set_bc(u.fld_bot, u.grad_bot, u.flux_bot, mbcbot, ubot, fields.visc, utrans);
+        set_bc(v.fld_bot, v.grad_bot, v.flux_bot, mbcbot, vbot, fields.visc, vtrans);
+        set_bc(u.fld_top, u.grad_top, u.flux_top, mbctop, utop, fields.visc, utrans);
+        set_bc(v.fld_top, v.grad_top, v.flux_top, mbctop, vtop, fields.visc, vtrans);
+        for (auto& it : fields.sp)
+        {
+            auto& s = *it.second;
+            set_bc(s.fld_bot, s.grad_bot, s.flux_bot, sbcbot, sbot, s.visc, 0.);
+            set_bc(s.fld_top, s.grad_top, s.flux_top, sbctop, stop, s.visc, 0.);
+        }
+    }
+
+    template<typename TF>
+    mhh_bc_desc Boundary<TF>::bc_desc() const
+    {
+        mhh_bc_desc bc{};
+        bc.mbcbot = mbcbot; bc.mbctop = mbctop;
+        for (int n = 0; n < MHH_MAX_SCALARS; ++n) { bc.sbcbot[n] = sbcbot; bc.sbctop[n] = sbctop; }
+        return bc;
+    }
+
+    template<typename TF>
+    void Boundary<TF>::set_prognostic_cyclic_bcs()
+    {
+        check(mhh_boundary_set_prognostic_cyclic_bcs(ctx), "Boundary::set_prognostic_cyclic_bcs");
+    }
+
+    template<typename TF>
+    void Boundary<TF>::set_ghost_cells()
+    {
+        const mhh_bc_desc bc = bc_desc();
+        check(mhh_boundary_set_ghost_cells(ctx, &bc), "Boundary::set_ghost_cells");
+    }
+
+    // ----------------------------------------------------------------- Advec
+    template<typename TF>
+    unsigned long Advec<TF>::get_time_limit(unsigned long idt, double dt)
+    {
+        // src/advec_2i5.cxx:973-992
+        double cfl = get_cfl(dt);
+        cfl = std::max(cflmin, cfl);
+        return idt * cflmax / cfl;
+    }
+
+    template<typename TF>
+    std::unique_ptr<Advec<TF>> Advec<TF>::factory(mhh_ctx* ctx, Grid<TF>& grid, int swadvec, double cflmax)
+    {
+        // Advec::factory (src/advec.cxx:54-96) for the in-scope schemes
+        if (swadvec == 0) return std::make_unique<Advec_2i5<TF>>(ctx, grid, cflmax);
+        if (swadvec == 1) return std::make_unique<Advec_2<TF>>(ctx, cflmax);
+        throw std::runtime_error("Illegal option for swadvec");
+    }
+
+    template<typename TF>
+    Advec_2i5<TF>::Advec_2i5(mhh_ctx* ctx, Grid<TF>& grid, double cflmax) : Advec<TF>(ctx, cflmax)
+    {
+        (void)grid;  // the ghost cells were raised before the grid was sized (Model::Model)
+    }
+
+    template<typename TF>
+    void Advec_2i5<TF>::exec() { check(mhh_advec_2i5_exec(this->ctx), "Advec_2i5::exec"); }
+
+    template<typename TF>
+    double Advec_2i5<TF>::get_cfl(double dt)
+    {
+        double cfl;
+        check(mhh_advec_2i5_cfl(this->ctx, dt, &cfl), "Advec_2i5::get_cfl");
+        return cfl;
+    }
+
+    template<typename TF>
+    void Advec_2<TF>::exec() { check(mhh_advec_2_exec(this->ctx), "Advec_2::exec"); }
+
+    template<typename TF>
+    double Advec_2<TF>::get_cfl(double dt)
+    {
+        double cfl;
+        check(mhh_advec_2_cfl(this->ctx, dt, &cfl), "Advec_2::get_cfl");
+        return cfl;
+    }
+
+    // ------------------------------------------------------------------ Diff
+    template<typename TF>
+    std::unique_ptr<Diff<TF>> Diff<TF>::factory(mhh_ctx* ctx, const Grid<TF>& grid, Fields<TF>& fields,
+                                                const mhh_case& c)
+    {
+        // Diff::factory (src/diff.cxx:55-95) for the in-scope schemes
+        if (c.diff == 0) return std::make_unique<Diff_smag2<TF>>(ctx, fields, c);
+        if (c.diff == 1) return std::make_unique<Diff_2<TF>>(ctx, grid, fields, c);
+        throw std::runtime_error("Illegal option for swdiff");
+    }
+
+    template<typename TF>
+    Diff_smag2<TF>::Diff_smag2(mhh_ctx* ctx, Fields<TF>& fields, const mhh_case& c) : Diff<TF>(ctx), dnmax(c.dnmax)
+    {
+        par.cs = c.cs; par.tPr = c.tPr; par.visc = fields.visc;
+        int n = 0;
+        for (auto& it : fields.sp) par.svisc[n++] = it.second->visc;
+        par.swmason = c.swmason;
+        par.surface_model = 0;       // boundary switch "default"
+        par.thermo = (c.thermo == 0) ? 1 : 0;
+    }
+
+    template<typename TF>
+    void Diff_smag2<TF>::exec_viscosity()
+    {
+        check(mhh_diff_smag2_exec_viscosity(this->ctx, &par), "Diff_smag2::exec_viscosity");
+    }
+
+    template<typename TF>
+    void Diff_smag2<TF>::exec() { check(mhh_diff_smag2_exec(this->ctx, &par), "Diff_smag2::exec"); }
+
+    template<typename TF>
+    unsigned long Diff_smag2<TF>::get_time_limit(unsigned long idt, double dt)
+    {
+        // src/diff_smag2.cxx:312-333
+        double dnmul;
+        check(mhh_diff_smag2_dnmul(this->ctx, par.tPr, &dnmul), "Diff_smag2::get_time_limit");
+        dnmul = std::max(1.e-9, dnmul);
+        return idt * dnmax / (dt * dnmul);
+    }
+
+    template<typename TF>
+    double Diff_smag2<TF>::get_dn(double dt)
+    {
+        double dnmul;
+        check(mhh_diff_smag2_dnmul(this->ctx, par.tPr, &dnmul), "Diff_smag2::get_dn");
+        return dnmul*dt;
+    }
+
+    template<typename TF>
+    Diff_2<TF>::Diff_2(mhh_ctx* ctx, const Grid<TF>& grid, Fields<TF>& fields, const mhh_case& c) :
+        Diff<TF>(ctx), visc(fields.visc), svisc(c.svisc), dnmax(c.dnmax)
+    {
+        // Diff_2::create (src/diff_2.cxx:114-131)
+        const auto& gd = grid.get_grid_data();
+        double viscmax = visc;
+        for (auto& it : fields.sp) viscmax = std::max(it.second->visc, viscmax);
+        dnmul = 0;
+        for (int k = gd.kstart; k < gd.kend; ++k)
+            dnmul = std::max(dnmul, std::abs(viscmax * (1./(gd.dx*gd.dx) + 1./(gd.dy*gd.dy) + 1./(gd.dz[k]*gd.dz[k]))));
+    }
+
+    template<typename TF>
+    void Diff_2<TF>::exec()
+    {
+        double sv[MHH_MAX_SCALARS];
+        std::fill(sv, sv + MHH_MAX_SCALARS, svisc);
+        check(mhh_diff_2_exec(this->ctx, visc, sv), "Diff_2::exec");
+    }
+
+    template<typename TF>
+    unsigned long Diff_2<TF>::get_time_limit(unsigned long idt, double dt)
+    {
+        return idt * dnmax / (dt * dnmul);
+    }
+
+    // ----------------------------------------------------------- Thermo_dry
+    template<typename TF>
+    Thermo_dry<TF>::Thermo_dry(mhh_ctx* ctx, const Grid<TF>& grid, double thref0) :
+        ctx(ctx), grid(grid), thref0(thref0) {}
+
+    template<typename TF>
+    void Thermo_dry<TF>::create_basestate(Fields<TF>& fields)
+    {
+        // Boussinesq branch of src/thermo_dry.cxx:558-585
+        const auto& gd = grid.get_grid_data();
+        thref.assign(gd.kcells, thref0);
+        threfh.assign(gd.kcells, thref0);
+        fields.rhoref.assign(gd.kcells, 1.);
+        fields.rhorefh.assign(gd.kcells, 1.);
+        check(mhh_set_base_state(ctx, thref.data(), threfh.data()), "Thermo_dry::create_basestate");
+    }
+
+    template<typename TF>
+    void Thermo_dry<TF>::exec(double) { check(mhh_thermo_dry_exec(ctx), "Thermo_dry::exec"); }
+
+    // ---------------------------------------------------------------- Pres_2
+    template<typename TF>
+    void Pres_2<TF>::exec(double dt) { check(mhh_pres_2_exec(ctx, dt), "Pres_2::exec"); }
+
+    template<typename TF>
+    double Pres_2<TF>::check_divergence()
+    {
+        double div;
+        check(mhh_pres_2_divergence(ctx, &div), "Pres_2::check_divergence");
+        return div;
+    }
+
+    // -------------------------------------------------------------- Timeloop
+    template<typename TF>
+    Timeloop<TF>::Timeloop(mhh_ctx* ctx, const mhh_case& c) : ctx(ctx)
+    {
+        // src/timeloop.cxx:40-130 (rkorder 3)
+        adaptivestep = c.adaptivestep != 0;
+        dt = c.dt;
+        idt = convert_to_itime(c.dt);
+        idtmax = convert_to_itime(c.dtmax);
+        iendtime = convert_to_itime(c.endtime);
+        isavetime = convert_to_itime(c.savetime);
+        if (isavetime == 0)
+            throw std::runtime_error("savetime must be positive");
+        idtlim = idt;
+    }
+
+    template<typename TF>
+    void Timeloop<TF>::exec()
+    {
+        check(mhh_timeloop_rk3(ctx, substep, dt), "Timeloop::exec");
+        substep = (substep + 1) % 3;
+    }
+
+    template<typename TF>
+    double Timeloop<TF>::get_sub_time_step() const
+    {
+        // rk3subdt (src/timeloop.cxx:338-342)
+        constexpr double cB[] = {1./3., 15./16., 8./15.};
+        return cB[substep]*dt;
+    }
+
+    template<typename TF>
+    void Timeloop<TF>::set_time_step_limit()
+    {
+        idtlim = idtmax;
+        idtlim = std::min(idtlim, isavetime - itime % isavetime);
+        if (itime < iendtime)
+            idtlim = std::min(idtlim, iendtime - itime);
+    }
+
+    template<typename TF>
+    void Timeloop<TF>::set_time_step()
+    {
+        if (in_substep()) return;
+        if (adaptivestep)
+        {
+            if (idt == 0 || idtlim == 0)
+                throw std::runtime_error("Required time step less than precision 1e-9 of the time stepping");
+            idt = idtlim;
+            dt = static_cast<double>(idt) / ifactor;
+        }
+    }
+
+    template<typename TF>
+    void Timeloop<TF>::step_time()
+    {
+        if (in_substep()) return;
+        time += dt;
+        itime += idt;
+        ++iteration;
+        if (itime >= iendtime)
+            loop = false;
+    }
+
+    // ----------------------------------------------------------------- Model
+    template<typename TF>
+    Model<TF>::Model(const mhh_case& c, const double* z, const double* u_prof, const double* v_prof,
+                     const double* th_prof, void* stream) : cs(c)
+    {
+        grid = std::make_unique<Grid<TF>>(c.itot, c.jtot, c.ktot, c.xsize, c.ysize, c.zsize);
+        if (c.advec == 0)
+            grid->set_minimum_ghost_cells(3, 3, 1);   // src/advec_2i5.cxx:42-45
+        grid->init();
+        grid->create(z);
+
+        fields = std::make_unique<Fields<TF>>(*grid, c.visc);
+        fields->init_momentum_field("u");
+        fields->init_momentum_field("v");
+        fields->init_momentum_field("w");
+        if (c.thermo == 0)
+            fields->init_prognostic_field("th", c.svisc);
+        fields->init_diagnostic_field("p");
+        if (c.diff == 0)
+            fields->init_diagnostic_field("evisc");
+        fields->init();
+
+        const auto& gd = grid->get_grid_data();
+        mhh_grid_desc g{};
+        g.itot = gd.itot; g.jtot = gd.jtot; g.ktot = gd.ktot;
+        g.igc = gd.igc; g.jgc = gd.jgc; g.kgc = gd.kgc;
+        g.dx = gd.dx; g.dy = gd.dy; g.zsize = gd.zsize;
+        g.z = gd.z.data(); g.zh = gd.zh.data(); g.dz = gd.dz.data(); g.dzh = gd.dzh.data();
+        g.dzi = gd.dzi.data(); g.dzhi = gd.dzhi.data();
+        g.rhoref = fields->rhoref.data(); g.rhorefh = fields->rhorefh.data();
+        const int prec = std::is_same<TF, double>::value ? MHH_FP64 : MHH_FP32;
+        check(mhh_ctx_create(&g, prec, stream, &ctx_), "mhh_ctx_create");
+
+        if (c.thermo == 0)
+        {
+            thermo = std::make_unique<Thermo_dry<TF>>(ctx_, *grid, c.thref0);
+            thermo->create_basestate(*fields);
+        }
+        fields->create(c, u_prof, v_prof, th_prof);
+        boundary = std::make_unique<Boundary<TF>>(ctx_, *grid, *fields, c);
+        boundary->set_values();
+        const mhh_field_set fs = fields->field_set();
+        check(mhh_fields_bind(ctx_, &fs), "mhh_fields_bind");
+
+        advec = Advec<TF>::factory(ctx_, *grid, c.advec, c.cflmax);
+        diff = Diff<TF>::factory(ctx_, *grid, *fields, c);
+        pres = std::make_unique<Pres_2<TF>>(ctx_);
+        timeloop = std::make_unique<Timeloop<TF>>(ctx_, c);
+    }
+
+    template<typename TF>
+    Model<TF>::~Model()
+    {
+        if (ctx_)
+            mhh_ctx_destroy(ctx_);
+    }
+
+    template<typename TF>
+    void Model<TF>::set_time_step()
+    {
+        // Model::set_time_step (src/model.cxx:764-786)
+        if (timeloop->in_substep()) return;
+        timeloop->set_time_step_limit();
+        timeloop->set_time_step_limit(advec->get_time_limit(timeloop->get_idt(), timeloop->get_dt()));
+        timeloop->set_time_step_limit(diff->get_time_limit(timeloop->get_idt(), timeloop->get_dt()));
+        if (thermo)
+            timeloop->set_time_step_limit(thermo->get_time_limit(timeloop->get_idt(), timeloop->get_dt()));
+        timeloop->set_time_step();
+    }
+
+    template<typename TF>
+    void Model<TF>::substep()
+    {
+        // The in-scope part of Model::exec (src/model.cxx:368-507).
+        boundary->set_prognostic_cyclic_bcs();
+        boundary->set_ghost_cells();
+        diff->exec_viscosity();
+        set_time_step();
+        const double subdt = timeloop->get_sub_time_step();
+        if (thermo)
+            thermo->exec(subdt);
+        boundary->set_ghost_cells();
+        advec->exec();
+        diff->exec();
+        pres->exec(subdt);
+        timeloop->exec();
+        timeloop->step_time();
+    }
+
+    template<typename TF>
+    void* Model<TF>::field_ptr(const std::string& name) { return fields->any(name).fld; }
+
+    template<typename TF>
+    void Model<TF>::get_field(const std::string& name, double* host)
+    {
+        check(mhh_sync(ctx_), "mhh_sync");
+        fields->get_host(name, host);
+    }
+
+    template<typename TF>
+    void Model<TF>::set_field(const std::string& name, const double* host)
+    {
+        check(mhh_sync(ctx_), "mhh_sync");
+        fields->set_host(name, host);
+    }
+
+    template<typename TF>
+    void Model<TF>::dims(int* d6) const
+    {
+        const auto& gd = grid->get_grid_data();
+        d6[0] = gd.icells; d6[1] = gd.jcells; d6[2] = gd.kcells;
+        d6[3] = gd.istart; d6[4] = gd.jstart; d6[5] = gd.kstart;
+    }
+
+    template<typename TF>
+    void Model<TF>::time(double* t, double* dt, int* sub, int* it) const
+    {
+        if (t) *t = timeloop->get_time();
+        if (dt) *dt = timeloop->get_dt();
+        if (sub) *sub = timeloop->get_substep();
+        if (it) *it = timeloop->get_iteration();
+    }
+
+    template class Model<double>;
+    template class Model<float>;
+}
+
+struct mhh_model
+{
+    std::unique_ptr<mhh_host::Model_base> m;
+};
+
+// The error slot of the operator ABI (mhh_ctx.hip) also carries model errors.
+namespace mhh { void set_error(const std::string& msg); }
+
+namespace
+{
+    void mhh_host_set_error(const char* msg) { mhh::set_error(msg); }
+
+    // C++ exceptions stop at the ABI: they become MHH_ESTATE plus the message.
+    template<typename F>
+    int guarded(F&& f)
+    {
+        try
+        {
+            return f();
+        }
+        catch (std::exception& e)
+        {
+            mhh_host_set_error(e.what());
+            return MHH_ESTATE;
+        }
+    }
+}
+
+extern "C" {
+
+int mhh_model_create(const mhh_case* c, const double* z, const double* u_prof, const double* v_prof,
+                     const double* th_prof, void* stream, mhh_model** out)
+{
+    if (!c || !z || !u_prof || !v_prof || !out || (c->thermo == 0 && !th_prof))
+    {
+        mhh_host_set_error("mhh_model_create: null argument");
+        return MHH_EINVAL;
+    }
+    *out = nullptr;
+    return guarded([&]() {
+        auto* m = new mhh_model();
+        try
+        {
+            if (c->precision == MHH_FP64)
+                m->m = std::make_unique<mhh_host::Model<double>>(*c, z, u_prof, v_prof, th_prof, stream);
+            else
+                m->m = std::make_unique<mhh_host::Model<float>>(*c, z, u_prof, v_prof, th_prof, stream);
+        }
+        catch (...)
+        {
+            delete m;
+            throw;
+        }
+        *out = m;
+        return 0;
+    });
+}
+
+int mhh_model_destroy(mhh_model* m)
+{
+    delete m;
+    return 0;
+}
+
+int mhh_model_substep(mhh_model* m)
+{
+    if (!m) { mhh_host_set_error("null model"); return MHH_EINVAL; }
+    return guarded([&]() { m->m->substep(); return 0; });
+}
+
+int mhh_model_steps(mhh_model* m, int n)
+{
+    if (!m || n < 0) { mhh_host_set_error("null model or negative step count"); return MHH_EINVAL; }
+    return guarded([&]() { for (int s = 0; s < 3*n; ++s) m->m->substep(); return 0; });
+}
+
+int mhh_model_field_ptr(mhh_model* m, const char* name, void** ptr)
+{
+    if (!m || !name || !ptr) { mhh_host_set_error("null argument"); return MHH_EINVAL; }
+    return guarded([&]() { *ptr = m->m->field_ptr(name); return 0; });
+}
+
+int mhh_model_get_field(mhh_model* m, const char* name, double* host)
+{
+    if (!m || !name || !host) { mhh_host_set_error("null argument"); return MHH_EINVAL; }
+    return guarded([&]() { m->m->get_field(name, host); return 0; });
+}
+
+int mhh_model_set_field(mhh_model* m, const char* name, const double* host)
+{
+    if (!m || !name || !host) { mhh_host_set_error("null argument"); return MHH_EINVAL; }
+    return guarded([&]() { m->m->set_field(name, host); return 0; });
+}
+
+int mhh_model_dims(const mhh_model* m, int* d6)
+{
+    if (!m || !d6) { mhh_host_set_error("null argument"); return MHH_EINVAL; }
+    m->m->dims(d6);
+    return 0;
+}
+
+int mhh_model_time(const mhh_model* m, double* t, double* dt, int* substep, int* iteration)
+{
+    if (!m) { mhh_host_set_error("null model"); return MHH_EINVAL; }
+    m->m->time(t, dt, substep, iteration);
+    return 0;
+}
+
+int mhh_model_cfl(mhh_model* m, double* v)
+{
+    if (!m || !v) { mhh_host_set_error("null argument"); return MHH_EINVAL; }
+    return guarded([&]() { *v = m->m->cfl(); return 0; });
+}
+
+int mhh_model_dn(mhh_model* m, double* v)
+{
+    if (!m || !v) { mhh_host_set_error("null argument"); return MHH_EINVAL; }
+    return guarded([&]() { *v = m->m->dn(); return 0; });
+}
+
+int mhh_model_div(mhh_model* m, double* v)
+{
+    if (!m || !v) { mhh_host_set_error("null argument"); return MHH_EINVAL; }
+    return guarded([&]() { *v = m->m->div(); return 0; });
+}
+
+int mhh_model_ctx(mhh_model* m, mhh_ctx** ctx)
+{
+    if (!m || !ctx) { mhh_host_set_error("null argument"); return MHH_EINVAL; }
+    *ctx = m->m->ctx();
+    return 0;
+}
+
+} // extern "C"

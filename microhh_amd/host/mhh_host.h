@@ -1,0 +1,311 @@
+// mhh_host.h -- C++ host layer mirroring the reference operator classes on top
+// of the C ABI (include/microhh_amd.h).  Same class names, method names,
+// argument meaning and error behaviour (std::runtime_error) as the reference:
+//   Grid          include/grid.h:49-138, src/grid.cxx:106-305
+//   Field3d       include/field3d.h:36-83
+//   Fields        include/fields.h:134-145, src/fields.cxx:994-1134
+//   Boundary      src/boundary.cxx:449-464, 599-696, 932-997 (switch "default")
+//   Advec_2i5     include/advec_2i5.h, src/advec_2i5.cxx:954-1063
+//   Advec_2       src/advec_2.cxx:280-350
+//   Diff_smag2    include/diff_smag2.h, src/diff_smag2.cxx:272-607
+//   Diff_2        src/diff_2.cxx:92-190
+//   Thermo_dry    src/thermo_dry.cxx:504-643 (Boussinesq base state)
+//   Pres_2        include/pres_2.h, src/pres_2.cxx:64-122, 389-422
+//   Timeloop      include/timeloop.h:48-146, src/timeloop.cxx:40-423
+//   Model         src/model.cxx:356-545, 764-786
+// The arithmetic runs in the HIP kernels behind the C ABI; this layer owns the
+// device fields, the boundary planes, the time loop and the call order.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "microhh_amd.h"
+#include "microhh_amd_model.h"
+
+namespace mhh_host
+{
+    inline void check(int rc, const char* what)
+    {
+        if (rc != 0)
+            throw std::runtime_error(std::string(what) + ": " + mhh_last_error());
+    }
+
+    template<typename TF>
+    struct Grid_data
+    {
+        int itot, jtot, ktot;
+        int imax, jmax, kmax;
+        int igc, jgc, kgc;
+        int icells, jcells, kcells, ijcells, ncells;
+        int istart, iend, jstart, jend, kstart, kend;
+        double xsize, ysize, zsize;
+        double dx, dy, dxi, dyi;
+        std::vector<double> z, zh, dz, dzh, dzi, dzhi;
+    };
+
+    template<typename TF>
+    class Grid
+    {
+        public:
+            Grid(int itot, int jtot, int ktot, double xsize, double ysize, double zsize);
+            void set_minimum_ghost_cells(int igc, int jgc, int kgc);
+            void init();
+            void create(const double* z_interior);
+            const Grid_data<TF>& get_grid_data() const { return gd; }
+
+        private:
+            void calculate();
+            Grid_data<TF> gd;
+    };
+
+    template<typename TF>
+    struct Field3d
+    {
+        std::string name;
+        double visc = 0.;
+        TF* fld = nullptr;
+        TF* fld_bot = nullptr; TF* fld_top = nullptr;
+        TF* grad_bot = nullptr; TF* grad_top = nullptr;
+        TF* flux_bot = nullptr; TF* flux_top = nullptr;
+    };
+
+    template<typename TF>
+    class Fields
+    {
+        public:
+            explicit Fields(const Grid<TF>& grid, double visc) : grid(grid), visc(visc) {}
+            ~Fields();
+            void init_momentum_field(const std::string& name);
+            void init_prognostic_field(const std::string& name, double visc);
+            void init_diagnostic_field(const std::string& name);
+            void init();
+            void create(const mhh_case& c, const double* u_prof, const double* v_prof, const double* th_prof);
+            mhh_field_set field_set() const;
+
+            void set_host(const std::string& name, const double* data);
+            void get_host(const std::string& name, double* data) const;
+            Field3d<TF>& any(const std::string& name) const;
+
+            std::map<std::string, std::shared_ptr<Field3d<TF>>> mp, mt, sp, st, sd;
+            std::vector<double> rhoref, rhorefh;
+            std::shared_ptr<Field3d<TF>> tmp1, tmp2;
+            double visc;
+
+        private:
+            void randomize(std::vector<double>& data, double rndamp, double rndz, double rndexp);
+            void allocate(Field3d<TF>& f, bool with_planes);
+            const Grid<TF>& grid;
+            std::vector<void*> allocations;
+    };
+
+    template<typename TF>
+    class Boundary
+    {
+        public:
+            Boundary(mhh_ctx* ctx, const Grid<TF>& grid, Fields<TF>& fields, const mhh_case& c);
+            void set_values();
+            void set_prognostic_cyclic_bcs();
+            void set_ghost_cells();
+            mhh_bc_desc bc_desc() const;
+            std::string get_switch() const { return "default"; }
+
+        private:
+            void set_bc(TF* a, TF* agrad, TF* aflux, int sw, double aval, double visc, double offset);
+            mhh_ctx* ctx;
+            const Grid<TF>& grid;
+            Fields<TF>& fields;
+            int mbcbot, mbctop, sbcbot, sbctop;
+            double ubot, vbot, utop, vtop, sbot, stop, utrans, vtrans;
+    };
+
+    template<typename TF>
+    class Advec
+    {
+        public:
+            Advec(mhh_ctx* ctx, double cflmax) : ctx(ctx), cflmax(cflmax) {}
+            virtual ~Advec() = default;
+            virtual void exec() = 0;
+            virtual double get_cfl(double dt) = 0;
+            unsigned long get_time_limit(unsigned long idt, double dt);
+            static std::unique_ptr<Advec<TF>> factory(mhh_ctx* ctx, Grid<TF>& grid, int swadvec, double cflmax);
+
+        protected:
+            mhh_ctx* ctx;
+            const double cflmin = 1.E-5;   // src/advec.cxx:44
+            double cflmax;
+    };
+
+    template<typename TF>
+    class Advec_2i5 : public Advec<TF>
+    {
+        public:
+            Advec_2i5(mhh_ctx* ctx, Grid<TF>& grid, double cflmax);
+            void exec() override;
+            double get_cfl(double dt) override;
+    };
+
+    template<typename TF>
+    class Advec_2 : public Advec<TF>
+    {
+        public:
+            Advec_2(mhh_ctx* ctx, double cflmax) : Advec<TF>(ctx, cflmax) {}
+            void exec() override;
+            double get_cfl(double dt) override;
+    };
+
+    template<typename TF>
+    class Diff
+    {
+        public:
+            explicit Diff(mhh_ctx* ctx) : ctx(ctx) {}
+            virtual ~Diff() = default;
+            virtual void exec_viscosity() = 0;
+            virtual void exec() = 0;
+            virtual unsigned long get_time_limit(unsigned long idt, double dt) = 0;
+            virtual double get_dn(double dt) = 0;
+            static std::unique_ptr<Diff<TF>> factory(mhh_ctx* ctx, const Grid<TF>& grid, Fields<TF>& fields,
+                                                     const mhh_case& c);
+        protected:
+            mhh_ctx* ctx;
+    };
+
+    template<typename TF>
+    class Diff_smag2 : public Diff<TF>
+    {
+        public:
+            Diff_smag2(mhh_ctx* ctx, Fields<TF>& fields, const mhh_case& c);
+            void exec_viscosity() override;
+            void exec() override;
+            unsigned long get_time_limit(unsigned long idt, double dt) override;
+            double get_dn(double dt) override;
+        private:
+            mhh_smag_params par{};
+            double dnmax;
+    };
+
+    template<typename TF>
+    class Diff_2 : public Diff<TF>
+    {
+        public:
+            Diff_2(mhh_ctx* ctx, const Grid<TF>& grid, Fields<TF>& fields, const mhh_case& c);
+            void exec_viscosity() override {}
+            void exec() override;
+            unsigned long get_time_limit(unsigned long idt, double dt) override;
+            double get_dn(double dt) override { return dnmul*dt; }
+        private:
+            double visc, svisc, dnmax, dnmul;
+    };
+
+    template<typename TF>
+    class Thermo_dry
+    {
+        public:
+            Thermo_dry(mhh_ctx* ctx, const Grid<TF>& grid, double thref0);
+            void create_basestate(Fields<TF>& fields);
+            void exec(double dt);
+            unsigned long get_time_limit(unsigned long, double) { return ulhuge; }
+        private:
+            static constexpr unsigned long ulhuge = static_cast<unsigned long>(-1);
+            mhh_ctx* ctx;
+            const Grid<TF>& grid;
+            double thref0;
+            std::vector<double> thref, threfh;
+    };
+
+    template<typename TF>
+    class Pres_2
+    {
+        public:
+            explicit Pres_2(mhh_ctx* ctx) : ctx(ctx) {}
+            void exec(double dt);
+            double check_divergence();
+        private:
+            mhh_ctx* ctx;
+    };
+
+    // Integer time (include/timeloop.h:142-146).
+    constexpr double ifactor = 1e9;
+    inline unsigned long convert_to_itime(double t) { return static_cast<unsigned long>(ifactor * t + 0.5); }
+
+    template<typename TF>
+    class Timeloop
+    {
+        public:
+            Timeloop(mhh_ctx* ctx, const mhh_case& c);
+            void exec();
+            double get_sub_time_step() const;
+            bool in_substep() const { return substep > 0; }
+            void set_time_step_limit();
+            void set_time_step_limit(unsigned long idtlimin) { idtlim = std::min(idtlim, idtlimin); }
+            void set_time_step();
+            void step_time();
+            int get_substep() const { return substep; }
+            unsigned long get_idt() const { return idt; }
+            double get_dt() const { return dt; }
+            double get_time() const { return time; }
+            int get_iteration() const { return iteration; }
+            bool is_finished() const { return !loop; }
+        private:
+            mhh_ctx* ctx;
+            int substep = 0;
+            bool adaptivestep;
+            double dt, time = 0.;
+            int iteration = 0;
+            bool loop = true;
+            unsigned long itime = 0, idt, idtmax, iendtime, isavetime, idtlim;
+    };
+
+    class Model_base
+    {
+        public:
+            virtual ~Model_base() = default;
+            virtual void substep() = 0;
+            virtual void* field_ptr(const std::string& name) = 0;
+            virtual void get_field(const std::string& name, double* host) = 0;
+            virtual void set_field(const std::string& name, const double* host) = 0;
+            virtual void dims(int* d6) const = 0;
+            virtual void time(double* t, double* dt, int* substep, int* iteration) const = 0;
+            virtual double cfl() = 0;
+            virtual double dn() = 0;
+            virtual double div() = 0;
+            virtual mhh_ctx* ctx() = 0;
+    };
+
+    template<typename TF>
+    class Model : public Model_base
+    {
+        public:
+            Model(const mhh_case& c, const double* z, const double* u_prof, const double* v_prof,
+                  const double* th_prof, void* stream);
+            ~Model() override;
+            void substep() override;
+            void* field_ptr(const std::string& name) override;
+            void get_field(const std::string& name, double* host) override;
+            void set_field(const std::string& name, const double* host) override;
+            void dims(int* d6) const override;
+            void time(double* t, double* dt, int* substep, int* iteration) const override;
+            // Evaluated on fresh halos, as at the reference's print_status
+            // (src/model.cxx:368-382): the cyclic update precedes the check.
+            double cfl() override { boundary->set_prognostic_cyclic_bcs(); return advec->get_cfl(timeloop->get_dt()); }
+            double dn() override { return diff->get_dn(timeloop->get_dt()); }
+            double div() override { boundary->set_prognostic_cyclic_bcs(); return pres->check_divergence(); }
+            mhh_ctx* ctx() override { return ctx_; }
+
+        private:
+            void set_time_step();
+            mhh_case cs;
+            mhh_ctx* ctx_ = nullptr;
+            std::unique_ptr<Grid<TF>> grid;
+            std::unique_ptr<Fields<TF>> fields;
+            std::unique_ptr<Boundary<TF>> boundary;
+            std::unique_ptr<Advec<TF>> advec;
+            std::unique_ptr<Diff<TF>> diff;
+            std::unique_ptr<Thermo_dry<TF>> thermo;
+            std::unique_ptr<Pres_2<TF>> pres;
+            std::unique_ptr<Timeloop<TF>> timeloop;
+    };
+}

@@ -1,0 +1,190 @@
+"""Python binding of the model driver (include/microhh_amd_model.h).
+
+The driver itself is the C++ host layer (microhh_amd/host); this module only
+declares the C entry points, builds the case struct and copies fields to numpy.
+It also holds the case presets of the dry convective boundary layer.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+MHH_FP64, MHH_FP32 = abi.MHH_FP64, abi.MHH_FP32
+
+
+class Case(C.Structure):
+    _fields_ = [
+        ("itot", C.c_int), ("jtot", C.c_int), ("ktot", C.c_int),
+        ("xsize", C.c_double), ("ysize", C.c_double), ("zsize", C.c_double),
+        ("advec", C.c_int), ("diff", C.c_int), ("thermo", C.c_int),
+        ("cflmax", C.c_double), ("dnmax", C.c_double), ("cs", C.c_double), ("tPr", C.c_double),
+        ("swmason", C.c_int),
+        ("visc", C.c_double), ("svisc", C.c_double), ("thref0", C.c_double),
+        ("mbcbot", C.c_int), ("mbctop", C.c_int), ("sbcbot", C.c_int), ("sbctop", C.c_int),
+        ("ubot", C.c_double), ("vbot", C.c_double), ("utop", C.c_double), ("vtop", C.c_double),
+        ("sbot", C.c_double), ("stop", C.c_double),
+        ("utrans", C.c_double), ("vtrans", C.c_double),
+        ("rndseed", C.c_int),
+        ("rndamp_u", C.c_double), ("rndamp_v", C.c_double), ("rndamp_w", C.c_double),
+        ("rndamp_th", C.c_double), ("rndz", C.c_double), ("rndexp", C.c_double),
+        ("adaptivestep", C.c_int),
+        ("dt", C.c_double), ("dtmax", C.c_double), ("endtime", C.c_double), ("savetime", C.c_double),
+        ("precision", C.c_int),
+    ]
+
+
+MODEL_SIGNATURES = {
+    "mhh_model_create": (C.c_int, [C.POINTER(Case), abi.DP, abi.DP, abi.DP, abi.DP, C.c_void_p,
+                                   C.POINTER(C.c_void_p)]),
+    "mhh_model_destroy": (C.c_int, [C.c_void_p]),
+    "mhh_model_substep": (C.c_int, [C.c_void_p]),
+    "mhh_model_steps": (C.c_int, [C.c_void_p, C.c_int]),
+    "mhh_model_field_ptr": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p)]),
+    "mhh_model_get_field": (C.c_int, [C.c_void_p, C.c_char_p, abi.DP]),
+    "mhh_model_set_field": (C.c_int, [C.c_void_p, C.c_char_p, abi.DP]),
+    "mhh_model_dims": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "mhh_model_time": (C.c_int, [C.c_void_p, abi.DP, abi.DP, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "mhh_model_cfl": (C.c_int, [C.c_void_p, abi.DP]),
+    "mhh_model_dn": (C.c_int, [C.c_void_p, abi.DP]),
+    "mhh_model_div": (C.c_int, [C.c_void_p, abi.DP]),
+    "mhh_model_ctx": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+}
+
+_declared = False
+
+
+def lib():
+    global _declared
+    L = abi.lib()
+    if not _declared:
+        for name, (res, args) in MODEL_SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _declared = True
+    return L
+
+
+CASE_KEYS = [f[0] for f in Case._fields_]
+
+
+def dry_cbl_case(itot=512, jtot=512, ktot=256, xsize=25600.0, ysize=25600.0, zsize=3200.0,
+                 precision=MHH_FP64, adaptive=True, dt=10.0, dtmax=60.0, rndamp_uvw=0.02, rndamp_th=0.2,
+                 rndz=None, rndexp=0.0, advec=0, diff=0, rndseed=2):
+    """The dry convective boundary layer of the hot path: advec_2i5 + smag2 +
+    pres_2, RK3, doubly periodic, Boussinesq theta with 300 K to 1000 m and a
+    0.006 K/m lapse rate above.  The surface is a fixed warm theta (Dirichlet,
+    301 K) because the "default" boundary switch has no surface layer."""
+    return dict(
+        itot=itot, jtot=jtot, ktot=ktot, xsize=xsize, ysize=ysize, zsize=zsize,
+        advec=advec, diff=diff, thermo=0, cflmax=1.2, dnmax=0.4, cs=0.23, tPr=1.0 / 3.0, swmason=1,
+        visc=1.0e-5, svisc=1.0e-5, thref0=300.0,
+        mbcbot=abi.BC["noslip"], mbctop=abi.BC["freeslip"], sbcbot=abi.BC["dirichlet"], sbctop=abi.BC["neumann"],
+        ubot=0.0, vbot=0.0, utop=0.0, vtop=0.0, sbot=301.0, stop=0.006, utrans=0.0, vtrans=0.0,
+        rndseed=rndseed, rndamp_u=rndamp_uvw, rndamp_v=rndamp_uvw, rndamp_w=rndamp_uvw, rndamp_th=rndamp_th,
+        rndz=zsize if rndz is None else rndz, rndexp=rndexp,
+        adaptivestep=1 if adaptive else 0, dt=dt, dtmax=dtmax, endtime=1.0e9, savetime=1.0e8,
+        precision=precision,
+    )
+
+
+def profiles(case):
+    """Uniform levels and the initial u, v, theta profiles of the case (ktot values each)."""
+    ktot, zsize = case["ktot"], case["zsize"]
+    dz = zsize / ktot
+    z = dz * (np.arange(ktot) + 0.5)
+    u = np.zeros(ktot)
+    v = np.zeros(ktot)
+    th = np.where(z <= 1000.0, 300.0, 300.0 + 0.006 * (z - 1000.0))
+    return z, u, v, th
+
+
+def oracle_case(case):
+    """The same case as the oracle's mo_case struct (test infrastructure)."""
+    from oracle import pyoracle as po
+    c = po.Case()
+    for name, _ in po.Case._fields_:
+        if name == "nthreads":
+            c.nthreads = 1
+        else:
+            setattr(c, name, case[name])
+    return c
+
+
+class Model:
+    """A dry-CBL model on the GPU, driven by the C++ host layer."""
+
+    def __init__(self, case, device=0, stream=None):
+        self.case = dict(case)
+        L = lib()
+        self._c = Case(*[self.case[k] for k in CASE_KEYS])
+        z, u, v, th = [np.ascontiguousarray(a, dtype=np.float64) for a in profiles(self.case)]
+        self._prof = (z, u, v, th)
+        h = C.c_void_p()
+        abi.check(L.mhh_model_create(C.byref(self._c), z.ctypes.data_as(abi.DP), u.ctypes.data_as(abi.DP),
+                                     v.ctypes.data_as(abi.DP), th.ctypes.data_as(abi.DP), stream, C.byref(h)),
+                  "mhh_model_create")
+        self.h = h
+        d6 = (C.c_int * 6)()
+        abi.check(L.mhh_model_dims(self.h, d6), "mhh_model_dims")
+        self.icells, self.jcells, self.kcells, self.istart, self.jstart, self.kstart = list(d6)
+
+    @property
+    def shape(self):
+        return (self.kcells, self.jcells, self.icells)
+
+    def substep(self):
+        abi.check(lib().mhh_model_substep(self.h), "mhh_model_substep")
+
+    def steps(self, n):
+        abi.check(lib().mhh_model_steps(self.h, n), "mhh_model_steps")
+
+    def field(self, name):
+        out = np.empty(self.shape, dtype=np.float64)
+        abi.check(lib().mhh_model_get_field(self.h, name.encode(), out.ctypes.data_as(abi.DP)), "get_field")
+        return out
+
+    def set_field(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        abi.check(lib().mhh_model_set_field(self.h, name.encode(), a.ctypes.data_as(abi.DP)), "set_field")
+
+    def field_ptr(self, name):
+        p = C.c_void_p()
+        abi.check(lib().mhh_model_field_ptr(self.h, name.encode(), C.byref(p)), "field_ptr")
+        return p.value
+
+    def time(self):
+        t, dt, s, it = C.c_double(), C.c_double(), C.c_int(), C.c_int()
+        abi.check(lib().mhh_model_time(self.h, C.byref(t), C.byref(dt), C.byref(s), C.byref(it)), "time")
+        return t.value, dt.value, s.value, it.value
+
+    def _scalar(self, fn):
+        v = C.c_double()
+        abi.check(getattr(lib(), fn)(self.h, C.byref(v)), fn)
+        return v.value
+
+    def cfl(self):
+        return self._scalar("mhh_model_cfl")
+
+    def dn(self):
+        return self._scalar("mhh_model_dn")
+
+    def div(self):
+        return self._scalar("mhh_model_div")
+
+    def ctx(self):
+        p = C.c_void_p()
+        abi.check(lib().mhh_model_ctx(self.h, C.byref(p)), "ctx")
+        return p
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mhh_model_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,282 @@
+"""Per-operator parity: the HIP kernels (through the C ABI) against the oracle.
+
+Inputs are the seeded fields of the golden fixtures (grids g1, g2) plus a
+larger seeded grid.  Metric: normwise max|gpu - oracle| / max|oracle| per
+field, tolerance 1e-12 (the per-substep tendency criterion); both are FP64.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests._golden import load, normwise
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _case(name):
+    if name in ("g1", "g2"):
+        p, i, _ = load(name)
+        return p, i
+    # a larger seeded case, built the same way as the fixtures
+    from tests.golden.make_golden import make_inputs
+    p, i = make_inputs(dict(itot=48, jtot=40, ktot=24, stretch=True, anelastic=True, dx=50.0, dy=40.0, seed=21))
+    return p, i
+
+
+CASES = ["g1", "g2", "big"]
+
+
+@pytest.fixture(params=CASES)
+def setup(request):
+    from tests._gpu import GpuCase
+    p, i = _case(request.param)
+    d = po.dims(p["itot"], p["jtot"], p["ktot"], p["igc"], p["jgc"], p["kgc"],
+                p["dx"] * p["itot"], p["dy"] * p["jtot"], float(i["zh"][p["kgc"] + p["ktot"]]))
+    thref = np.full(d.kcells, 300.0) + 0.01 * np.arange(d.kcells)
+    threfh = thref + 0.005
+    g = GpuCase(p["itot"], p["jtot"], p["ktot"], p["igc"], p["jgc"], p["kgc"], p["dx"], p["dy"], i,
+                nscalars=1, thref=thref, threfh=threfh)
+    yield p, i, d, g, thref, threfh
+    g.close()
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64).copy()
+
+
+def _load_common(g, i):
+    g.put("u", i["u"]); g.put("v", i["v"]); g.put("w", i["w"]); g.put("s0", i["s"])
+    g.put("ut", i["ut"]); g.put("vt", i["vt"]); g.put("wt", i["wt"]); g.put("st0", i["st"])
+
+
+@pytest.mark.parametrize("scheme", ["2i5", "2"])
+def test_advec_exec(setup, scheme):
+    p, i, d, g, *_ = setup
+    _load_common(g, i)
+    g.call(f"mhh_advec_{scheme}_exec")
+    L = po.lib()
+    args = [po.ptr(_c(i[k])) for k in ("u", "v", "w")]
+    rho, rhoh = po.ptr(_c(i["rhoref"])), po.ptr(_c(i["rhorefh"]))
+    for comp, tname, gname in (("u", "ut", "ut"), ("v", "vt", "vt"), ("w", "wt", "wt"), ("s", "st", "st0")):
+        ref = _c(i[tname])
+        dz = po.ptr(_c(i["dzhi"] if comp == "w" else i["dzi"]))
+        fn = getattr(L, f"mo_advec_{scheme}_{comp}")
+        if comp == "s":
+            fn(d, po.ptr(ref), po.ptr(_c(i["s"])), *args, dz, rho, rhoh)
+        else:
+            fn(d, po.ptr(ref), *args, dz, rho, rhoh)
+        err = normwise(g.get(gname), ref)
+        assert err < TOL, f"{scheme} {comp}: {err:.3e}"
+
+
+@pytest.mark.parametrize("scheme", ["2i5", "2"])
+def test_cfl(setup, scheme):
+    p, i, d, g, *_ = setup
+    _load_common(g, i)
+    out = C.c_double()
+    g.call(f"mhh_advec_{scheme}_cfl", p["dt"], C.byref(out))
+    ref = getattr(po.lib(), f"mo_advec_{scheme}_cfl")(d, po.ptr(_c(i["u"])), po.ptr(_c(i["v"])),
+                                                      po.ptr(_c(i["w"])), po.ptr(_c(i["dzi"])), p["dt"])
+    assert abs(out.value - ref) <= TOL * ref
+
+
+@pytest.mark.parametrize("sfc,thermo,mason", [(0, 1, 1), (0, 0, 1), (1, 1, 1), (1, 1, 0), (1, 0, 1)])
+def test_viscosity(setup, sfc, thermo, mason):
+    from microhh_amd import abi
+    p, i, d, g, thref, threfh = setup
+    _load_common(g, i)
+    rng = np.random.default_rng(5)
+    ev0 = rng.uniform(0.0, 1.0, d.kcells * d.jcells * d.icells).reshape(i["u"].shape)
+    planes = {n: rng.uniform(0.01, 0.1, (d.jcells, d.icells)) for n in ("dudz", "dvdz", "dbdz", "z0m")}
+    planes["dbdz"] -= 0.05
+    g.put("evisc", ev0)
+    for n, a in planes.items():
+        g.put(n, a)
+    # th with a stable lapse rate plus noise so that N2 varies in sign
+    th = i["s"] + 0.003 * np.broadcast_to(np.asarray(i["z"])[:, None, None], i["s"].shape)
+    g.put("s0", th)
+    par = abi.SmagParams(0.23, 1.0 / 3.0, 1e-5, (C.c_double * 8)(*([1e-5] * 8)), mason, sfc, thermo)
+    g.call("mhh_diff_smag2_exec_viscosity", C.byref(par))
+
+    L = po.lib()
+    ev = _c(ev0)
+    L.mo_calc_strain2(d, sfc, po.ptr(ev), po.ptr(_c(i["u"])), po.ptr(_c(i["v"])), po.ptr(_c(i["w"])),
+                      po.ptr(_c(planes["dudz"])), po.ptr(_c(planes["dvdz"])),
+                      po.ptr(_c(i["z"])), po.ptr(_c(i["dzi"])), po.ptr(_c(i["dzhi"])))
+    if thermo:
+        N2 = np.zeros_like(ev)
+        L.mo_calc_N2(d, po.ptr(N2), po.ptr(_c(th)), po.ptr(_c(i["dzi"])), po.ptr(_c(thref)))
+        L.mo_calc_evisc(d, sfc, mason, po.ptr(ev), po.ptr(N2), po.ptr(_c(planes["dbdz"])), po.ptr(_c(i["z"])),
+                        po.ptr(_c(i["dz"])), po.ptr(_c(planes["z0m"])), 0.23, 1.0 / 3.0)
+    else:
+        L.mo_calc_evisc_neutral(d, sfc, mason, po.ptr(ev), po.ptr(_c(i["u"])), po.ptr(_c(i["v"])),
+                                po.ptr(_c(i["z"])), po.ptr(_c(i["dz"])), po.ptr(_c(i["dzhi"])),
+                                po.ptr(_c(planes["z0m"])), 0.23, 1e-5)
+    err = normwise(g.get("evisc"), ev)
+    assert err < TOL, f"evisc sfc={sfc} thermo={thermo} mason={mason}: {err:.3e}"
+
+
+@pytest.mark.parametrize("sfc", [0, 1])
+def test_diffusion(setup, sfc):
+    from microhh_amd import abi
+    p, i, d, g, *_ = setup
+    _load_common(g, i)
+    g.put("evisc", i["evisc"])
+    rng = np.random.default_rng(6)
+    fl = {n: rng.uniform(-0.1, 0.1, (d.jcells, d.icells)) for n in
+          ("u_fluxbot", "u_fluxtop", "v_fluxbot", "v_fluxtop", "s_fluxbot0", "s_fluxtop0")}
+    for n, a in fl.items():
+        g.put(n, a)
+    svisc = [2e-5] * 8
+    par = abi.SmagParams(0.23, 1.0 / 3.0, 1.5e-5, (C.c_double * 8)(*svisc), 1, sfc, 1)
+    g.call("mhh_diff_smag2_exec", C.byref(par))
+    L = po.lib()
+    u, v, w = (po.ptr(_c(i[k])) for k in ("u", "v", "w"))
+    dzi, dzhi, ev = po.ptr(_c(i["dzi"])), po.ptr(_c(i["dzhi"])), po.ptr(_c(i["evisc"]))
+    rho, rhoh = po.ptr(_c(i["rhoref"])), po.ptr(_c(i["rhorefh"]))
+    refs = {}
+    refs["ut"] = _c(i["ut"])
+    L.mo_diff_u(d, sfc, po.ptr(refs["ut"]), u, v, w, dzi, dzhi, ev, po.ptr(_c(fl["u_fluxbot"])),
+                po.ptr(_c(fl["u_fluxtop"])), rho, rhoh, 1.5e-5)
+    refs["vt"] = _c(i["vt"])
+    L.mo_diff_v(d, sfc, po.ptr(refs["vt"]), u, v, w, dzi, dzhi, ev, po.ptr(_c(fl["v_fluxbot"])),
+                po.ptr(_c(fl["v_fluxtop"])), rho, rhoh, 1.5e-5)
+    refs["wt"] = _c(i["wt"])
+    L.mo_diff_w(d, po.ptr(refs["wt"]), u, v, w, dzi, dzhi, ev, rho, rhoh, 1.5e-5)
+    refs["st0"] = _c(i["st"])
+    L.mo_diff_c(d, sfc, po.ptr(refs["st0"]), po.ptr(_c(i["s"])), dzi, dzhi, ev, po.ptr(_c(fl["s_fluxbot0"])),
+                po.ptr(_c(fl["s_fluxtop0"])), rho, rhoh, 1.0 / 3.0, 2e-5)
+    for n, ref in refs.items():
+        err = normwise(g.get(n), ref)
+        assert err < TOL, f"diffusion {n} sfc={sfc}: {err:.3e}"
+
+
+def test_dnmul(setup):
+    p, i, d, g, *_ = setup
+    g.put("evisc", i["evisc"])
+    out = C.c_double()
+    g.call("mhh_diff_smag2_dnmul", 1.0 / 3.0, C.byref(out))
+    ref = po.lib().mo_calc_dnmul(d, po.ptr(_c(i["evisc"])), po.ptr(_c(i["dzi"])), 1.0 / 3.0)
+    assert abs(out.value - ref) <= TOL * ref
+
+
+def test_diff_2(setup):
+    p, i, d, g, *_ = setup
+    _load_common(g, i)
+    sv = (C.c_double * 8)(*([3e-5] * 8))
+    g.call("mhh_diff_2_exec", 2e-5, sv)
+    L = po.lib()
+    dzi, dzhi = po.ptr(_c(i["dzi"])), po.ptr(_c(i["dzhi"]))
+    for n, src, visc in (("ut", "u", 2e-5), ("vt", "v", 2e-5), ("st0", "s", 3e-5)):
+        ref = _c(i[n if n != "st0" else "st"])
+        L.mo_diff_2_c(d, po.ptr(ref), po.ptr(_c(i[src])), visc, dzi, dzhi)
+        assert normwise(g.get(n), ref) < TOL, n
+    ref = _c(i["wt"])
+    L.mo_diff_2_w(d, po.ptr(ref), po.ptr(_c(i["w"])), 2e-5, dzi, dzhi)
+    assert normwise(g.get("wt"), ref) < TOL
+
+
+def test_buoyancy(setup):
+    p, i, d, g, thref, threfh = setup
+    _load_common(g, i)
+    g.call("mhh_thermo_dry_exec")
+    ref = _c(i["wt"])
+    po.lib().mo_buoyancy_tend_2nd(d, po.ptr(ref), po.ptr(_c(i["s"])), po.ptr(_c(threfh)))
+    assert normwise(g.get("wt"), ref) < TOL
+
+
+@pytest.mark.parametrize("edge", [0, 1, 2])
+def test_cyclic(setup, edge):
+    p, i, d, g, *_ = setup
+    g.put("u", i["u"])
+    g.call("mhh_boundary_cyclic", g.ptr("u"), edge)
+    ref = _c(i["u"])
+    po.lib().mo_boundary_cyclic(d, po.ptr(ref), edge)
+    assert np.array_equal(g.get("u"), ref)
+
+
+@pytest.mark.parametrize("bot,top", [(0, 1), (1, 0), (2, 4), (0, 0)])
+def test_ghost_cells(setup, bot, top):
+    from microhh_amd import abi
+    p, i, d, g, *_ = setup
+    _load_common(g, i)
+    rng = np.random.default_rng(7)
+    planes = {}
+    for n in ("u_bot", "u_top", "u_gradbot", "u_gradtop", "v_bot", "v_top", "v_gradbot", "v_gradtop",
+              "s_bot0", "s_top0", "s_gradbot0", "s_gradtop0"):
+        planes[n] = rng.uniform(-1, 1, (d.jcells, d.icells))
+        g.put(n, planes[n])
+    bc = abi.BcDesc(bot, top, (C.c_int * 8)(*([bot] * 8)), (C.c_int * 8)(*([top] * 8)))
+    g.call("mhh_boundary_set_ghost_cells", C.byref(bc))
+    L = po.lib()
+    dzh = po.ptr(_c(i["dzh"]))
+    for fld, pre in (("u", "u_"), ("v", "v_"), ("s0", "s_")):
+        sfx = "0" if fld == "s0" else ""
+        ref = _c(i["s"] if fld == "s0" else i[fld])
+        top_plane = _c(planes[pre + "top" + sfx])
+        L.mo_ghost_cells_bot_2nd(d, po.ptr(ref), dzh, bot, po.ptr(_c(planes[pre + "bot" + sfx])),
+                                 po.ptr(_c(planes[pre + "gradbot" + sfx])))
+        L.mo_ghost_cells_top_2nd(d, po.ptr(ref), dzh, top, po.ptr(top_plane),
+                                 po.ptr(_c(planes[pre + "gradtop" + sfx])))
+        assert normwise(g.get(fld), ref) < 1e-15, fld
+
+
+def test_pres(setup):
+    p, i, d, g, *_ = setup
+    _load_common(g, i)
+    # make the halos of the prognostic fields periodic, as they are when pres runs
+    for n in ("u", "v", "w"):
+        g.call("mhh_boundary_cyclic", g.ptr(n), 2)
+    u, v, w = g.get("u"), g.get("v"), g.get("w")
+    dt = 0.37
+    g.call("mhh_pres_2_exec", dt)
+    L = po.lib()
+    ut, vt, wt = _c(i["ut"]), _c(i["vt"]), _c(i["wt"])
+    pfield, tmp1, tmp2 = np.zeros_like(ut), np.zeros_like(ut), np.zeros_like(ut)
+    bmati, bmatj = np.zeros(d.itot), np.zeros(d.jtot)
+    a, c = np.zeros(d.kmax), np.zeros(d.kmax)
+    L.mo_pres_2_set_values(d, po.ptr(bmati), po.ptr(bmatj), po.ptr(a), po.ptr(c), po.ptr(_c(i["dz"])),
+                           po.ptr(_c(i["dzhi"])), po.ptr(_c(i["rhorefh"])))
+    L.mo_pres_2_input(d, po.ptr(pfield), po.ptr(_c(u)), po.ptr(_c(v)), po.ptr(_c(w)), po.ptr(ut), po.ptr(vt),
+                      po.ptr(wt), po.ptr(_c(i["dzi"])), po.ptr(_c(i["rhoref"])), po.ptr(_c(i["rhorefh"])), dt)
+    L.mo_pres_2_solve(d, po.ptr(pfield), po.ptr(tmp1), po.ptr(tmp2), po.ptr(_c(i["dz"])), po.ptr(_c(i["rhoref"])),
+                      po.ptr(bmati), po.ptr(bmatj), po.ptr(a), po.ptr(c))
+    L.mo_pres_2_output(d, po.ptr(ut), po.ptr(vt), po.ptr(wt), po.ptr(pfield), po.ptr(_c(i["dzhi"])))
+    from tests._golden import interior
+    m = interior(d)
+    for n, ref in (("ut", ut), ("vt", vt), ("wt", wt)):
+        err = normwise(g.get(n), ref, m)
+        assert err < TOL, f"pres {n}: {err:.3e}"
+    # p on every level the reference defines (kstart-1 .. kend-1), all columns
+    pg = g.get("p")
+    ks, ke = d.kstart, d.kend
+    err = normwise(pg[ks - 1:ke], pfield[ks - 1:ke])
+    assert err < TOL, f"pres p: {err:.3e}"
+
+
+def test_divergence(setup):
+    p, i, d, g, *_ = setup
+    _load_common(g, i)
+    out = C.c_double()
+    g.call("mhh_pres_2_divergence", C.byref(out))
+    ref = po.lib().mo_pres_2_divergence(d, po.ptr(_c(i["u"])), po.ptr(_c(i["v"])), po.ptr(_c(i["w"])),
+                                        po.ptr(_c(i["dzi"])), po.ptr(_c(i["rhoref"])), po.ptr(_c(i["rhorefh"])))
+    assert abs(out.value - ref) <= TOL * ref
+
+
+@pytest.mark.parametrize("substep", [0, 1, 2])
+def test_rk3(setup, substep):
+    p, i, d, g, *_ = setup
+    _load_common(g, i)
+    dt = 0.61
+    g.call("mhh_timeloop_rk3", substep, dt)
+    L = po.lib()
+    for f, t, gf, gt in (("u", "ut", "u", "ut"), ("v", "vt", "v", "vt"), ("w", "wt", "w", "wt"),
+                         ("s", "st", "s0", "st0")):
+        a, at = _c(i[f]), _c(i[t])
+        L.mo_rk3(d, po.ptr(a), po.ptr(at), substep, dt)
+        assert normwise(g.get(gf), a) < 1e-15, f
+        assert normwise(g.get(gt), at) < 1e-15, t
