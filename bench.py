@@ -1,0 +1,185 @@
+"""Throughput of the dry-CBL dynamical core on one MI355X.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 2] [--precision fp64]
+
+Workload: the 512x512x256 dry convective boundary layer (25.6 x 25.6 x 3.2 km,
+FP64, advec_2i5 + smag2 + pres_2, RK3, adaptive dt at CFL 1.2) with seeded
+synthetic initial fields, run through the C++ model driver on the GPU.  A
+"step" is one RK3 time step (3 substeps).  value = interior grid points x
+substeps / wall time of the K timed steps, in Mpoint-substeps/s.
+
+The line also carries
+  roofline     : the dominant kernel's algorithmic HBM bytes per launch over
+                 its mean launch time (HIP events on the model's stream during
+                 the timed steps), against the 8 TB/s HBM3E peak;
+  substep_roofline : the whole substep's canonical 40 field passes per point
+                 over the measured substep time;
+  cpu_baseline : the FP64 CPU oracle (C restatement of the reference loops)
+                 timed on this host on a bounded sample of the same workload.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "Mpoint-substeps/s at 512²×256, FP64 vs mixed (≤1e-5 err); % HBM peak"
+UNIT = "Mpoint-substeps/s"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic HBM traffic per launch of each kernel, in full-field passes of
+# the element type over the interior points (reads + writes of distinct arrays):
+#   advec u/v/w: u, v, w, at in; at out = 5.  advec s: s, u, v, w, st; st = 6.
+#   evisc: u, v, w, th in; evisc out = 5.  diff u/v/w: u, v, w, evisc, at; at = 6.
+#   diff c: s, evisc, st; st = 4.  pres_in: u, v, w, ut, vt, wt; P = 7.
+#   fft x/y (fwd, bwd): spectrum in and out = 2.  tdma: spectrum in and out = 2.
+#   pres_out: p, ut, vt, wt; ut, vt, wt = 7.  rk3 (one field): a, at; a, at = 4.
+#   buoyancy: th, wt; wt = 3.
+PASSES = {
+    "advec_2i5_u": 5, "advec_2i5_v": 5, "advec_2i5_w": 5, "advec_2i5_s": 6,
+    "evisc": 5, "diff_u": 6, "diff_v": 6, "diff_w": 6, "diff_c": 4,
+    "pres_in": 7, "fft_x_fwd": 2, "fft_y_fwd": 2, "tdma": 2, "fft_y_bwd": 2, "fft_x_bwd": 2,
+    "pres_out": 7, "rk3": 4, "buoyancy": 3,
+}
+KERNELS = list(PASSES) + ["cyclic", "ghost_cells", "mirror_walls", "rk3_zero_ghosts", "cfl", "dnmul"]
+# The canonical fused minimum per grid-point-substep (40 passes).
+B_ALG_PASSES = 40
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(precision_bytes):
+    """Time the serial FP64 oracle on a bounded sample: one RK3 step (3 substeps)
+    of the same physics on a 192x192x256 slab at the workload's grid spacing."""
+    from microhh_amd import model as mm
+    from oracle import pyoracle as po
+    n = 192
+    case = mm.dry_cbl_case(itot=n, jtot=n, ktot=256, xsize=50.0 * n, ysize=50.0 * n, zsize=3200.0)
+    o = po.Model(mm.oracle_case(case), *mm.profiles(case))
+    t0 = time.perf_counter()
+    for _ in range(3):
+        o.substep()
+    el = time.perf_counter() - t0
+    o.close()
+    pts = n * n * 256 * 3
+    return {"value": round(pts / el / 1e6, 3), "unit": UNIT, "cores": 1, "kind": "port",
+            "sample": f"1 RK3 step (3 substeps) of the dry CBL on a {n}x{n}x256 slab (dx=dy=50 m, dz=12.5 m), "
+                      f"serial FP64 oracle (C restatement of the reference loops), {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--itot", type=int, default=512)
+    ap.add_argument("--jtot", type=int, default=512)
+    ap.add_argument("--ktot", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-timing", action="store_true", help="no per-kernel events in the timed steps")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a "
+                                   "design in DESIGN.md, not a measured path)"}))
+        sys.exit(2)
+
+    from microhh_amd import abi, model as mm
+    prec = mm.MHH_FP64 if args.precision == "fp64" else mm.MHH_FP32
+    esize = 8 if prec == mm.MHH_FP64 else 4
+    itot, jtot, ktot = args.itot, args.jtot, args.ktot
+    case = mm.dry_cbl_case(itot=itot, jtot=jtot, ktot=ktot, xsize=50.0 * itot, ysize=50.0 * jtot,
+                           zsize=12.5 * ktot, precision=prec)
+    log(f"bench: creating {itot}x{jtot}x{ktot} {args.precision} model")
+    m = mm.Model(case)
+    L = abi.lib()
+    ctx = m.ctx()
+
+    log(f"bench: {args.warmup} warmup steps")
+    m.steps(args.warmup)
+    abi.check(L.mhh_device_synchronize(), "sync")
+
+    timing = not args.no_timing
+    if timing:
+        abi.check(L.mhh_timing_enable(ctx, 1), "timing_enable")
+        abi.check(L.mhh_timing_reset(ctx), "timing_reset")
+    log(f"bench: {args.steps} timed steps")
+    abi.check(L.mhh_device_synchronize(), "sync")
+    t0 = time.perf_counter()
+    m.steps(args.steps)
+    abi.check(L.mhh_device_synchronize(), "sync")
+    elapsed = time.perf_counter() - t0
+
+    npts = itot * jtot * ktot
+    nsub = 3 * args.steps
+    value = npts * nsub / elapsed / 1e6
+    ms_per_step = elapsed / args.steps * 1e3
+
+    kernels = {}
+    if timing:
+        for k in KERNELS:
+            tot, cnt = C.c_double(), C.c_int()
+            abi.check(L.mhh_timing_query(ctx, k.encode(), C.byref(tot), C.byref(cnt)), "timing_query")
+            if cnt.value:
+                entry = {"ms_total": round(tot.value, 3), "launches": cnt.value,
+                         "ms_mean": round(tot.value / cnt.value, 4)}
+                if k in PASSES:
+                    nbytes = PASSES[k] * npts * esize
+                    if k.startswith("fft") or k == "tdma":
+                        nbytes = int(nbytes * (itot + 2) / itot)
+                    entry["alg_bytes"] = nbytes
+                    entry["gbs"] = round(nbytes / (tot.value / cnt.value * 1e-3) / 1e9, 1)
+                kernels[k] = entry
+        abi.check(L.mhh_timing_enable(ctx, 0), "timing_disable")
+
+    roof = None
+    if kernels:
+        dom = max((k for k in kernels if k in PASSES), key=lambda k: kernels[k]["ms_total"])
+        e = kernels[dom]
+        roof = {"bound": "hbm", "achieved": e["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(e["gbs"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "alg_bytes_per_launch": e["alg_bytes"], "ms_per_launch": e["ms_mean"]}
+    sub_ms = elapsed / nsub * 1e3
+    substep_gbs = B_ALG_PASSES * esize * npts / (sub_ms * 1e-3) / 1e9
+    div = m.div()
+    t, dt, _, it = m.time()
+    m.close()
+
+    cpu = None
+    if not args.no_cpu:
+        log("bench: CPU baseline (serial oracle, bounded sample)")
+        try:
+            cpu = cpu_baseline(esize)
+        except Exception as exc:  # the oracle library is test infrastructure; report, do not fail the bench
+            cpu = {"error": str(exc)}
+
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": UNIT, "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if prec == mm.MHH_FP64 else "f32", "data": "synthetic",
+        "config": {"workload": f"dry CBL LES {itot}x{jtot}x{ktot} {args.precision.upper()}, advec_2i5 + smag2 + "
+                               f"pres_2 (FFT + tridiagonal), RK3, adaptive dt (CFL 1.2), seeded noise fields",
+                   "itot": itot, "jtot": jtot, "ktot": ktot, "substeps_per_step": 3,
+                   "parallelism": "single GPU"},
+        "roofline": roof,
+        "substep_roofline": {"alg_passes": B_ALG_PASSES, "ms_per_substep": round(sub_ms, 3),
+                             "achieved": round(substep_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": round(substep_gbs / HBM_PEAK_GBS, 4)},
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+        "state": {"model_time_s": round(t, 3), "dt_s": round(dt, 4), "iteration": it, "div_max": div},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
