@@ -1,0 +1,296 @@
+// mhh_fft.h -- complex Stockham FFT of one row in LDS, executed by one
+// 64-lane wavefront, for the spectral pressure solver.
+//
+// Two engines share a Plan:
+//   fast (in place): every pass reads the lane's butterfly inputs into
+//     registers, applies twiddles from an LDS table and a radix-8/4/2/3/5 DFT,
+//     then (after a barrier) writes the outputs back into the same row buffer.
+//     Used when every radix is in {2, 3, 4, 5, 8} and no pass has more than
+//     256 butterflies (<= 4 per lane), i.e. n <= 1024 for powers of two.
+//   ping-pong: the general fallback for any radix up to 13 (two row buffers).
+// tw[t] = exp(-2 pi i t / n); sign = -1 forward, +1 inverse (unnormalised).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace mhh
+{
+    constexpr int FFT_MAXPASS = 16;
+    constexpr int FFT_MAXRADIX = 13;
+    constexpr int FFT_MAXB = 4;     // butterflies per lane and pass in the fast engine
+
+    struct Plan
+    {
+        int n;
+        int npass;
+        int fast;
+        int radix[FFT_MAXPASS];
+    };
+
+    // Factor n into radices: as many 8s as possible (a leftover 2 turns one
+    // 8 into 4*4), then 3, 5 and larger primes.
+    inline bool make_plan(int n, Plan& p)
+    {
+        p.n = n; p.npass = 0; p.fast = 1;
+        int m = n, e = 0;
+        while (m % 2 == 0) { m /= 2; ++e; }
+        int n8 = e / 3, rem = e % 3;
+        if (rem == 1 && n8 > 0) { --n8; p.radix[p.npass++] = 4; p.radix[p.npass++] = 4; }
+        else if (rem == 1) p.radix[p.npass++] = 2;
+        else if (rem == 2) p.radix[p.npass++] = 4;
+        for (int q = 0; q < n8; ++q) p.radix[p.npass++] = 8;
+        const int odd[] = {3, 5, 7, 11, 13};
+        for (int r : odd)
+            while (m % r == 0)
+            {
+                if (p.npass >= FFT_MAXPASS) return false;
+                p.radix[p.npass++] = r; m /= r;
+            }
+        if (m != 1) return false;
+        for (int q = 0; q < p.npass; ++q)
+        {
+            const int r = p.radix[q];
+            if (!(r == 2 || r == 3 || r == 4 || r == 5 || r == 8) || n / r > 64*FFT_MAXB)
+                p.fast = 0;
+        }
+        return true;
+    }
+
+    template<typename TF> struct C2;
+    template<> struct C2<double> { using type = double2; };
+    template<> struct C2<float> { using type = float2; };
+    template<typename TF> using cplx = typename C2<TF>::type;
+
+    template<typename T> __device__ __forceinline__ T cmul(T a, T b)
+    { T r; r.x = a.x*b.x - a.y*b.y; r.y = a.x*b.y + a.y*b.x; return r; }
+    template<typename T> __device__ __forceinline__ T cadd(T a, T b) { T r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+    template<typename T> __device__ __forceinline__ T csub(T a, T b) { T r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+    // multiply by sign*i
+    template<typename T> __device__ __forceinline__ T cmuli(T a, int sign)
+    { T r; r.x = -sign*a.y; r.y = sign*a.x; return r; }
+
+    // In-register DFT of size R, sign -1 forward / +1 inverse.
+    template<typename TF, int R>
+    __device__ __forceinline__ void dft(cplx<TF>* v, int sign)
+    {
+        using T = cplx<TF>;
+        if constexpr (R == 2)
+        {
+            const T a = v[0], b = v[1];
+            v[0] = cadd(a, b); v[1] = csub(a, b);
+        }
+        else if constexpr (R == 4)
+        {
+            const T t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+            const T t2 = cadd(v[1], v[3]), t3 = cmuli(csub(v[1], v[3]), sign);
+            v[0] = cadd(t0, t2); v[2] = csub(t0, t2);
+            v[1] = cadd(t1, t3); v[3] = csub(t1, t3);
+        }
+        else if constexpr (R == 8)
+        {
+            // split into even/odd DFT4s, combine with W8^q
+            T e[4] = {v[0], v[2], v[4], v[6]};
+            T o[4] = {v[1], v[3], v[5], v[7]};
+            dft<TF, 4>(e, sign);
+            dft<TF, 4>(o, sign);
+            const TF h = TF(0.70710678118654752440);
+            T w1; w1.x = h*(o[1].x - sign*o[1].y); w1.y = h*(o[1].y + sign*o[1].x);     // W8^1 * o1
+            T w2 = cmuli(o[2], sign);                                                   // W8^2 * o2
+            T w3; w3.x = h*(-o[3].x - sign*o[3].y); w3.y = h*(-o[3].y + sign*o[3].x);   // W8^3 * o3
+            v[0] = cadd(e[0], o[0]); v[4] = csub(e[0], o[0]);
+            v[1] = cadd(e[1], w1);   v[5] = csub(e[1], w1);
+            v[2] = cadd(e[2], w2);   v[6] = csub(e[2], w2);
+            v[3] = cadd(e[3], w3);   v[7] = csub(e[3], w3);
+        }
+        else if constexpr (R == 3)
+        {
+            const TF h = TF(0.86602540378443864676);  // sqrt(3)/2
+            const T s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+            const T y0 = cadd(v[0], s);
+            T m; m.x = v[0].x - TF(0.5)*s.x; m.y = v[0].y - TF(0.5)*s.y;
+            T id = cmuli(d, sign); id.x *= h; id.y *= h;
+            v[0] = y0; v[1] = cadd(m, id); v[2] = csub(m, id);
+        }
+        else if constexpr (R == 5)
+        {
+            const TF c1 = TF(0.30901699437494742410), c2 = TF(-0.80901699437494742410);
+            const TF s1 = TF(0.95105651629515357212), s2 = TF(0.58778525229247312917);
+            const T p14 = cadd(v[1], v[4]), m14 = csub(v[1], v[4]);
+            const T p23 = cadd(v[2], v[3]), m23 = csub(v[2], v[3]);
+            const T y0 = cadd(cadd(v[0], p14), p23);
+            T r1; r1.x = v[0].x + c1*p14.x + c2*p23.x; r1.y = v[0].y + c1*p14.y + c2*p23.y;
+            T r2; r2.x = v[0].x + c2*p14.x + c1*p23.x; r2.y = v[0].y + c2*p14.y + c1*p23.y;
+            T q1; q1.x = s1*m14.x + s2*m23.x; q1.y = s1*m14.y + s2*m23.y;
+            T q2; q2.x = s2*m14.x - s1*m23.x; q2.y = s2*m14.y - s1*m23.y;
+            q1 = cmuli(q1, sign); q2 = cmuli(q2, sign);
+            v[0] = y0; v[1] = cadd(r1, q1); v[4] = csub(r1, q1);
+            v[2] = cadd(r2, q2); v[3] = csub(r2, q2);
+        }
+    }
+
+    template<typename TF, int R>
+    __device__ __forceinline__ void pass_load(cplx<TF>* v, const cplx<TF>* a, const cplx<TF>* tw,
+                                              int j, int nr, int k, int tstride, int sign)
+    {
+        using T = cplx<TF>;
+        #pragma unroll
+        for (int r = 0; r < R; ++r)
+            v[r] = a[j + r*nr];
+        if (k > 0)
+        {
+            #pragma unroll
+            for (int r = 1; r < R; ++r)
+            {
+                T w = tw[r*k*tstride];
+                if (sign > 0) w.y = -w.y;
+                v[r] = cmul(v[r], w);
+            }
+        }
+        dft<TF, R>(v, sign);
+    }
+
+    template<typename TF, int R>
+    __device__ __forceinline__ void pass_store(cplx<TF>* a, const cplx<TF>* v, int j, int Ns, int k)
+    {
+        const int idxD = (j / Ns)*Ns*R + k;
+        #pragma unroll
+        for (int r = 0; r < R; ++r)
+            a[idxD + r*Ns] = v[r];
+    }
+
+    // Fast engine: in place in row a; tw is the LDS twiddle table of length n.
+    // Contains block barriers: every thread of the block must call it.
+    template<typename TF>
+    __device__ void fft_row_inplace(cplx<TF>* a, const cplx<TF>* tw, const Plan& plan, int sign, int lane)
+    {
+        using T = cplx<TF>;
+        const int n = plan.n;
+        int Ns = 1;
+        for (int p = 0; p < plan.npass; ++p)
+        {
+            const int R = plan.radix[p];
+            const int nr = n / R;
+            const int tstride = n / (Ns*R);
+            T v[FFT_MAXB][8];
+            #pragma unroll
+            for (int b = 0; b < FFT_MAXB; ++b)
+            {
+                const int j = lane + 64*b;
+                if (j < nr)
+                {
+                    const int k = j % Ns;
+                    switch (R)
+                    {
+                        case 8: pass_load<TF, 8>(v[b], a, tw, j, nr, k, tstride, sign); break;
+                        case 4: pass_load<TF, 4>(v[b], a, tw, j, nr, k, tstride, sign); break;
+                        case 2: pass_load<TF, 2>(v[b], a, tw, j, nr, k, tstride, sign); break;
+                        case 3: pass_load<TF, 3>(v[b], a, tw, j, nr, k, tstride, sign); break;
+                        default: pass_load<TF, 5>(v[b], a, tw, j, nr, k, tstride, sign); break;
+                    }
+                }
+            }
+            __syncthreads();
+            #pragma unroll
+            for (int b = 0; b < FFT_MAXB; ++b)
+            {
+                const int j = lane + 64*b;
+                if (j < nr)
+                {
+                    const int k = j % Ns;
+                    switch (R)
+                    {
+                        case 8: pass_store<TF, 8>(a, v[b], j, Ns, k); break;
+                        case 4: pass_store<TF, 4>(a, v[b], j, Ns, k); break;
+                        case 2: pass_store<TF, 2>(a, v[b], j, Ns, k); break;
+                        case 3: pass_store<TF, 3>(a, v[b], j, Ns, k); break;
+                        default: pass_store<TF, 5>(a, v[b], j, Ns, k); break;
+                    }
+                }
+            }
+            __syncthreads();
+            Ns *= R;
+        }
+    }
+
+    // Fallback engine: ping-pong between a and b, twiddles from the global
+    // table; returns the buffer holding the result.  Contains block barriers.
+    template<typename TF>
+    __device__ cplx<TF>* fft_row_pingpong(cplx<TF>* a, cplx<TF>* b, const Plan& plan,
+                                          const double2* __restrict__ tw, int sign, int lane)
+    {
+        using T = cplx<TF>;
+        const int n = plan.n;
+        int Ns = 1;
+        for (int p = 0; p < plan.npass; ++p)
+        {
+            const int R = plan.radix[p];
+            const int nr = n / R;
+            const int tstride = n / (Ns*R);
+            for (int j = lane; j < nr; j += 64)
+            {
+                const int k = j % Ns;
+                T v[FFT_MAXRADIX];
+                for (int r = 0; r < R; ++r)
+                {
+                    T x = a[j + r*nr];
+                    if (r > 0 && k > 0)
+                    {
+                        const double2 w = tw[r*k*tstride];
+                        T wt; wt.x = TF(w.x); wt.y = TF(sign < 0 ? w.y : -w.y);
+                        x = cmul(x, wt);
+                    }
+                    v[r] = x;
+                }
+                T y[FFT_MAXRADIX];
+                const int step = n / R;
+                for (int q = 0; q < R; ++q)
+                {
+                    T acc = v[0];
+                    for (int r = 1; r < R; ++r)
+                    {
+                        const double2 w = tw[((r*q) % R)*step];
+                        T wt; wt.x = TF(w.x); wt.y = TF(sign < 0 ? w.y : -w.y);
+                        acc = cadd(acc, cmul(v[r], wt));
+                    }
+                    y[q] = acc;
+                }
+                const int idxD = (j / Ns)*Ns*R + k;
+                for (int r = 0; r < R; ++r)
+                    b[idxD + r*Ns] = y[r];
+            }
+            __syncthreads();
+            T* t = a; a = b; b = t;
+            Ns *= R;
+        }
+        return a;
+    }
+
+    // Row FFT through whichever engine the plan allows.  Row layout per block:
+    // fast plans use one buffer of n per row, the fallback two (a, a + n).
+    template<typename TF>
+    __device__ __forceinline__ cplx<TF>* fft_row(cplx<TF>* a, const cplx<TF>* tw_lds, const double2* __restrict__ tw_glob,
+                                                 const Plan& plan, int sign, int lane)
+    {
+        if (plan.fast)
+        {
+            fft_row_inplace<TF>(a, tw_lds, plan, sign, lane);
+            return a;
+        }
+        return fft_row_pingpong<TF>(a, a + plan.n, plan, tw_glob, sign, lane);
+    }
+
+    // Copy the twiddle table into LDS (all threads of the block participate).
+    template<typename TF>
+    __device__ __forceinline__ void load_twiddles(cplx<TF>* tw_lds, const double2* __restrict__ tw, int n)
+    {
+        const int nthr = blockDim.x*blockDim.y;
+        for (int t = threadIdx.x + blockDim.x*threadIdx.y; t < n; t += nthr)
+        {
+            cplx<TF> w; w.x = TF(tw[t].x); w.y = TF(tw[t].y);
+            tw_lds[t] = w;
+        }
+    }
+
+    // Elements per row buffer (complex) for a plan.
+    __host__ __device__ inline int row_elems(const Plan& p) { return p.fast ? p.n : 2*p.n; }
+}

@@ -22,139 +22,11 @@
 #include <vector>
 
 #include "mhh_internal.h"
+#include "mhh_fft.h"
 
 namespace
 {
     using namespace mhh;
-
-    constexpr int MAXPASS = 16;
-    constexpr int MAXRADIX = 13;
-
-    struct Plan
-    {
-        int n;
-        int npass;
-        int radix[MAXPASS];
-    };
-
-    bool make_plan(int n, Plan& p)
-    {
-        p.n = n;
-        p.npass = 0;
-        int m = n;
-        while (m % 4 == 0 && m > 4) { p.radix[p.npass++] = 4; m /= 4; }
-        const int rs[] = {4, 2, 3, 5, 7, 11, 13};
-        for (int r : rs)
-            while (m % r == 0) { if (p.npass >= MAXPASS) return false; p.radix[p.npass++] = r; m /= r; }
-        return m == 1;
-    }
-
-    template<typename TF> struct C2;
-    template<> struct C2<double> { using type = double2; };
-    template<> struct C2<float> { using type = float2; };
-
-    template<typename TF> using cplx = typename C2<TF>::type;
-
-    template<typename T> __device__ __forceinline__ T cmul(T a, T b)
-    { T r; r.x = a.x*b.x - a.y*b.y; r.y = a.x*b.y + a.y*b.x; return r; }
-    template<typename T> __device__ __forceinline__ T cadd(T a, T b) { T r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
-    template<typename T> __device__ __forceinline__ T csub(T a, T b) { T r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
-    // multiply by sign*i
-    template<typename T> __device__ __forceinline__ T cmuli(T a, int sign)
-    { T r; r.x = -sign*a.y; r.y = sign*a.x; return r; }
-
-    // In-LDS Stockham FFT of one row of length plan.n, executed by the 64 lanes
-    // of one wavefront.  src/dst ping-pong; returns the buffer holding the result.
-    // tw[t] = exp(-2 pi i t / n); sign = -1 forward, +1 inverse (unnormalised).
-    template<typename TF>
-    __device__ cplx<TF>* fft_row(cplx<TF>* a, cplx<TF>* b, const Plan& plan,
-                                 const double2* __restrict__ tw, int sign, int lane)
-    {
-        using T = cplx<TF>;
-        const int n = plan.n;
-        int Ns = 1;
-        for (int p = 0; p < plan.npass; ++p)
-        {
-            const int R = plan.radix[p];
-            const int nr = n / R;
-            const int tstride = n / (Ns*R);
-            for (int j = lane; j < nr; j += 64)
-            {
-                const int k = j % Ns;
-                T v[MAXRADIX];
-                for (int r = 0; r < R; ++r)
-                {
-                    T x = a[j + r*nr];
-                    if (r > 0 && k > 0)
-                    {
-                        const double2 w = tw[(r*k*tstride) % n];
-                        T wt; wt.x = TF(w.x); wt.y = TF(sign < 0 ? w.y : -w.y);
-                        x = cmul(x, wt);
-                    }
-                    v[r] = x;
-                }
-                T y[MAXRADIX];
-                if (R == 2)
-                {
-                    y[0] = cadd(v[0], v[1]); y[1] = csub(v[0], v[1]);
-                }
-                else if (R == 4)
-                {
-                    const T t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
-                    const T t2 = cadd(v[1], v[3]), t3 = cmuli(csub(v[1], v[3]), sign);
-                    y[0] = cadd(t0, t2); y[2] = csub(t0, t2);
-                    y[1] = cadd(t1, t3); y[3] = csub(t1, t3);
-                }
-                else if (R == 3)
-                {
-                    const TF h = TF(0.86602540378443864676);  // sqrt(3)/2
-                    const T s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
-                    y[0] = cadd(v[0], s);
-                    T m; m.x = v[0].x - TF(0.5)*s.x; m.y = v[0].y - TF(0.5)*s.y;
-                    T id = cmuli(d, sign); id.x *= h; id.y *= h;
-                    y[1] = cadd(m, id); y[2] = csub(m, id);
-                }
-                else if (R == 5)
-                {
-                    const TF c1 = TF(0.30901699437494742410), c2 = TF(-0.80901699437494742410);
-                    const TF s1 = TF(0.95105651629515357212), s2 = TF(0.58778525229247312917);
-                    const T p14 = cadd(v[1], v[4]), m14 = csub(v[1], v[4]);
-                    const T p23 = cadd(v[2], v[3]), m23 = csub(v[2], v[3]);
-                    y[0] = cadd(cadd(v[0], p14), p23);
-                    T r1; r1.x = v[0].x + c1*p14.x + c2*p23.x; r1.y = v[0].y + c1*p14.y + c2*p23.y;
-                    T r2; r2.x = v[0].x + c2*p14.x + c1*p23.x; r2.y = v[0].y + c2*p14.y + c1*p23.y;
-                    T q1; q1.x = s1*m14.x + s2*m23.x; q1.y = s1*m14.y + s2*m23.y;
-                    T q2; q2.x = s2*m14.x - s1*m23.x; q2.y = s2*m14.y - s1*m23.y;
-                    q1 = cmuli(q1, sign); q2 = cmuli(q2, sign);
-                    y[1] = cadd(r1, q1); y[4] = csub(r1, q1);
-                    y[2] = cadd(r2, q2); y[3] = csub(r2, q2);
-                }
-                else
-                {
-                    // generic small prime: y_q = sum_r v_r exp(sign 2 pi i r q / R)
-                    const int step = n / R;
-                    for (int q = 0; q < R; ++q)
-                    {
-                        T acc = v[0];
-                        for (int r = 1; r < R; ++r)
-                        {
-                            const double2 w = tw[((r*q) % R)*step];
-                            T wt; wt.x = TF(w.x); wt.y = TF(sign < 0 ? w.y : -w.y);
-                            acc = cadd(acc, cmul(v[r], wt));
-                        }
-                        y[q] = acc;
-                    }
-                }
-                const int idxD = (j / Ns)*Ns*R + k;
-                for (int r = 0; r < R; ++r)
-                    b[idxD + r*Ns] = y[r];
-            }
-            __syncthreads();
-            T* t = a; a = b; b = t;
-            Ns *= R;
-        }
-        return a;
-    }
 
     // Pres_2::input (src/pres_2.cxx:155-196): divergence of u/dt + ut into the
     // dense [k][j][i] array P.  The periodic neighbours of ut and vt at the east
@@ -187,6 +59,25 @@ namespace
     // slot each as (row a, row b); every other kx holds (Xa, Xb) in two slots.
     __device__ __forceinline__ int stage_base(int kx, int n) { return (n & 1) ? 2*kx - 1 : 2*kx; }
 
+    constexpr int MAXQ = 9;   // (n/2 + 1)/64 rounded up for the largest fast plan (n = 1024)
+
+    template<typename TF>
+    __device__ __forceinline__ void unpack_pair(cplx<TF> z, cplx<TF> c, cplx<TF>& xa, cplx<TF>& xb)
+    {
+        // spectra of the real rows a, b from Z = FFT(a + i b): Xa = (Z[k] + conj(Z[-k]))/2,
+        // Xb = (Z[k] - conj(Z[-k]))/(2i)
+        xa.x = (z.x + c.x)*TF(0.5); xa.y = (z.y - c.y)*TF(0.5);
+        xb.x = (z.y + c.y)*TF(0.5); xb.y = (c.x - z.x)*TF(0.5);
+    }
+
+    template<typename TF>
+    __device__ __forceinline__ void stage_store(cplx<TF>* S, int kx, int n, cplx<TF> xa, cplx<TF> xb)
+    {
+        if (kx == 0) { cplx<TF> d; d.x = xa.x; d.y = xb.x; S[0] = d; }
+        else if (2*kx == n) { cplx<TF> d; d.x = xa.x; d.y = xb.x; S[1] = d; }
+        else { const int b = stage_base(kx, n); S[b] = xa; S[b+1] = xb; }
+    }
+
     // Forward x transform: RB complex rows per block, each packing two real
     // rows of P at level k; writes C[k][kx][j] for kx <= itot/2.
     template<typename TF>
@@ -195,14 +86,15 @@ namespace
     {
         using T = cplx<TF>;
         extern __shared__ unsigned char smem_raw[];
-        T* smem = reinterpret_cast<T*>(smem_raw);
-        const int n = plan.n, nx = n/2 + 1, M = g.jtot;
+        T* tws = reinterpret_cast<T*>(smem_raw);
+        const int n = plan.n, nx = n/2 + 1, M = g.jtot, re = row_elems(plan);
+        T* rows = tws + n;
         const int row = threadIdx.y, lane = threadIdx.x;
         const int k = blockIdx.y;
         const int j0 = blockIdx.x*2*RB;              // first real row of the block
         const int ja = j0 + 2*row, jb = ja + 1;
-        T* A = smem + row*2*n;
-        T* B = A + n;
+        T* A = rows + row*re;
+        if (plan.fast) load_twiddles<TF>(tws, tw, n);
         const TF* Pk = P + static_cast<size_t>(k)*n*M;
         for (int x = lane; x < n; x += 64)
         {
@@ -212,24 +104,38 @@ namespace
             A[x] = z;
         }
         __syncthreads();
-        T* Z = fft_row<TF>(A, B, plan, tw, -1, lane);
-        // Every row ran the same passes, so the result buffer has the same parity
-        // in all rows; the other buffer is free for staging.
-        const int free_off = (Z == A) ? n : 0;
-        T* S = smem + row*2*n + free_off;
-        for (int kx = lane; kx < nx; kx += 64)
+        T* Z = fft_row<TF>(A, tws, tw, plan, -1, lane);
+        // Stage the two half-spectra packed in n slots: in the row buffer itself
+        // for the in-place engine (through registers), else in the free buffer.
+        int soff;
+        if (plan.fast)
         {
-            const T z = Z[kx];
-            const T c = Z[(n - kx) % n];
-            T xa, xb;
-            xa.x = (z.x + c.x)*TF(0.5); xa.y = (z.y - c.y)*TF(0.5);
-            xb.x = (z.y + c.y)*TF(0.5); xb.y = (c.x - z.x)*TF(0.5);
-            if (kx == 0)
-            { T d; d.x = xa.x; d.y = xb.x; S[0] = d; }
-            else if (2*kx == n)
-            { T d; d.x = xa.x; d.y = xb.x; S[1] = d; }
-            else
-            { const int b = stage_base(kx, n); S[b] = xa; S[b+1] = xb; }
+            T xs[MAXQ][2];
+            #pragma unroll
+            for (int q = 0; q < MAXQ; ++q)
+            {
+                const int kx = lane + 64*q;
+                if (kx < nx) unpack_pair<TF>(Z[kx], Z[(n - kx) % n], xs[q][0], xs[q][1]);
+            }
+            __syncthreads();
+            #pragma unroll
+            for (int q = 0; q < MAXQ; ++q)
+            {
+                const int kx = lane + 64*q;
+                if (kx < nx) stage_store<TF>(A, kx, n, xs[q][0], xs[q][1]);
+            }
+            soff = 0;
+        }
+        else
+        {
+            soff = (Z == A) ? n : 0;
+            T* S = rows + row*re + soff;
+            for (int kx = lane; kx < nx; kx += 64)
+            {
+                T xa, xb;
+                unpack_pair<TF>(Z[kx], Z[(n - kx) % n], xa, xb);
+                stage_store<TF>(S, kx, n, xa, xb);
+            }
         }
         __syncthreads();
         // The block writes C[k][kx][j0 .. j0+2RB) with consecutive threads on consecutive j.
@@ -242,7 +148,7 @@ namespace
             const int r = jl >> 1, part = jl & 1;
             const int j = j0 + jl;
             if (j >= M) continue;
-            const T* Sr = smem + r*2*n + free_off;
+            const T* Sr = rows + r*re + soff;
             T val;
             if (kx == 0 || 2*kx == n)
             {
@@ -263,18 +169,18 @@ namespace
     {
         using T = cplx<TF>;
         extern __shared__ unsigned char smem_raw[];
-        T* smem = reinterpret_cast<T*>(smem_raw);
-        const int n = plan.n;
+        T* tws = reinterpret_cast<T*>(smem_raw);
+        const int n = plan.n, re = row_elems(plan);
         const int row = threadIdx.y, lane = threadIdx.x;
         const int r = blockIdx.x*RB + row;
-        T* A = smem + row*2*n;
-        T* B = A + n;
+        T* A = tws + n + row*re;
+        if (plan.fast) load_twiddles<TF>(tws, tw, n);
         const bool active = r < nrows;
         T* Cr = C + static_cast<size_t>(active ? r : 0)*n;
         if (active)
             for (int x = lane; x < n; x += 64) A[x] = Cr[x];
         __syncthreads();
-        T* Z = fft_row<TF>(A, B, plan, tw, sign, lane);
+        T* Z = fft_row<TF>(A, tws, tw, plan, sign, lane);
         if (!active) return;
         if (sign < 0)
             for (int x = lane; x < n; x += 64) Cr[x] = Z[x];
@@ -349,16 +255,20 @@ namespace
     {
         using T = cplx<TF>;
         extern __shared__ unsigned char smem_raw[];
-        T* smem = reinterpret_cast<T*>(smem_raw);
-        const int n = plan.n, nx = n/2 + 1, M = g.jtot;
+        T* tws = reinterpret_cast<T*>(smem_raw);
+        const int n = plan.n, nx = n/2 + 1, M = g.jtot, re = row_elems(plan);
+        T* rows = tws + n;
         const int row = threadIdx.y, lane = threadIdx.x;
         const int k = blockIdx.y;
         const int j0 = blockIdx.x*2*RB;
         const int nthr = 64*blockDim.y, tid = lane + 64*row;
         const int JB = 2*RB;
+        if (plan.fast) load_twiddles<TF>(tws, tw, n);
         const T* Ck = C + static_cast<size_t>(k)*nx*M;
-        // Stage the block's spectra into each row's B buffer, packed as in the
-        // forward kernel; the DC and Nyquist entries of a real row are real.
+        // Stage the block's spectra packed as in the forward kernel; the DC and
+        // Nyquist entries of a real row are real.  The staging buffer is the row
+        // buffer itself (in-place engine) or the row's second buffer (fallback).
+        const int soff = plan.fast ? 0 : n;
         for (int idx = tid; idx < nx*JB; idx += nthr)
         {
             const int kx = idx / JB, jl = idx % JB;
@@ -367,7 +277,7 @@ namespace
             T val;
             if (j < M) val = Ck[static_cast<size_t>(kx)*M + j];
             else { val.x = TF(0); val.y = TF(0); }
-            T* Sr = smem + r*2*n + n;
+            T* Sr = rows + r*re + soff;
             if (kx == 0 || 2*kx == n)
             {
                 TF* d = reinterpret_cast<TF*>(&Sr[kx == 0 ? 0 : 1]);
@@ -377,11 +287,10 @@ namespace
                 Sr[stage_base(kx, n) + part] = val;
         }
         __syncthreads();
-        T* A = smem + row*2*n;
-        T* B = A + n;
+        T* A = rows + row*re;
+        const T* B = A + soff;
         // Hermitian extension of both spectra and packing Z = Xa + i Xb.
-        for (int x = lane; x < n; x += 64)
-        {
+        auto zval = [&](int x) -> T {
             T xa, xb;
             const int kx = (x <= n/2) ? x : n - x;
             if (kx == 0 || 2*kx == n)
@@ -396,10 +305,30 @@ namespace
                 if (x > n/2) { xa.y = -xa.y; xb.y = -xb.y; }
             }
             T z; z.x = xa.x - xb.y; z.y = xa.y + xb.x;
-            A[x] = z;
+            return z;
+        };
+        if (plan.fast)
+        {
+            constexpr int MAXX = 2*MAXQ;   // n/64 values per lane, n <= 1024
+            T zs[MAXX];
+            #pragma unroll
+            for (int q = 0; q < MAXX; ++q)
+            {
+                const int x = lane + 64*q;
+                if (x < n) zs[q] = zval(x);
+            }
+            __syncthreads();
+            #pragma unroll
+            for (int q = 0; q < MAXX; ++q)
+            {
+                const int x = lane + 64*q;
+                if (x < n) A[x] = zs[q];
+            }
         }
+        else
+            for (int x = lane; x < n; x += 64) A[x] = zval(x);
         __syncthreads();
-        T* Z = fft_row<TF>(A, B, plan, tw, +1, lane);
+        T* Z = fft_row<TF>(A, tws, tw, plan, +1, lane);
         const int ja = j0 + 2*row, jb = ja + 1;
         const int jj = g.icells, kk = g.ijcells;
         const int kc = k + g.kgc;
@@ -408,10 +337,10 @@ namespace
             const int j = pr ? jb : ja;
             if (j >= M) continue;
             // destination rows: the interior row and, within jgc of an edge, its periodic image
-            int rows[3]; int nrow = 0;
-            rows[nrow++] = j + g.jgc;
-            if (j >= M - g.jgc) rows[nrow++] = j - (M - g.jgc);
-            if (j < g.jgc) rows[nrow++] = g.jend + j;
+            int rows_[3]; int nrow = 0;
+            rows_[nrow++] = j + g.jgc;
+            if (j >= M - g.jgc) rows_[nrow++] = j - (M - g.jgc);
+            if (j < g.jgc) rows_[nrow++] = g.jend + j;
             for (int x = lane; x < g.icells; x += 64)
             {
                 // ghosted column x maps to interior column x - igc, wrapped
@@ -421,7 +350,7 @@ namespace
                 const TF val = (pr ? z.y : z.x) / TF(n);
                 for (int q = 0; q < nrow; ++q)
                 {
-                    const int ij = x + rows[q]*jj;
+                    const int ij = x + rows_[q]*jj;
                     p[ij + kc*kk] = val;
                     if (k == 0) p[ij + (kc-1)*kk] = val;
                 }
@@ -478,12 +407,18 @@ namespace
             red[1 + blockIdx.x + gridDim.x*(blockIdx.y + gridDim.y*blockIdx.z)] = sm[0];
     }
 
-    int rows_per_block(int n, size_t elem)
+    // Rows per block so that the twiddle table and the row buffers fit in 64 KiB.
+    int rows_per_block(const Plan& p, size_t elem)
     {
-        // two LDS buffers per row, at most 64 KiB per block and at most 4 rows
-        const size_t per_row = 2*static_cast<size_t>(n)*2*elem;
-        int rb = static_cast<int>(65536 / per_row);
-        return std::max(1, std::min(4, rb));
+        const size_t budget = 65536 / elem;          // complex elements
+        const size_t per_row = static_cast<size_t>(row_elems(p));
+        const size_t avail = budget > static_cast<size_t>(p.n) ? budget - p.n : 0;
+        return std::max(1, std::min(4, static_cast<int>(avail / per_row)));
+    }
+
+    size_t lds_bytes(const Plan& p, int RB, size_t elem)
+    {
+        return (static_cast<size_t>(p.n) + static_cast<size_t>(RB)*row_elems(p))*elem;
     }
 
     template<typename TF>
@@ -571,16 +506,16 @@ namespace
         }
         // 2. forward x transform
         {
-            const int RB = rows_per_block(N, sizeof(TF));
+            const int RB = rows_per_block(px, sizeof(T));
             const dim3 grd((M + 2*RB - 1)/(2*RB), K);
-            const size_t lds = 2*static_cast<size_t>(N)*sizeof(T)*RB;
+            const size_t lds = lds_bytes(px, RB, sizeof(T));
             { mhh::Timer _t(c, "fft_x_fwd"); fft_x_fwd_g<TF><<<grd, dim3(64, RB), lds, c->stream>>>(P, C, px, reinterpret_cast<const double2*>(t.twx), RB, g); }
             MHH_CHECK(check_launch("fft_x_fwd"));
         }
         // 3. forward y transform
         const int nrows = nx*K;
-        const int RBy = rows_per_block(M, sizeof(TF));
-        const size_t ldsy = 2*static_cast<size_t>(M)*sizeof(T)*RBy;
+        const int RBy = rows_per_block(py, sizeof(T));
+        const size_t ldsy = lds_bytes(py, RBy, sizeof(T));
         { mhh::Timer _t(c, "fft_y_fwd"); fft_y_g<TF><<<(nrows + RBy - 1)/RBy, dim3(64, RBy), ldsy, c->stream>>>(C, py,
                 reinterpret_cast<const double2*>(t.twy), -1, nrows, RBy); }
         MHH_CHECK(check_launch("fft_y_fwd"));
@@ -597,9 +532,9 @@ namespace
         MHH_CHECK(check_launch("fft_y_bwd"));
         // 6. inverse x transform (divides by itot) into the ghosted p with halos
         {
-            const int RB = rows_per_block(N, sizeof(TF));
+            const int RB = rows_per_block(px, sizeof(T));
             const dim3 grd((M + 2*RB - 1)/(2*RB), K);
-            const size_t lds = 2*static_cast<size_t>(N)*sizeof(T)*RB;
+            const size_t lds = lds_bytes(px, RB, sizeof(T));
             { mhh::Timer _t(c, "fft_x_bwd"); fft_x_bwd_g<TF><<<grd, dim3(64, RB), lds, c->stream>>>(C, P, px, reinterpret_cast<const double2*>(t.twx), RB, g); }
             MHH_CHECK(check_launch("fft_x_bwd"));
         }
