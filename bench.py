@@ -40,11 +40,16 @@ HBM_PEAK_GBS = 8000.0
 #   fft x/y (fwd, bwd): spectrum in and out = 2.  tdma: spectrum in and out = 2.
 #   pres_out: p, ut, vt, wt; ut, vt, wt = 7.  rk3 (one field): a, at; a, at = 4.
 #   buoyancy: th, wt; wt = 3.
+#   pres_out_rk3 (gradient + RK3 of u, v, w): p, u, v, w, ut, vt, wt in;
+#       u, v, w, ut, vt, wt out = 13.
+#   tend_2i5_smag2 (fused buoyancy + advection + diffusion of u, v, w, th):
+#       u, v, w, th, evisc, ut, vt, wt, tht in; ut, vt, wt, tht out = 13.
 PASSES = {
+    "tend_2i5_smag2": 13,
     "advec_2i5_u": 5, "advec_2i5_v": 5, "advec_2i5_w": 5, "advec_2i5_s": 6,
     "evisc": 5, "diff_u": 6, "diff_v": 6, "diff_w": 6, "diff_c": 4,
     "pres_in": 7, "fft_x_fwd": 2, "fft_y_fwd": 2, "tdma": 2, "fft_y_bwd": 2, "fft_x_bwd": 2,
-    "pres_out": 7, "rk3": 4, "buoyancy": 3,
+    "pres_out": 7, "pres_out_rk3": 13, "rk3": 4, "buoyancy": 3,
 }
 KERNELS = list(PASSES) + ["cyclic", "ghost_cells", "mirror_walls", "rk3_zero_ghosts", "cfl", "dnmul"]
 # The canonical fused minimum per grid-point-substep (40 passes).

@@ -148,13 +148,30 @@ int  mhh_thermo_dry_exec(mhh_ctx* ctx);
 /* Pres_2::exec (src/pres_2.cxx:64-94): input, FFT, tridiagonal, inverse FFT,
  * output.  dt is the RK sub-time step.  Uses p, tmp1 and tmp2 as work. */
 int  mhh_pres_2_exec(mhh_ctx* ctx, double dt);
+/* Pres_2::exec followed by Timeloop::exec for u, v, w (src/pres_2.cxx:64-94,
+ * src/timeloop.cxx:251-286): the pressure gradient and the RK3 update of the
+ * momentum fields in one pass; dt is the sub time step of the solve, substep
+ * and rkdt those of the RK3 update.  The scalars are integrated with
+ * mhh_timeloop_rk3_scalars. */
+int  mhh_pres_2_exec_rk3(mhh_ctx* ctx, double dt, int substep, double rkdt);
 /* Pres_2::check_divergence (src/pres_2.cxx:389-422). */
 int  mhh_pres_2_divergence(mhh_ctx* ctx, double* divmax);
+
+/* ---- fused tendency ----
+ * Thermo_dry::exec + Advec_2i5::exec + Diff_smag2::exec of one substep
+ * (src/model.cxx:388-414; src/thermo_dry.cxx:163-180, src/advec_2i5.cxx:150-728,
+ * include/diff_kernels.h:144-484) in one pass over u, v, w, scalar 0 and
+ * evisc, for the default boundary (no surface model).  The terms are added to
+ * ut, vt, wt, st[0] in the reference order; buoyancy != 0 includes the
+ * buoyancy of scalar 0 on wt.  Scalars beyond the first are not touched. */
+int  mhh_tendency_2i5_smag2(mhh_ctx* ctx, const mhh_smag_params* par, int buoyancy);
 
 /* ---- time integration ---- */
 /* Timeloop::exec, rkorder 3 (src/timeloop.cxx:358-412, 251-286) on u, v, w
  * and every bound scalar. */
 int  mhh_timeloop_rk3(mhh_ctx* ctx, int substep, double dt);
+/* The same update for the bound scalars only. */
+int  mhh_timeloop_rk3_scalars(mhh_ctx* ctx, int substep, double dt);
 
 /* ---- device buffers ----
  * RAII device vectors of the reference (cuda_vector, include/cuda_buffer.h)

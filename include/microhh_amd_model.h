@@ -43,6 +43,7 @@ typedef struct
     int adaptivestep;
     double dt, dtmax, endtime, savetime;
     int precision;      /* MHH_FP64 or MHH_FP32 */
+    int fused;          /* 1: fused kernels where the configuration allows (same results) */
 } mhh_case;
 
 /* z_interior, u_prof, v_prof, th_prof: ktot host values each (the "init"

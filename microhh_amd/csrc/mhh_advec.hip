@@ -182,9 +182,13 @@ namespace
     {
         __shared__ double sm[BX*BY];
         const int tid = threadIdx.x + threadIdx.y*BX;
-        int i, j, k;
+        const int i = blockIdx.x*BX + threadIdx.x + g.istart;
+        const int j = blockIdx.y*BY + threadIdx.y + g.jstart;
+        const int kb = g.kstart + blockIdx.z*RED_KCHUNK;
+        const int ke = min(kb + RED_KCHUNK, g.kend);
         double c = 0.;
-        if (point(g, g.kstart, i, j, k))
+        if (i < g.iend && j < g.jend)
+        for (int k = kb; k < ke; ++k)
         {
             const int ii = 1, jj = g.icells, kk = g.ijcells;
             const int ijk = i + j*jj + k*kk;
@@ -205,7 +209,7 @@ namespace
             else
                 val = fabs(i2(u[ijk], u[ijk+ii]))*dxi + fabs(i2(v[ijk], v[ijk+jj]))*dyi
                     + fabs(i2(w[ijk], w[ijk+kk]))*dzi[k];
-            c = static_cast<double>(val);
+            c = fmax(c, static_cast<double>(val));
         }
         sm[tid] = c;
         __syncthreads();
@@ -265,7 +269,7 @@ namespace
     int cfl(mhh_ctx* c, const Prof<TF>& p, double dt, double* out)
     {
         const Gd& g = c->gd;
-        const dim3 grd = grid3(g, g.kstart, g.kend);
+        const dim3 grd = grid_red(g);
         const int nb = static_cast<int>(grd.x*grd.y*grd.z);
         MHH_CHECK(ensure_reduction(c, nb));
         { mhh::Timer _t(c, "cfl"); cfl_g<TF, order><<<grd, dim3(BX, BY), 0, c->stream>>>(

@@ -111,16 +111,62 @@ namespace
         at[ijk] = zero ? TF(0) : t*cAn;
     }
 
-    // Zero the ghost cells of a tendency (the whole-field reset of the last substep).
+    // Zero the ghost cells of a tendency (the whole-field reset of the last
+    // substep, src/timeloop.cxx:272-276, restricted to the cells the interior
+    // update does not touch).  Thread n maps to: the kgc full planes below and
+    // above the interior, then per interior level the jgc rows at each end and
+    // the igc columns at each end of the interior rows.
     template<typename TF>
     __global__ void zero_ghosts_g(TF* __restrict__ at, Gd g)
     {
-        const int n = blockIdx.x*blockDim.x + threadIdx.x;
-        if (n >= g.ncells) return;
-        const int k = n / g.ijcells, r = n % g.ijcells, j = r / g.icells, i = r % g.icells;
-        if (k < g.kstart || k >= g.kend || j < g.jstart || j >= g.jend || i < g.istart || i >= g.iend)
-            at[n] = TF(0);
+        const long long n = static_cast<long long>(blockIdx.x)*blockDim.x + threadIdx.x;
+        const long long nfull = 2LL*g.kgc*g.ijcells;
+        const int nring = g.ijcells - g.itot*g.jtot;
+        const long long total = nfull + static_cast<long long>(nring)*g.ktot;
+        if (n >= total) return;
+        if (n < nfull)
+        {
+            const int plane = static_cast<int>(n / g.ijcells), r = static_cast<int>(n % g.ijcells);
+            const int k = plane < g.kgc ? plane : g.kend + plane - g.kgc;
+            at[r + static_cast<size_t>(k)*g.ijcells] = TF(0);
+            return;
+        }
+        const long long m = n - nfull;
+        const int k = g.kstart + static_cast<int>(m / nring);
+        int r = static_cast<int>(m % nring);
+        int i, j;
+        const int rowcells = g.jgc*g.icells;
+        if (r < rowcells) { j = r / g.icells; i = r % g.icells; }
+        else if (r < 2*rowcells) { r -= rowcells; j = g.jend + r / g.icells; i = r % g.icells; }
+        else
+        {
+            r -= 2*rowcells;
+            j = g.jstart + r / (2*g.igc);
+            const int cidx = r % (2*g.igc);
+            i = cidx < g.igc ? cidx : g.iend + cidx - g.igc;
+        }
+        at[i + j*g.icells + static_cast<size_t>(k)*g.ijcells] = TF(0);
     }
+
+}
+
+namespace mhh
+{
+    template<typename TF>
+    int zero_tendency_ghosts(mhh_ctx* c, TF* at)
+    {
+        const Gd& g = c->gd;
+        const long long total = 2LL*g.kgc*g.ijcells + static_cast<long long>(g.ijcells - g.itot*g.jtot)*g.ktot;
+        const unsigned int nb = static_cast<unsigned int>((total + 255)/256);
+        { mhh::Timer _t(c, "rk3_zero_ghosts"); zero_ghosts_g<TF><<<nb, 256, 0, c->stream>>>(at, g); }
+        return check_launch("zero_ghosts");
+    }
+    template int zero_tendency_ghosts<double>(mhh_ctx*, double*);
+    template int zero_tendency_ghosts<float>(mhh_ctx*, float*);
+}
+
+namespace
+{
 
     template<typename TF>
     int cyclic(mhh_ctx* c, TF* a, int edge)
@@ -180,7 +226,7 @@ namespace
     }
 
     template<typename TF>
-    int rk3(mhh_ctx* c, int substep, double dt)
+    int rk3(mhh_ctx* c, int substep, double dt, bool momentum = true, bool scalars = true)
     {
         // Williamson low-storage coefficients (src/timeloop.cxx:255-256).
         const TF cA[] = {TF(0.), TF(-5./9.), TF(-153./128.)};
@@ -191,16 +237,20 @@ namespace
         const int zero = substepn == 0;
         void* fld[3 + MHH_MAX_SCALARS]; void* ten[3 + MHH_MAX_SCALARS];
         int nf = 0;
-        for (int n = 0; n < c->f.nscalars; ++n) { fld[nf] = c->f.s[n]; ten[nf++] = c->f.st[n]; }
-        fld[nf] = c->f.u; ten[nf++] = c->f.ut;
-        fld[nf] = c->f.v; ten[nf++] = c->f.vt;
-        fld[nf] = c->f.w; ten[nf++] = c->f.wt;
+        if (scalars)
+            for (int n = 0; n < c->f.nscalars; ++n) { fld[nf] = c->f.s[n]; ten[nf++] = c->f.st[n]; }
+        if (momentum)
+        {
+            fld[nf] = c->f.u; ten[nf++] = c->f.ut;
+            fld[nf] = c->f.v; ten[nf++] = c->f.vt;
+            fld[nf] = c->f.w; ten[nf++] = c->f.wt;
+        }
         for (int q = 0; q < nf; ++q)
         {
             { mhh::Timer _t(c, "rk3"); rk3_g<TF><<<grid3(g, g.kstart, g.kend), dim3(BX, BY), 0, c->stream>>>(
                     static_cast<TF*>(fld[q]), static_cast<TF*>(ten[q]), cBdt, cA[substepn], zero, g); }
             if (zero)
-                { mhh::Timer _t(c, "rk3_zero_ghosts"); zero_ghosts_g<TF><<<(g.ncells + 255)/256, 256, 0, c->stream>>>(static_cast<TF*>(ten[q]), g); }
+                MHH_CHECK(zero_tendency_ghosts<TF>(c, static_cast<TF*>(ten[q])));
         }
         return check_launch("rk3");
     }
@@ -247,6 +297,14 @@ int mhh_timeloop_rk3(mhh_ctx* c, int substep, double dt)
     MHH_REQUIRE_BOUND(c);
     if (substep < 0 || substep > 2) return mhh::fail(MHH_EINVAL, "rk3 substep must be 0, 1 or 2");
     return c->precision == MHH_FP64 ? rk3<double>(c, substep, dt) : rk3<float>(c, substep, dt);
+}
+
+int mhh_timeloop_rk3_scalars(mhh_ctx* c, int substep, double dt)
+{
+    MHH_REQUIRE_BOUND(c);
+    if (substep < 0 || substep > 2) return mhh::fail(MHH_EINVAL, "rk3 substep must be 0, 1 or 2");
+    return c->precision == MHH_FP64 ? rk3<double>(c, substep, dt, false, true)
+                                    : rk3<float>(c, substep, dt, false, true);
 }
 
 } // extern "C"

@@ -189,7 +189,7 @@ namespace
         const TF* __restrict__ dzhi;
         const TF* __restrict__ rho;
         const TF* __restrict__ rhoh;
-        TF dxi, dyi, visc, tPr;
+        TF dxi, dyi, dxidxi, dyidyi, visc, tPr;
     };
 
     // diff_u (include/diff_kernels.h:144-240).
@@ -315,7 +315,7 @@ namespace
         const TF* __restrict__ s = a.a;
         const TF* __restrict__ e = a.evisc;
         const TF visc = a.visc, tPr_i = TF(1)/a.tPr;
-        const TF dxidxi = a.dxi*a.dxi, dyidyi = a.dyi*a.dyi;
+        const TF dxidxi = a.dxidxi, dyidyi = a.dyidyi;
         const TF evisce = TF(0.5)*(e[ijk   ]+e[ijk+ii]) * tPr_i + visc;
         const TF eviscw = TF(0.5)*(e[ijk-ii]+e[ijk   ]) * tPr_i + visc;
         const TF eviscn = TF(0.5)*(e[ijk   ]+e[ijk+jj]) * tPr_i + visc;
@@ -355,13 +355,17 @@ namespace
     {
         __shared__ double sm[BX*BY];
         const int tid = threadIdx.x + threadIdx.y*BX;
-        int i, j, k;
+        const int i = blockIdx.x*BX + threadIdx.x + g.istart;
+        const int j = blockIdx.y*BY + threadIdx.y + g.jstart;
+        const int kb = g.kstart + blockIdx.z*RED_KCHUNK;
+        const int ke = min(kb + RED_KCHUNK, g.kend);
         double m = 0.;
-        if (point(g, g.kstart, i, j, k))
-        {
-            const int ijk = i + j*g.icells + k*g.ijcells;
-            m = static_cast<double>(fabs(evisc[ijk]*tPrfac_i*(dxidxi + dyidyi + dzi[k]*dzi[k])));
-        }
+        if (i < g.iend && j < g.jend)
+            for (int k = kb; k < ke; ++k)
+            {
+                const int ijk = i + j*g.icells + k*g.ijcells;
+                m = fmax(m, static_cast<double>(fabs(evisc[ijk]*tPrfac_i*(dxidxi + dyidyi + dzi[k]*dzi[k]))));
+            }
         sm[tid] = m;
         __syncthreads();
         for (int s = BX*BY/2; s > 0; s >>= 1)
@@ -493,6 +497,7 @@ namespace
         a.evisc = dptr<TF>(c->f.evisc);
         a.dzi = p.dzi; a.dzhi = p.dzhi; a.rho = p.rhoref; a.rhoh = p.rhorefh;
         a.dxi = TF(1.)/TF(g.dx); a.dyi = TF(1.)/TF(g.dy);
+        a.dxidxi = TF(1./(g.dx*g.dx)); a.dyidyi = TF(1./(g.dy*g.dy));
         a.visc = TF(par.visc); a.tPr = TF(par.tPr);
         const dim3 blk(BX, BY);
 
@@ -522,7 +527,7 @@ namespace
     int dnmul(mhh_ctx* c, const Prof<TF>& p, double tPr, double* out)
     {
         const Gd& g = c->gd;
-        const dim3 grd = grid3(g, g.kstart, g.kend);
+        const dim3 grd = grid_red(g);
         const int nb = static_cast<int>(grd.x*grd.y*grd.z);
         MHH_CHECK(ensure_reduction(c, nb));
         const TF dxi = TF(1.)/TF(g.dx), dyi = TF(1.)/TF(g.dy);

@@ -93,6 +93,14 @@ namespace mhh
         return dim3((g.itot + BX - 1)/BX, (g.jtot + BY - 1)/BY, klast > kfirst ? klast - kfirst : 0);
     }
 
+    // Max-reductions: each thread folds a chunk of RED_KCHUNK levels of its
+    // column, each block writes one partial, one block folds the partials.
+    constexpr int RED_KCHUNK = 32;
+    inline dim3 grid_red(const Gd& g)
+    {
+        return dim3((g.itot + BX - 1)/BX, (g.jtot + BY - 1)/BY, (g.ktot + RED_KCHUNK - 1)/RED_KCHUNK);
+    }
+
     // Maximum over a device array of per-block partial maxima, returned on the host.
     int reduce_max_partials(mhh_ctx* ctx, int npartials, double* out);
     int ensure_reduction(mhh_ctx* ctx, int nblocks);
@@ -108,6 +116,9 @@ namespace mhh
             mhh_ctx* c;
             hipEvent_t stop = nullptr;
     };
+
+    // Zero the ghost cells of a tendency field (mhh_boundary.hip).
+    template<typename TF> int zero_tendency_ghosts(mhh_ctx* c, TF* at);
 
     // Pressure solver setup (mhh_pres.hip).
     int pres_init(mhh_ctx* ctx);

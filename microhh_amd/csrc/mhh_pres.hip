@@ -375,6 +375,29 @@ namespace
         wt[ijk] -= (pc - p[ijk-g.ijcells]) * dzhi[k];
     }
 
+    // Pres_2::output followed by the RK3 update of u, v, w (src/pres_2.cxx:364-387,
+    // src/timeloop.cxx:251-286): the same operations in one pass.
+    template<typename TF>
+    __global__ void __launch_bounds__(BX*BY) pres_out_rk3_g(
+            TF* __restrict__ u, TF* __restrict__ v, TF* __restrict__ w,
+            TF* __restrict__ ut, TF* __restrict__ vt, TF* __restrict__ wt,
+            const TF* __restrict__ p, const TF* __restrict__ dzhi, TF dxi, TF dyi,
+            TF cBdt, TF cAn, int zero, Gd g)
+    {
+        const int i = blockIdx.x*BX + threadIdx.x + g.istart;
+        const int j = blockIdx.y*BY + threadIdx.y + g.jstart;
+        const int k = blockIdx.z + g.kstart;
+        if (i >= g.iend || j >= g.jend) return;
+        const int ijk = i + j*g.icells + k*g.ijcells;
+        const TF pc = p[ijk];
+        TF tu = ut[ijk]; tu -= (pc - p[ijk-1]) * dxi;
+        TF tv = vt[ijk]; tv -= (pc - p[ijk-g.icells]) * dyi;
+        TF tw = wt[ijk]; tw -= (pc - p[ijk-g.ijcells]) * dzhi[k];
+        u[ijk] += cBdt*tu; ut[ijk] = zero ? TF(0) : tu*cAn;
+        v[ijk] += cBdt*tv; vt[ijk] = zero ? TF(0) : tv*cAn;
+        w[ijk] += cBdt*tw; wt[ijk] = zero ? TF(0) : tw*cAn;
+    }
+
     // Pres_2::calc_divergence (src/pres_2.cxx:389-422), per-block maxima.
     template<typename TF>
     __global__ void __launch_bounds__(BX*BY) divergence_g(const TF* __restrict__ u, const TF* __restrict__ v,
@@ -386,16 +409,18 @@ namespace
         const int tid = threadIdx.x + threadIdx.y*BX;
         const int i = blockIdx.x*BX + threadIdx.x + g.istart;
         const int j = blockIdx.y*BY + threadIdx.y + g.jstart;
-        const int k = blockIdx.z + g.kstart;
+        const int kb = g.kstart + blockIdx.z*RED_KCHUNK;
+        const int ke = min(kb + RED_KCHUNK, g.kend);
         double m = 0.;
         if (i < g.iend && j < g.jend)
-        {
-            const int jj = g.icells, kk = g.ijcells;
-            const int ijk = i + j*jj + k*kk;
-            const TF div = rho[k]*((u[ijk+1]-u[ijk])*dxi + (v[ijk+jj]-v[ijk])*dyi)
-                         + (rhoh[k+1]*w[ijk+kk]-rhoh[k]*w[ijk])*dzi[k];
-            m = static_cast<double>(fabs(div));
-        }
+            for (int k = kb; k < ke; ++k)
+            {
+                const int jj = g.icells, kk = g.ijcells;
+                const int ijk = i + j*jj + k*kk;
+                const TF div = rho[k]*((u[ijk+1]-u[ijk])*dxi + (v[ijk+jj]-v[ijk])*dyi)
+                             + (rhoh[k+1]*w[ijk+kk]-rhoh[k]*w[ijk])*dzi[k];
+                m = fmax(m, static_cast<double>(fabs(div)));
+            }
         sm[tid] = m;
         __syncthreads();
         for (int s = BX*BY/2; s > 0; s >>= 1)
@@ -478,8 +503,10 @@ namespace
         t = Pres_tables<TF>();
     }
 
+    // rk3 >= 0: apply the gradient together with the RK3 update of substep
+    // rk3 (time step rkdt) instead of Pres_2::output alone.
     template<typename TF>
-    int exec(mhh_ctx* c, const Prof<TF>& p, const Pres_tables<TF>& t, double dt)
+    int exec(mhh_ctx* c, const Prof<TF>& p, const Pres_tables<TF>& t, double dt, int rk3 = -1, double rkdt = 0.)
     {
         using T = cplx<TF>;
         const Gd& g = c->gd;
@@ -539,17 +566,36 @@ namespace
             MHH_CHECK(check_launch("fft_x_bwd"));
         }
         // 7. pressure gradient on the tendencies
-        { mhh::Timer _t(c, "pres_out"); pres_out_g<TF><<<grid3(g, g.kstart, g.kend), dim3(BX, BY), 0, c->stream>>>(
+        if (rk3 < 0)
+        {
+            { mhh::Timer _t(c, "pres_out"); pres_out_g<TF><<<grid3(g, g.kstart, g.kend), dim3(BX, BY), 0, c->stream>>>(
+                    static_cast<TF*>(c->f.ut), static_cast<TF*>(c->f.vt), static_cast<TF*>(c->f.wt),
+                    P, p.dzhi, dxi, dyi, g); }
+            return check_launch("pres_out");
+        }
+        // Williamson low-storage coefficients (src/timeloop.cxx:255-256).
+        const TF cA[] = {TF(0.), TF(-5./9.), TF(-153./128.)};
+        const TF cB[] = {TF(1./3.), TF(15./16.), TF(8./15.)};
+        const int substepn = (rk3 + 1) % 3;
+        { mhh::Timer _t(c, "pres_out_rk3"); pres_out_rk3_g<TF><<<grid3(g, g.kstart, g.kend), dim3(BX, BY), 0, c->stream>>>(
+                static_cast<TF*>(c->f.u), static_cast<TF*>(c->f.v), static_cast<TF*>(c->f.w),
                 static_cast<TF*>(c->f.ut), static_cast<TF*>(c->f.vt), static_cast<TF*>(c->f.wt),
-                P, p.dzhi, dxi, dyi, g); }
-        return check_launch("pres_out");
+                P, p.dzhi, dxi, dyi, cB[rk3]*TF(rkdt), cA[substepn], substepn == 0, g); }
+        MHH_CHECK(check_launch("pres_out_rk3"));
+        if (substepn == 0)
+        {
+            MHH_CHECK(zero_tendency_ghosts<TF>(c, static_cast<TF*>(c->f.ut)));
+            MHH_CHECK(zero_tendency_ghosts<TF>(c, static_cast<TF*>(c->f.vt)));
+            MHH_CHECK(zero_tendency_ghosts<TF>(c, static_cast<TF*>(c->f.wt)));
+        }
+        return 0;
     }
 
     template<typename TF>
     int divergence(mhh_ctx* c, const Prof<TF>& p, double* out)
     {
         const Gd& g = c->gd;
-        const dim3 grd = grid3(g, g.kstart, g.kend);
+        const dim3 grd = grid_red(g);
         const int nb = static_cast<int>(grd.x*grd.y*grd.z);
         MHH_CHECK(ensure_reduction(c, nb));
         { mhh::Timer _t(c, "divergence"); divergence_g<TF><<<grd, dim3(BX, BY), 0, c->stream>>>(
@@ -584,6 +630,15 @@ int mhh_pres_2_exec(mhh_ctx* c, double dt)
     MHH_REQUIRE_BOUND(c);
     if (!(dt > 0)) return mhh::fail(MHH_EINVAL, "pres_2: dt must be positive");
     return c->precision == MHH_FP64 ? exec<double>(c, c->pd, c->td, dt) : exec<float>(c, c->pf, c->tf, dt);
+}
+
+int mhh_pres_2_exec_rk3(mhh_ctx* c, double dt, int substep, double rkdt)
+{
+    MHH_REQUIRE_BOUND(c);
+    if (!(dt > 0)) return mhh::fail(MHH_EINVAL, "pres_2: dt must be positive");
+    if (substep < 0 || substep > 2) return mhh::fail(MHH_EINVAL, "rk3 substep must be 0, 1 or 2");
+    return c->precision == MHH_FP64 ? exec<double>(c, c->pd, c->td, dt, substep, rkdt)
+                                    : exec<float>(c, c->pf, c->tf, dt, substep, rkdt);
 }
 
 int mhh_pres_2_divergence(mhh_ctx* c, double* out)

@@ -1,0 +1,341 @@
+// mhh_tend.hip -- fused tendency of one dry-CBL substep: buoyancy
+// (src/thermo_dry.cxx:163-180), 2i5 advection (src/advec_2i5.cxx:150-728) and
+// Smagorinsky diffusion without a surface model (include/diff_kernels.h:144-484)
+// of u, v, w and one scalar, accumulated onto ut, vt, wt, st in the order the
+// reference driver applies them (src/model.cxx:388-414): buoyancy, horizontal
+// advection, vertical advection, diffusion.
+//
+// Layout of the work.  A block owns a 64 x 4 column tile and marches upward
+// through a chunk of levels.  At level k the LDS holds
+//     u, v at k-1 and k,  w at k and k+1,  s at k      (halo 3 in x and y)
+//     evisc at k-1, k, k+1                              (halo 1)
+// rotating as k advances, so every plane is read from HBM once per chunk;
+// the +-3 vertical neighbours of the thread's own column come from register
+// queues.  Block ids are swizzled so that the tiles one XCD works on are
+// neighbours: their halos are then served by that XCD's L2.
+#include "mhh_internal.h"
+#include "mhh_stencil.h"
+
+namespace
+{
+    using namespace mhh;
+
+    constexpr int TX = 64, TY = 4;
+    constexpr int H = 3;                  // horizontal halo of the 2i5 stencil
+    constexpr int PW = TX + 2*H, PH = TY + 2*H, PN = PW*PH;
+    constexpr int EW = TX + 2, EH = TY + 2, EN = EW*EH;
+    constexpr int KCHUNK = 64;
+    constexpr double grav = 9.81;        // include/constants.h:29
+
+    template<typename TF>
+    struct Tend_args
+    {
+        TF* __restrict__ ut; TF* __restrict__ vt; TF* __restrict__ wt; TF* __restrict__ st;
+        const TF* __restrict__ u; const TF* __restrict__ v; const TF* __restrict__ w;
+        const TF* __restrict__ s; const TF* __restrict__ evisc;
+        const TF* __restrict__ dzi; const TF* __restrict__ dzhi;
+        const TF* __restrict__ rho; const TF* __restrict__ rhoh; const TF* __restrict__ threfh;
+        TF dxi, dyi, dxidxi, dyidyi, visc, svisc, tPr_i;
+        int buoyancy;
+        int nti, ntj, nkc;
+    };
+
+    template<typename TF>
+    __device__ __forceinline__ void load_plane(TF* __restrict__ dst, const TF* __restrict__ src, int k,
+                                               int i0, int j0, int halo, int w, int n, const Gd& g, int tid)
+    {
+        const TF* __restrict__ sk = src + static_cast<size_t>(k)*g.ijcells;
+        for (int idx = tid; idx < n; idx += TX*TY)
+        {
+            const int pj = idx / w, pi = idx - pj*w;
+            const int gi = i0 - halo + pi, gj = j0 - halo + pj;
+            dst[idx] = (gi < g.icells && gj < g.jcells) ? sk[gi + gj*g.icells] : TF(0);
+        }
+    }
+
+    // Per-thread share of a plane: elements tid, tid+256, tid+512 (halo-3
+    // planes, 700 elements) or tid, tid+256 (halo-1 planes, 396 elements).
+    constexpr int NQ3 = (PN + TX*TY - 1)/(TX*TY);
+    constexpr int NQ1 = (EN + TX*TY - 1)/(TX*TY);
+
+    template<typename TF, int NQ>
+    __device__ __forceinline__ void fetch_plane(TF (&r)[NQ], const TF* __restrict__ src, int k,
+                                                int i0, int j0, int halo, int w, int n, const Gd& g, int tid)
+    {
+        const TF* __restrict__ sk = src + static_cast<size_t>(k)*g.ijcells;
+        #pragma unroll
+        for (int q = 0; q < NQ; ++q)
+        {
+            const int idx = tid + q*TX*TY;
+            const int pj = idx / w, pi = idx - pj*w;
+            const int gi = i0 - halo + pi, gj = j0 - halo + pj;
+            r[q] = (idx < n && k < g.kcells && gi < g.icells && gj < g.jcells) ? sk[gi + gj*g.icells] : TF(0);
+        }
+    }
+
+    template<typename TF, int NQ>
+    __device__ __forceinline__ void store_plane(TF* __restrict__ dst, const TF (&r)[NQ], int n, int tid)
+    {
+        #pragma unroll
+        for (int q = 0; q < NQ; ++q)
+        {
+            const int idx = tid + q*TX*TY;
+            if (idx < n) dst[idx] = r[q];
+        }
+    }
+
+    template<typename TF>
+    __global__ void __launch_bounds__(TX*TY) tend_2i5_smag2_g(Tend_args<TF> a, Gd g)
+    {
+        __shared__ TF lds[7*PN + 3*EN];
+
+        // XCD-aware tile order: block b runs on XCD b % 8; give each XCD a
+        // contiguous range of tiles (i fastest, then j, then the k chunk).
+        const int nb = gridDim.x;
+        const int ntiles = a.nti*a.ntj*a.nkc;
+        const int tile = (blockIdx.x % 8)*(nb/8) + blockIdx.x/8;
+        if (tile >= ntiles) return;
+        const int ti = tile % a.nti;
+        const int tj = (tile / a.nti) % a.ntj;
+        const int kc = tile / (a.nti*a.ntj);
+
+        const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX*ty;
+        const int i0 = g.istart + ti*TX, j0 = g.jstart + tj*TY;
+        const int i = i0 + tx, j = j0 + ty;
+        const bool active = i < g.iend && j < g.jend;
+        const int kb = g.kstart + kc*KCHUNK;
+        const int ke = min(kb + KCHUNK, g.kend);
+        const int jj = g.icells, kk = g.ijcells;
+
+        TF* Up = lds;        TF* Uc = Up + PN;
+        TF* Vp = Uc + PN;    TF* Vc = Vp + PN;
+        TF* Wc = Vc + PN;    TF* Wn = Wc + PN;
+        TF* Sc = Wn + PN;
+        TF* Ep = Sc + PN;    TF* Ec = Ep + EN;   TF* En = Ec + EN;
+
+        load_plane(Up, a.u, kb-1, i0, j0, H, PW, PN, g, tid);
+        load_plane(Vp, a.v, kb-1, i0, j0, H, PW, PN, g, tid);
+        load_plane(Wc, a.w, kb, i0, j0, H, PW, PN, g, tid);
+        load_plane(Ep, a.evisc, kb-1, i0, j0, 1, EW, EN, g, tid);
+        load_plane(Ec, a.evisc, kb, i0, j0, 1, EW, EN, g, tid);
+
+        // Column queues: q[m] holds level k-3+m of the thread's own column.
+        TF qu[7], qv[7], qw[7], qs[7];
+        const int ij = i + j*jj;
+        #pragma unroll
+        for (int m = 0; m < 7; ++m)
+        {
+            const int kq = kb - 3 + m;
+            const bool ok = active && kq >= 0 && kq < g.kcells;
+            qu[m] = ok ? a.u[ij + kq*kk] : TF(0);
+            qv[m] = ok ? a.v[ij + kq*kk] : TF(0);
+            qw[m] = ok ? a.w[ij + kq*kk] : TF(0);
+            qs[m] = ok ? a.s[ij + kq*kk] : TF(0);
+        }
+
+        const int c = (ty + H)*PW + (tx + H);     // centre in the halo-3 planes
+        const int ce = (ty + 1)*EW + (tx + 1);    // centre in the halo-1 planes
+        const TF dxi = a.dxi, dyi = a.dyi, visc = a.visc;
+
+        // Planes of the first level of the chunk go straight to LDS; the planes
+        // of every next level are fetched into registers while the current one
+        // is computed (software pipelining over k).
+        load_plane(Uc, a.u, kb, i0, j0, H, PW, PN, g, tid);
+        load_plane(Vc, a.v, kb, i0, j0, H, PW, PN, g, tid);
+        load_plane(Wn, a.w, kb+1, i0, j0, H, PW, PN, g, tid);
+        load_plane(Sc, a.s, kb, i0, j0, H, PW, PN, g, tid);
+        load_plane(En, a.evisc, kb+1, i0, j0, 1, EW, EN, g, tid);
+        TF ru[NQ3], rv[NQ3], rw[NQ3], rs[NQ3], re[NQ1];
+
+        for (int k = kb; k < ke; ++k)
+        {
+            const bool more = k + 1 < ke;
+            if (more)
+            {
+                fetch_plane(ru, a.u, k+1, i0, j0, H, PW, PN, g, tid);
+                fetch_plane(rv, a.v, k+1, i0, j0, H, PW, PN, g, tid);
+                fetch_plane(rw, a.w, k+2, i0, j0, H, PW, PN, g, tid);
+                fetch_plane(rs, a.s, k+1, i0, j0, H, PW, PN, g, tid);
+                fetch_plane(re, a.evisc, k+2, i0, j0, 1, EW, EN, g, tid);
+            }
+            __syncthreads();
+
+            if (active)
+            {
+                const int ijk = ij + k*kk;
+                const int lvl = level_class_c(k, g.kstart, g.kend);
+                const TF rhok = a.rho[k], rhohk = a.rhoh[k], rhohkp = a.rhoh[k+1];
+                const TF dzik = a.dzi[k], dzhik = a.dzhi[k], dzhikp = a.dzhi[k+1];
+
+                // ---------------------------------------------------------- u
+                {
+                    TF t = a.ut[ijk];
+                    t += hsum(hflux_2i5(i2(Uc[c], Uc[c+1]), i2(Uc[c-1], Uc[c]), Uc, c, 1, dxi),
+                              hflux_2i5(i2(Vc[c-1+PW], Vc[c+PW]), i2(Vc[c-1], Vc[c]), Uc, c, PW, dyi));
+                    t += vflux_2i5_c(lvl, i2(Wn[c-1], Wn[c]), i2(Wc[c-1], Wc[c]),
+                                     rhohkp, rhohk, rhok, dzik, qu + 3, 0, 1);
+                    const TF evisce = Ec[ce] + visc;
+                    const TF eviscw = Ec[ce-1] + visc;
+                    const TF eviscn = TF(0.25)*(Ec[ce-1] + Ec[ce] + Ec[ce-1+EW] + Ec[ce+EW]) + visc;
+                    const TF eviscs = TF(0.25)*(Ec[ce-1-EW] + Ec[ce-EW] + Ec[ce-1] + Ec[ce]) + visc;
+                    const TF evisct = TF(0.25)*(Ec[ce-1] + Ec[ce] + En[ce-1] + En[ce]) + visc;
+                    const TF eviscb = TF(0.25)*(Ep[ce-1] + Ep[ce] + Ec[ce-1] + Ec[ce]) + visc;
+                    const TF hor =
+                             + ( evisce*(Uc[c+1]-Uc[c  ])*dxi
+                               - eviscw*(Uc[c  ]-Uc[c-1])*dxi ) * TF(2.)*dxi
+                             + ( eviscn*((Uc[c+PW]-Uc[c   ])*dyi + (Vc[c+PW]-Vc[c-1+PW])*dxi)
+                               - eviscs*((Uc[c   ]-Uc[c-PW])*dyi + (Vc[c   ]-Vc[c-1   ])*dxi) ) * dyi;
+                    const TF ver =
+                             ( rhohkp * evisct*((qu[4]-qu[3])*dzhikp + (Wn[c]-Wn[c-1])*dxi)
+                             - rhohk  * eviscb*((qu[3]-qu[2])*dzhik  + (Wc[c]-Wc[c-1])*dxi) ) / rhok * dzik;
+                    t += hor + ver;
+                    a.ut[ijk] = t;
+                }
+                // ---------------------------------------------------------- v
+                {
+                    TF t = a.vt[ijk];
+                    t += hsum(hflux_2i5(i2(Uc[c+1-PW], Uc[c+1]), i2(Uc[c-PW], Uc[c]), Vc, c, 1, dxi),
+                              hflux_2i5(i2(Vc[c], Vc[c+PW]), i2(Vc[c-PW], Vc[c]), Vc, c, PW, dyi));
+                    t += vflux_2i5_c(lvl, i2(Wn[c-PW], Wn[c]), i2(Wc[c-PW], Wc[c]),
+                                     rhohkp, rhohk, rhok, dzik, qv + 3, 0, 1);
+                    const TF evisce = TF(0.25)*(Ec[ce-EW] + Ec[ce] + Ec[ce+1-EW] + Ec[ce+1]) + visc;
+                    const TF eviscw = TF(0.25)*(Ec[ce-1-EW] + Ec[ce-1] + Ec[ce-EW] + Ec[ce]) + visc;
+                    const TF eviscn = Ec[ce] + visc;
+                    const TF eviscs = Ec[ce-EW] + visc;
+                    const TF evisct = TF(0.25)*(Ec[ce-EW] + Ec[ce] + En[ce-EW] + En[ce]) + visc;
+                    const TF eviscb = TF(0.25)*(Ep[ce-EW] + Ep[ce] + Ec[ce-EW] + Ec[ce]) + visc;
+                    const TF hor =
+                             + ( evisce*((Vc[c+1]-Vc[c  ])*dxi + (Uc[c+1]-Uc[c+1-PW])*dyi)
+                               - eviscw*((Vc[c  ]-Vc[c-1])*dxi + (Uc[c  ]-Uc[c  -PW])*dyi) ) * dxi
+                             + ( eviscn*(Vc[c+PW]-Vc[c   ])*dyi
+                               - eviscs*(Vc[c   ]-Vc[c-PW])*dyi ) * TF(2.)*dyi;
+                    const TF ver =
+                             ( rhohkp * evisct*((qv[4]-qv[3])*dzhikp + (Wn[c]-Wn[c-PW])*dyi)
+                             - rhohk  * eviscb*((qv[3]-qv[2])*dzhik  + (Wc[c]-Wc[c-PW])*dyi) ) / rhok * dzik;
+                    t += hor + ver;
+                    a.vt[ijk] = t;
+                }
+                // ---------------------------------------------------------- w
+                if (k > g.kstart)
+                {
+                    TF t = a.wt[ijk];
+                    if (a.buoyancy)
+                        t += TF(grav)/a.threfh[k] * (i2(qs[2], qs[3]) - a.threfh[k]);
+                    t += hsum(hflux_2i5(i2(Up[c+1], Uc[c+1]), i2(Up[c], Uc[c]), Wc, c, 1, dxi),
+                              hflux_2i5(i2(Vp[c+PW], Vc[c+PW]), i2(Vp[c], Vc[c]), Wc, c, PW, dyi));
+                    t += vflux_2i5_w(level_class_w(k, g.kstart, g.kend), a.rho[k], a.rho[k-1], rhohk, dzhik,
+                                     qw + 3, 0, 1);
+                    const TF evisce = TF(0.25)*(Ep[ce] + Ec[ce] + Ep[ce+1] + Ec[ce+1]) + visc;
+                    const TF eviscw = TF(0.25)*(Ep[ce-1] + Ec[ce-1] + Ep[ce] + Ec[ce]) + visc;
+                    const TF eviscn = TF(0.25)*(Ep[ce] + Ec[ce] + Ep[ce+EW] + Ec[ce+EW]) + visc;
+                    const TF eviscs = TF(0.25)*(Ep[ce-EW] + Ec[ce-EW] + Ep[ce] + Ec[ce]) + visc;
+                    const TF evisct = Ec[ce] + visc;
+                    const TF eviscb = Ep[ce] + visc;
+                    t += + ( evisce*((Wc[c+1]-Wc[c  ])*dxi + (Uc[c+1]-Up[c+1])*dzhik)
+                           - eviscw*((Wc[c  ]-Wc[c-1])*dxi + (Uc[c  ]-Up[c  ])*dzhik) ) * dxi
+                         + ( eviscn*((Wc[c+PW]-Wc[c   ])*dyi + (Vc[c+PW]-Vp[c+PW])*dzhik)
+                           - eviscs*((Wc[c   ]-Wc[c-PW])*dyi + (Vc[c   ]-Vp[c   ])*dzhik) ) * dyi
+                         + ( a.rho[k  ] * evisct*(qw[4]-qw[3])*dzik
+                           - a.rho[k-1] * eviscb*(qw[3]-qw[2])*a.dzi[k-1] ) / rhohk * TF(2.)*dzhik;
+                    a.wt[ijk] = t;
+                }
+                // ---------------------------------------------------------- s
+                {
+                    TF t = a.st[ijk];
+                    t += hsum(hflux_2i5(Uc[c+1], Uc[c], Sc, c, 1, dxi),
+                              hflux_2i5(Vc[c+PW], Vc[c], Sc, c, PW, dyi));
+                    t += vflux_2i5_c(lvl, Wn[c], Wc[c], rhohkp, rhohk, rhok, dzik, qs + 3, 0, 1);
+                    const TF ti = a.tPr_i, sv = a.svisc;
+                    const TF evisce = TF(0.5)*(Ec[ce   ]+Ec[ce+1 ]) * ti + sv;
+                    const TF eviscw = TF(0.5)*(Ec[ce-1 ]+Ec[ce   ]) * ti + sv;
+                    const TF eviscn = TF(0.5)*(Ec[ce   ]+Ec[ce+EW]) * ti + sv;
+                    const TF eviscs = TF(0.5)*(Ec[ce-EW]+Ec[ce   ]) * ti + sv;
+                    const TF evisct = TF(0.5)*(Ec[ce   ]+En[ce   ]) * ti + sv;
+                    const TF eviscb = TF(0.5)*(Ep[ce   ]+Ec[ce   ]) * ti + sv;
+                    const TF hor =
+                             + ( evisce*(Sc[c+1]-Sc[c  ])
+                               - eviscw*(Sc[c  ]-Sc[c-1]) ) * a.dxidxi
+                             + ( eviscn*(Sc[c+PW]-Sc[c   ])
+                               - eviscs*(Sc[c   ]-Sc[c-PW]) ) * a.dyidyi;
+                    const TF ver =
+                             ( rhohkp * evisct*(qs[4]-qs[3])*dzhikp
+                             - rhohk  * eviscb*(qs[3]-qs[2])*dzhik ) / rhok * dzik;
+                    t += hor + ver;
+                    a.st[ijk] = t;
+                }
+            }
+            __syncthreads();
+
+            // rotate the plane ring, store the prefetched planes, advance the queues
+            TF* x;
+            x = Up; Up = Uc; Uc = x;
+            x = Vp; Vp = Vc; Vc = x;
+            x = Wc; Wc = Wn; Wn = x;
+            x = Ep; Ep = Ec; Ec = En; En = x;
+            if (more)
+            {
+                store_plane(Uc, ru, PN, tid);
+                store_plane(Vc, rv, PN, tid);
+                store_plane(Wn, rw, PN, tid);
+                store_plane(Sc, rs, PN, tid);
+                store_plane(En, re, EN, tid);
+            }
+            #pragma unroll
+            for (int m = 0; m < 6; ++m)
+            {
+                qu[m] = qu[m+1]; qv[m] = qv[m+1]; qw[m] = qw[m+1]; qs[m] = qs[m+1];
+            }
+            const int kq = k + 4;
+            const bool ok = active && kq < g.kcells;
+            qu[6] = ok ? a.u[ij + kq*kk] : TF(0);
+            qv[6] = ok ? a.v[ij + kq*kk] : TF(0);
+            qw[6] = ok ? a.w[ij + kq*kk] : TF(0);
+            qs[6] = ok ? a.s[ij + kq*kk] : TF(0);
+        }
+    }
+
+    template<typename TF>
+    int tend(mhh_ctx* c, const Prof<TF>& p, const mhh_smag_params& par, int buoyancy)
+    {
+        const Gd& g = c->gd;
+        if (c->f.nscalars < 1)
+            return fail(MHH_ESTATE, "the fused tendency needs scalar 0");
+        if (g.igc < 3 || g.jgc < 3)
+            return fail(MHH_EINVAL, "the fused tendency needs igc = jgc >= 3");
+        if (par.surface_model)
+            return fail(MHH_EINVAL, "the fused tendency covers the default boundary (no surface model)");
+        if (buoyancy && !p.threfh)
+            return fail(MHH_ESTATE, "buoyancy needs mhh_set_base_state()");
+        Tend_args<TF> a{};
+        a.ut = static_cast<TF*>(c->f.ut); a.vt = static_cast<TF*>(c->f.vt); a.wt = static_cast<TF*>(c->f.wt);
+        a.st = static_cast<TF*>(c->f.st[0]);
+        a.u = static_cast<const TF*>(c->f.u); a.v = static_cast<const TF*>(c->f.v);
+        a.w = static_cast<const TF*>(c->f.w); a.s = static_cast<const TF*>(c->f.s[0]);
+        a.evisc = static_cast<const TF*>(c->f.evisc);
+        a.dzi = p.dzi; a.dzhi = p.dzhi; a.rho = p.rhoref; a.rhoh = p.rhorefh; a.threfh = p.threfh;
+        a.dxi = TF(1.)/TF(g.dx); a.dyi = TF(1.)/TF(g.dy);
+        a.dxidxi = TF(1./(g.dx*g.dx)); a.dyidyi = TF(1./(g.dy*g.dy));
+        a.visc = TF(par.visc); a.svisc = TF(par.svisc[0]); a.tPr_i = TF(1)/TF(par.tPr);
+        a.buoyancy = buoyancy;
+        a.nti = (g.itot + TX - 1)/TX;
+        a.ntj = (g.jtot + TY - 1)/TY;
+        a.nkc = (g.ktot + KCHUNK - 1)/KCHUNK;
+        const int ntiles = a.nti*a.ntj*a.nkc;
+        const int nblocks = (ntiles + 7)/8*8;
+        { mhh::Timer _t(c, "tend_2i5_smag2"); tend_2i5_smag2_g<TF><<<nblocks, dim3(TX, TY), 0, c->stream>>>(a, g); }
+        return check_launch("tend_2i5_smag2");
+    }
+}
+
+extern "C" {
+
+int mhh_tendency_2i5_smag2(mhh_ctx* c, const mhh_smag_params* par, int buoyancy)
+{
+    MHH_REQUIRE_BOUND(c);
+    if (!par) return mhh::fail(MHH_EINVAL, "null smag params");
+    if (!c->f.evisc) return mhh::fail(MHH_ESTATE, "evisc field is not bound");
+    if (buoyancy && !c->have_base_state) return mhh::fail(MHH_ESTATE, "buoyancy needs mhh_set_base_state()");
+    return c->precision == MHH_FP64 ? tend<double>(c, c->pd, *par, buoyancy) : tend<float>(c, c->pf, *par, buoyancy);
+}
+
+} // extern "C"

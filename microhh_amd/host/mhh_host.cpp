@@ -641,6 +641,9 @@ namespace mhh_host
         diff = Diff<TF>::factory(ctx_, *grid, *fields, c);
         pres = std::make_unique<Pres_2<TF>>(ctx_);
         timeloop = std::make_unique<Timeloop<TF>>(ctx_, c);
+        // The fused tendency kernel covers 2i5 + smag2 + dry thermo with the
+        // default boundary: one pass instead of the three operator calls.
+        fused_tendency = c.fused && c.advec == 0 && c.diff == 0 && c.thermo == 0;
     }
 
     template<typename TF>
@@ -672,13 +675,39 @@ namespace mhh_host
         diff->exec_viscosity();
         set_time_step();
         const double subdt = timeloop->get_sub_time_step();
-        if (thermo)
-            thermo->exec(subdt);
-        boundary->set_ghost_cells();
-        advec->exec();
-        diff->exec();
-        pres->exec(subdt);
-        timeloop->exec();
+        if (fused_tendency)
+        {
+            // Thermo_dry::exec, Advec_2i5::exec and Diff_smag2::exec in one
+            // kernel, terms added in the same order.  The ghost-cell update
+            // that the reference runs between buoyancy and advection only
+            // rewrites the ghost levels, which buoyancy does not read.
+            boundary->set_ghost_cells();
+            const auto& par = static_cast<Diff_smag2<TF>&>(*diff).params();
+            check(mhh_tendency_2i5_smag2(ctx_, &par, 1), "mhh_tendency_2i5_smag2");
+        }
+        else
+        {
+            if (thermo)
+                thermo->exec(subdt);
+            boundary->set_ghost_cells();
+            advec->exec();
+            diff->exec();
+        }
+        if (fused_tendency)
+        {
+            // Pres_2::exec with the RK3 update of u, v, w in its last kernel,
+            // then the RK3 update of the scalars (Timeloop::exec).
+            check(mhh_pres_2_exec_rk3(ctx_, subdt, timeloop->get_substep(), timeloop->get_dt()),
+                  "mhh_pres_2_exec_rk3");
+            check(mhh_timeloop_rk3_scalars(ctx_, timeloop->get_substep(), timeloop->get_dt()),
+                  "mhh_timeloop_rk3_scalars");
+            timeloop->advance_substep();
+        }
+        else
+        {
+            pres->exec(subdt);
+            timeloop->exec();
+        }
         timeloop->step_time();
     }
 

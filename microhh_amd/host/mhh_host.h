@@ -182,6 +182,7 @@ namespace mhh_host
             void exec() override;
             unsigned long get_time_limit(unsigned long idt, double dt) override;
             double get_dn(double dt) override;
+            const mhh_smag_params& params() const { return par; }
         private:
             mhh_smag_params par{};
             double dnmax;
@@ -237,6 +238,8 @@ namespace mhh_host
         public:
             Timeloop(mhh_ctx* ctx, const mhh_case& c);
             void exec();
+            // Advance the substep counter after an RK3 update done elsewhere.
+            void advance_substep() { substep = (substep + 1) % 3; }
             double get_sub_time_step() const;
             bool in_substep() const { return substep > 0; }
             void set_time_step_limit();
@@ -298,6 +301,7 @@ namespace mhh_host
         private:
             void set_time_step();
             mhh_case cs;
+            bool fused_tendency = false;
             mhh_ctx* ctx_ = nullptr;
             std::unique_ptr<Grid<TF>> grid;
             std::unique_ptr<Fields<TF>> fields;

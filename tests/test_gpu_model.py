@@ -67,6 +67,19 @@ def test_substep_tendencies(advec, diff):
     g.close(); o.close()
 
 
+def test_fused_matches_unfused():
+    kw = dict(itot=64, jtot=48, ktot=40, xsize=6400.0, ysize=4800.0, zsize=2000.0,
+              adaptive=False, dt=2.0, rndamp_uvw=0.5, rndamp_th=0.5)
+    a = mm.Model(mm.dry_cbl_case(fused=True, **kw))
+    b = mm.Model(mm.dry_cbl_case(fused=False, **kw))
+    for _ in range(6):
+        a.substep(); b.substep()
+    for n in ("u", "v", "w", "th", "ut", "vt", "wt", "tht"):
+        err = _nw(a.field(n), b.field(n))
+        assert err < 1e-13, f"{n}: {err:.3e}"
+    a.close(); b.close()
+
+
 def test_short_run_fixed_dt():
     case, g, o = _pair(itot=64, jtot=64, ktot=32, xsize=6400.0, ysize=6400.0, zsize=1600.0,
                        adaptive=False, dt=2.0, rndamp_uvw=0.5, rndamp_th=0.5)

@@ -280,3 +280,66 @@ def test_rk3(setup, substep):
         L.mo_rk3(d, po.ptr(a), po.ptr(at), substep, dt)
         assert normwise(g.get(gf), a) < 1e-15, f
         assert normwise(g.get(gt), at) < 1e-15, t
+
+
+@pytest.mark.parametrize("substep", [0, 1, 2])
+def test_pres_rk3_matches_pres_then_rk3(setup, substep):
+    """mhh_pres_2_exec_rk3 + mhh_timeloop_rk3_scalars against mhh_pres_2_exec
+    followed by mhh_timeloop_rk3: whole fields, ghost cells included."""
+    p, i, d, g, *_ = setup
+    dt, rkdt = 0.37, 0.61
+
+    def load():
+        _load_common(g, i)
+        for n in ("u", "v", "w"):
+            g.call("mhh_boundary_cyclic", g.ptr(n), 2)
+
+    load()
+    g.call("mhh_pres_2_exec_rk3", dt, substep, rkdt)
+    g.call("mhh_timeloop_rk3_scalars", substep, rkdt)
+    names = ("u", "v", "w", "ut", "vt", "wt", "s0", "st0")
+    fused = {n: g.get(n) for n in names}
+    load()
+    g.call("mhh_pres_2_exec", dt)
+    g.call("mhh_timeloop_rk3", substep, rkdt)
+    for n in names:
+        err = normwise(fused[n], g.get(n))
+        assert err < 1e-14, f"{n}: {err:.3e}"
+
+
+def test_fused_tendency_matches_operator_sequence(setup):
+    """mhh_tendency_2i5_smag2 against thermo_dry + advec_2i5 + diff_smag2 run
+    one after another through the ABI, then against the oracle chain."""
+    from microhh_amd import abi
+    p, i, d, g, thref, threfh = setup
+    par = abi.SmagParams(0.23, 1.0 / 3.0, 1.5e-5, (C.c_double * 8)(*([2e-5] * 8)), 1, 0, 1)
+    _load_common(g, i)
+    g.put("evisc", i["evisc"])
+    g.call("mhh_tendency_2i5_smag2", C.byref(par), 1)
+    fused = {n: g.get(n) for n in ("ut", "vt", "wt", "st0")}
+    _load_common(g, i)
+    g.call("mhh_thermo_dry_exec")
+    g.call("mhh_advec_2i5_exec")
+    g.call("mhh_diff_smag2_exec", C.byref(par))
+    for n in ("ut", "vt", "wt", "st0"):
+        err = normwise(fused[n], g.get(n))
+        assert err < 1e-13, f"{n}: {err:.3e}"
+    # and the oracle chain on the same inputs
+    L = po.lib()
+    u, v, w = (po.ptr(_c(i[k])) for k in ("u", "v", "w"))
+    rho, rhoh = po.ptr(_c(i["rhoref"])), po.ptr(_c(i["rhorefh"]))
+    dzi, dzhi, ev = po.ptr(_c(i["dzi"])), po.ptr(_c(i["dzhi"])), po.ptr(_c(i["evisc"]))
+    zero = po.ptr(np.zeros((d.jcells, d.icells)))
+    ut, vt, wt, st = _c(i["ut"]), _c(i["vt"]), _c(i["wt"]), _c(i["st"])
+    L.mo_buoyancy_tend_2nd(d, po.ptr(wt), po.ptr(_c(i["s"])), po.ptr(_c(threfh)))
+    L.mo_advec_2i5_u(d, po.ptr(ut), u, v, w, dzi, rho, rhoh)
+    L.mo_advec_2i5_v(d, po.ptr(vt), u, v, w, dzi, rho, rhoh)
+    L.mo_advec_2i5_w(d, po.ptr(wt), u, v, w, dzhi, rho, rhoh)
+    L.mo_advec_2i5_s(d, po.ptr(st), po.ptr(_c(i["s"])), u, v, w, dzi, rho, rhoh)
+    L.mo_diff_u(d, 0, po.ptr(ut), u, v, w, dzi, dzhi, ev, zero, zero, rho, rhoh, 1.5e-5)
+    L.mo_diff_v(d, 0, po.ptr(vt), u, v, w, dzi, dzhi, ev, zero, zero, rho, rhoh, 1.5e-5)
+    L.mo_diff_w(d, po.ptr(wt), u, v, w, dzi, dzhi, ev, rho, rhoh, 1.5e-5)
+    L.mo_diff_c(d, 0, po.ptr(st), po.ptr(_c(i["s"])), dzi, dzhi, ev, zero, zero, rho, rhoh, 1.0 / 3.0, 2e-5)
+    for n, ref in (("ut", ut), ("vt", vt), ("wt", wt), ("st0", st)):
+        err = normwise(fused[n], ref)
+        assert err < TOL, f"oracle {n}: {err:.3e}"
