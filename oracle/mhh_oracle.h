@@ -13,11 +13,16 @@
  * Parity pins (tests/test_oracle_*.py):
  *   - golden vectors written by the compiled reference translation units
  *     (oracle/_ref, built by oracle/Makefile from /root/reference sources):
- *     advection 2i5 and 2, strain2, diff_u/v/w/c, dnmul, diff_2, N2,
- *     buoyancy, rk3;
- *   - closed forms / invariants / brute force for the parts whose reference
- *     translation unit needs FFTW, NetCDF or Boost (pres_2 solve+tdma, FFT,
- *     grid calculate, evisc, cyclic, ghost cells, randomize).
+ *     advection 2i5 and 2 (tendencies and CFL), strain2 (both surface
+ *     variants), diff_u/v/w/c (both surface variants), dnmul, diff_2, rk3;
+ *   - closed forms, invariants, library routines and brute force for the
+ *     parts whose reference translation unit needs FFTW, NetCDF, Boost or a
+ *     Grid object: FFT R2HC/HC2R (numpy rfft), tdma (dense solve), pres_2
+ *     set_values/input/solve (dense 7-point operator), output + divergence
+ *     (divergence-free projection, linear fields), grid calculate, glibc rand
+ *     and randomize, cyclic, set_bc and ghost cells, N2, buoyancy, evisc and
+ *     evisc_neutral (all variants), rk3 amplification polynomial, and the
+ *     model driver (rest state, divergence-free substeps, integer time).
  */
 #ifndef MHH_ORACLE_H
 #define MHH_ORACLE_H

@@ -197,6 +197,11 @@ double mo_model_dn(mo_model* m)
 
 double mo_model_div(mo_model* m)
 {
+    /* Model::print_status (src/model.cxx:382, 852) runs after
+       set_prognostic_cyclic_bcs (src/model.cxx:368): fresh halos. */
+    mo_boundary_cyclic(&m->d, m->u, MO_EDGE_BOTH);
+    mo_boundary_cyclic(&m->d, m->v, MO_EDGE_BOTH);
+    mo_boundary_cyclic(&m->d, m->w, MO_EDGE_BOTH);
     return mo_pres_2_divergence(&m->d, m->u, m->v, m->w, m->dzi, m->rhoref, m->rhorefh);
 }
 

@@ -1,0 +1,521 @@
+"""Pins of the oracle functions that the golden vectors do not cover.
+
+The golden fixtures (tests/test_oracle_golden.py) come from the reference's
+own translation units and pin advection, strain2, diff_u/v/w/c, dnmul, diff_2
+and rk3.  The translation units of the pressure solver (src/pres_2.cxx,
+src/fft.cxx need fftw3.h), of the grid, boundary and fields (netcdf.h, Boost)
+and of calc_evisc (needs a Grid) do not build here, so their restatements are
+pinned against what the mathematics fixes:
+
+  - FFT R2HC/HC2R: numpy's rfft in FFTW's halfcomplex order (library routine);
+  - tdma: a dense solve of the same tridiagonal systems;
+  - Pres_2 set_values+input+solve: a dense solve of the 7-point operator with
+    the reference's boundary conditions, assembled in physical space;
+  - Pres_2 output + divergence: the projected field is divergence free, and a
+    linear field has a known divergence;
+  - grid: the defining relations of the 2nd-order staggered grid;
+  - glibc rand: its published first outputs; randomize: loop order and bounds;
+  - cyclic: periodic extension by modular indexing;
+  - set_bc + ghost cells: the interpolated wall value / wall gradient equal the
+    boundary value;
+  - N2, buoyancy, evisc, evisc_neutral, strain2: closed forms on linear fields;
+  - rk3: the RK3 amplification polynomial on dy/dt = lambda*y;
+  - the model driver: a resting stratified state stays at rest, the projected
+    field is divergence free after every substep, integer time advances.
+
+A deliberately wrong oracle (a dropped term, a flipped sign, a transposed
+index) fails at least one of these; see test_pins_catch_planted_mistakes.
+CPU only.
+"""
+import ctypes as C
+import ctypes.util
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests._golden import normwise
+
+GRAV = 9.81
+DSMALL = 1e-9
+KAPPA = 0.4
+
+
+def ptr(a):
+    return po.ptr(a)
+
+
+def hc_numpy(x, axis=-1):
+    """R2HC along an axis via numpy: r0..r_{n/2}, i_{(n-1)/2}..i_1 (FFTW's halfcomplex)."""
+    x = np.moveaxis(np.asarray(x, dtype=np.float64), axis, -1)
+    n = x.shape[-1]
+    F = np.fft.rfft(x, axis=-1)
+    out = np.empty_like(x)
+    out[..., :n // 2 + 1] = F.real
+    for k in range(1, (n - 1) // 2 + 1):
+        out[..., n - k] = F.imag[..., k]
+    return np.moveaxis(out, -1, axis)
+
+
+def stretched_z(ktot, zsize, growth=1.04):
+    dz = growth ** np.arange(ktot)
+    dz *= zsize / dz.sum()
+    zh = np.concatenate([[0.0], np.cumsum(dz)])
+    return 0.5 * (zh[1:] + zh[:-1])
+
+
+# ------------------------------------------------------------------ FFT
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 12, 15, 16, 30, 64, 100, 512])
+def test_r2hc_matches_numpy_rfft(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n)
+    out = np.zeros(n)
+    po.lib().mo_fft_r2hc(n, ptr(x), ptr(out))
+    assert np.max(np.abs(out - hc_numpy(x))) < 1e-13 * max(1.0, np.max(np.abs(out)))
+    back = np.zeros(n)
+    po.lib().mo_fft_hc2r(n, ptr(out), ptr(back))
+    assert np.max(np.abs(back - n * x)) < 1e-12 * n
+
+
+@pytest.mark.parametrize("shape", [(6, 4, 3), (8, 5, 2), (7, 6, 4)])
+def test_fft_forward_backward_per_slice(shape):
+    itot, jtot, ktot = shape
+    d = po.dims(itot, jtot, ktot, 3, 3, 1, 1.0, 1.0, 1.0)
+    rng = np.random.default_rng(sum(shape))
+    x = rng.standard_normal((ktot, jtot, itot))
+    data = x.copy()
+    po.lib().mo_fft_forward(C.byref(d), ptr(data))
+    ref = hc_numpy(hc_numpy(x, axis=2), axis=1)
+    assert normwise(data, ref) < 1e-13
+    tmp = np.zeros_like(data)
+    po.lib().mo_fft_backward(C.byref(d), ptr(data), ptr(tmp))
+    assert normwise(tmp, x) < 1e-13
+
+
+# ------------------------------------------------------------------ tdma
+def test_tdma_matches_dense_solve():
+    rng = np.random.default_rng(5)
+    ib, jb, km = 5, 3, 9
+    a = rng.uniform(0.5, 1.5, km)
+    c = rng.uniform(0.5, 1.5, km)
+    b = -(a + c)[:, None, None] - rng.uniform(0.1, 2.0, (km, jb, ib))
+    rhs = rng.standard_normal((km, jb, ib))
+    p = rhs.copy()
+    w2, w3 = np.zeros((jb, ib)), np.zeros((km, jb, ib))
+    po.lib().mo_tdma(ptr(a.copy()), ptr(b.copy()), ptr(c.copy()), ptr(p), ptr(w2), ptr(w3), ib, jb, km)
+    for j in range(jb):
+        for i in range(ib):
+            M = np.diag(b[:, j, i]) + np.diag(a[1:], -1) + np.diag(c[:-1], 1)
+            x = np.linalg.solve(M, rhs[:, j, i])
+            assert np.max(np.abs(p[:, j, i] - x)) < 1e-13 * np.max(np.abs(x))
+
+
+# ---------------------------------------------------------------- pres_2
+def _pres_setup(itot=6, jtot=4, ktot=5, seed=3):
+    d = po.dims(itot, jtot, ktot, 3, 3, 1, 60.0 * itot, 40.0 * jtot, 500.0)
+    g = po.grid_arrays(d, stretched_z(ktot, 500.0))
+    rng = np.random.default_rng(seed)
+    rho = po.prof(d); rhoh = po.prof(d)
+    rho[:] = 1.0 + 0.3 * rng.random(d.kcells)
+    rhoh[:] = 1.0 + 0.3 * rng.random(d.kcells)
+    bmati, bmatj = np.zeros(d.itot), np.zeros(d.jtot)
+    a, c = np.zeros(d.kmax), np.zeros(d.kmax)
+    po.lib().mo_pres_2_set_values(C.byref(d), ptr(bmati), ptr(bmatj), ptr(a), ptr(c),
+                                  ptr(g["dz"]), ptr(g["dzhi"]), ptr(rhoh))
+    return d, g, rho, rhoh, bmati, bmatj, a, c
+
+
+def test_set_values_are_eigenvalues_of_periodic_second_difference():
+    d, g, rho, rhoh, bmati, bmatj, a, c = _pres_setup()
+    for n, h, lam in ((d.itot, d.dx, bmati), (d.jtot, d.dy, bmatj)):
+        D = (np.roll(np.eye(n), 1, 0) - 2 * np.eye(n) + np.roll(np.eye(n), -1, 0)) / h ** 2
+        x = np.arange(n)
+        for m in range(n):
+            v = np.cos(2 * np.pi * m * x / n)
+            assert np.max(np.abs(D @ v - lam[m] * v)) < 1e-12 * max(1.0, abs(lam[m]))
+
+
+def _dense_operator(d, g, rho, rhoh):
+    """The 7-point operator of the projection with the reference's boundary
+    conditions, in physical space: periodic x and y; zero gradient at the
+    bottom; zero gradient at the top for the horizontal fluctuations and
+    p = 0 at the top face for the horizontal mean."""
+    ni, nj, nk = d.itot, d.jtot, d.ktot
+    N = ni * nj * nk
+    A = np.zeros((N, N))
+    idx = lambda i, j, k: (i % ni) + (j % nj) * ni + k * ni * nj
+    ks = d.kstart
+    for k in range(nk):
+        kg = k + ks
+        for j in range(nj):
+            for i in range(ni):
+                r = idx(i, j, k)
+                for off, h in ((1, d.dx), (-1, d.dx)):
+                    A[r, idx(i + off, j, k)] += rho[kg] / h ** 2
+                    A[r, r] -= rho[kg] / h ** 2
+                for off, h in ((1, d.dy), (-1, d.dy)):
+                    A[r, idx(i, j + off, k)] += rho[kg] / h ** 2
+                    A[r, r] -= rho[kg] / h ** 2
+                if k + 1 < nk:
+                    f = rhoh[kg + 1] * g["dzhi"][kg + 1] * g["dzi"][kg]
+                    A[r, idx(i, j, k + 1)] += f
+                    A[r, r] -= f
+                else:
+                    f = rhoh[kg + 1] * g["dzhi"][kg + 1] * g["dzi"][kg]
+                    # mean part: p_ghost = -p_top -> -2 f * mean(p_top)
+                    for jj in range(nj):
+                        for ii in range(ni):
+                            A[r, idx(ii, jj, k)] += -2.0 * f / (ni * nj)
+                if k > 0:
+                    f = rhoh[kg] * g["dzhi"][kg] * g["dzi"][kg]
+                    A[r, idx(i, j, k - 1)] += f
+                    A[r, r] -= f
+    return A
+
+
+@pytest.mark.parametrize("shape", [(6, 4, 5), (5, 3, 4), (4, 1, 6)])
+def test_pres_solve_matches_dense_operator(shape):
+    d, g, rho, rhoh, bmati, bmatj, a, c = _pres_setup(*shape)
+    rng = np.random.default_rng(7)
+    rhs = rng.standard_normal((d.ktot, d.jtot, d.itot))
+    p = po.field(d)
+    p.reshape(-1)[:rhs.size] = rhs.reshape(-1)
+    work, b = po.field(d), po.field(d)
+    po.lib().mo_pres_2_solve(C.byref(d), ptr(p), ptr(work), ptr(b), ptr(g["dz"]), ptr(rho),
+                             ptr(bmati), ptr(bmatj), ptr(a), ptr(c))
+    A = _dense_operator(d, g, rho, rhoh)
+    x = np.linalg.solve(A, rhs.reshape(-1)).reshape(rhs.shape)
+    ks, ke, js, je, is_, ie = d.kstart, d.kend, d.jstart, d.jend, d.istart, d.iend
+    assert normwise(p[ks:ke, js:je, is_:ie], x) < 1e-11
+    # bottom ghost level copies kstart; halos are periodic
+    assert np.array_equal(p[ks - 1, js:je, is_:ie], p[ks, js:je, is_:ie])
+    assert np.array_equal(p[ks:ke, js:je, :is_], p[ks:ke, js:je, ie - d.igc:ie])
+
+
+def test_projection_is_divergence_free_and_divergence_is_exact_on_linear_fields():
+    d, g, rho, rhoh, bmati, bmatj, a, c = _pres_setup(8, 6, 7, seed=9)
+    L = po.lib()
+    rng = np.random.default_rng(11)
+    u, v, w, ut, vt, wt = (rng.standard_normal((d.kcells, d.jcells, d.icells)) for _ in range(6))
+    for f in (w, wt):  # impermeable walls
+        f[:d.kstart + 1] = 0.0
+        f[d.kend:] = 0.0
+    for f in (u, v, w):
+        L.mo_boundary_cyclic(C.byref(d), ptr(f), 2)
+    dt = 0.3
+    new = lambda f, ft: f + dt * ft
+    before = L.mo_pres_2_divergence(C.byref(d), ptr(new(u, ut)), ptr(new(v, vt)), ptr(new(w, wt)),
+                                    ptr(g["dzi"]), ptr(rho), ptr(rhoh))
+    p, work, b = po.field(d), po.field(d), po.field(d)
+    L.mo_pres_2_input(C.byref(d), ptr(p), ptr(u), ptr(v), ptr(w), ptr(ut), ptr(vt), ptr(wt),
+                      ptr(g["dzi"]), ptr(rho), ptr(rhoh), dt)
+    L.mo_pres_2_solve(C.byref(d), ptr(p), ptr(work), ptr(b), ptr(g["dz"]), ptr(rho),
+                      ptr(bmati), ptr(bmatj), ptr(a), ptr(c))
+    L.mo_pres_2_output(C.byref(d), ptr(ut), ptr(vt), ptr(wt), ptr(p), ptr(g["dzhi"]))
+    un, vn, wn = new(u, ut), new(v, vt), new(w, wt)
+    for f in (un, vn, wn):
+        L.mo_boundary_cyclic(C.byref(d), ptr(f), 2)
+    after = L.mo_pres_2_divergence(C.byref(d), ptr(un), ptr(vn), ptr(wn),
+                                   ptr(g["dzi"]), ptr(rho), ptr(rhoh))
+    assert before > 1e-2
+    assert after < 1e-12 * before
+
+    # u = x, v = 2y, w = 3z: div = rho*(1 + 2) + 3 with rhoh = 1
+    ones = po.prof(d, 1.0)
+    ii = np.arange(d.icells)[None, None, :] * d.dx
+    jj = np.arange(d.jcells)[None, :, None] * d.dy
+    ul = np.broadcast_to(ii, (d.kcells, d.jcells, d.icells)).copy()
+    vl = np.broadcast_to(2.0 * jj, (d.kcells, d.jcells, d.icells)).copy()
+    wl = np.broadcast_to(3.0 * g["zh"][:, None, None], (d.kcells, d.jcells, d.icells)).copy()
+    div = L.mo_pres_2_divergence(C.byref(d), ptr(ul), ptr(vl), ptr(wl), ptr(g["dzi"]), ptr(rho), ptr(ones))
+    expect = np.max(3.0 * rho[d.kstart:d.kend] + 3.0)
+    assert abs(div - expect) < 1e-12 * expect
+
+
+# ------------------------------------------------------------------ grid
+def test_grid_calculate_defining_relations():
+    d = po.dims(4, 4, 12, 3, 3, 1, 1.0, 1.0, 900.0)
+    zi = stretched_z(12, 900.0, growth=1.07)
+    g = po.grid_arrays(d, zi)
+    ks, ke = d.kstart, d.kend
+    z, zh = g["z"], g["zh"]
+    assert zh[ks] == 0.0 and zh[ke] == 900.0
+    assert np.allclose(zh[ks + 1:ke], 0.5 * (z[ks:ke - 1] + z[ks + 1:ke]), rtol=0, atol=1e-12)
+    assert z[ks - 1] == -z[ks] and abs(z[ke] - (1800.0 - z[ke - 1])) < 1e-12
+    assert abs(g["dz"][ks:ke].sum() - 900.0) < 1e-10
+    assert np.allclose(g["dzh"][1:], np.diff(z), rtol=0, atol=1e-12)
+    assert np.allclose(g["dzi"][ks:ke] * g["dz"][ks:ke], 1.0, rtol=1e-15)
+    assert np.allclose(g["dzhi"][ks:ke + 1] * g["dzh"][ks:ke + 1], 1.0, rtol=1e-15)
+
+
+# ---------------------------------------------------------------- fields
+def test_rand_is_glibc_and_randomize_order():
+    L = po.lib()
+    L.mo_srand(1)
+    # the published first outputs of glibc's rand() after srand(1)
+    assert [L.mo_rand() for _ in range(3)] == [1804289383, 846930886, 1681692777]
+    libc = C.CDLL(ctypes.util.find_library("c"))
+    d = po.dims(5, 4, 8, 3, 3, 1, 1.0, 1.0, 800.0)
+    g = po.grid_arrays(d, stretched_z(8, 800.0, 1.0))
+    data = po.field(d)
+    rndz, amp, rexp = 450.0, 0.2, 2.0
+    L.mo_srand(4)
+    L.mo_randomize(C.byref(d), ptr(data), ptr(g["z"]), amp, rndz, rexp)
+    libc.srand(4)
+    kend = d.kstart + int(np.sum(g["z"][d.kstart:d.kend] < rndz))
+    for k in range(d.kstart, kend):
+        fac = ((rndz - g["z"][k]) / rndz) ** rexp
+        for j in range(d.jstart, d.jend):
+            for i in range(d.istart, d.iend):
+                r = libc.rand()
+                assert data[k, j, i] == fac * amp * (r / 2147483647.0 - 0.5)
+                assert abs(data[k, j, i]) <= 0.5 * amp * fac
+    assert not data[kend:].any() and not data[:d.kstart].any()
+
+
+# -------------------------------------------------------------- boundary
+@pytest.mark.parametrize("edge", [0, 1, 2])
+@pytest.mark.parametrize("jtot", [4, 1])
+def test_cyclic_is_periodic_extension(edge, jtot):
+    d = po.dims(5, jtot, 3, 3, 3, 1, 1.0, 1.0, 1.0)
+    rng = np.random.default_rng(edge)
+    a = rng.standard_normal((d.kcells, d.jcells, d.icells))
+    out = a.copy()
+    po.lib().mo_boundary_cyclic(C.byref(d), ptr(out), edge)
+    exp = a.copy()
+    ii = d.istart + (np.arange(d.icells) - d.istart) % d.itot
+    jj = d.jstart + (np.arange(d.jcells) - d.jstart) % d.jtot
+    if edge in (0, 2):
+        exp = exp[:, :, ii]
+    if edge in (1, 2):
+        if jtot > 1:
+            exp = exp[:, jj, :]
+        else:
+            exp[d.kstart:d.kend] = exp[d.kstart:d.kend][:, jj, :]
+    assert np.array_equal(out, exp)
+
+
+def test_set_bc_and_ghost_cells_reproduce_wall_values():
+    L = po.lib()
+    d = po.dims(4, 3, 6, 3, 3, 1, 1.0, 1.0, 600.0)
+    g = po.grid_arrays(d, stretched_z(6, 600.0))
+    rng = np.random.default_rng(2)
+    ks, ke = d.kstart, d.kend
+    a, grad, flux = po.plane(d), po.plane(d), po.plane(d)
+    L.mo_set_bc(C.byref(d), ptr(a), ptr(grad), ptr(flux), 0, 301.5, 1e-5, 300.0, 1)
+    assert np.all(a == 1.5)
+    L.mo_set_bc(C.byref(d), ptr(a), ptr(grad), ptr(flux), 1, 0.006, 2e-5, 0.0, 1)
+    assert np.all(grad == 0.006) and np.allclose(flux, -0.006 * 2e-5)
+    L.mo_set_bc(C.byref(d), ptr(a), ptr(grad), ptr(flux), 2, 0.1, 2e-5, 0.0, 1)
+    assert np.all(flux == 0.1) and np.allclose(grad, -0.1 / 2e-5)
+
+    f = rng.standard_normal((d.kcells, d.jcells, d.icells))
+    bot, top = rng.standard_normal((d.jcells, d.icells)), rng.standard_normal((d.jcells, d.icells))
+    L.mo_ghost_cells_bot_2nd(C.byref(d), ptr(f), ptr(g["dzh"]), 0, ptr(bot), None)
+    L.mo_ghost_cells_top_2nd(C.byref(d), ptr(f), ptr(g["dzh"]), 0, ptr(top), None)
+    assert np.allclose(0.5 * (f[ks - 1] + f[ks]), bot, atol=1e-14)
+    assert np.allclose(0.5 * (f[ke - 1] + f[ke]), top, atol=1e-14)
+    L.mo_ghost_cells_bot_2nd(C.byref(d), ptr(f), ptr(g["dzh"]), 1, None, ptr(bot))
+    L.mo_ghost_cells_top_2nd(C.byref(d), ptr(f), ptr(g["dzh"]), 1, None, ptr(top))
+    assert np.allclose((f[ks] - f[ks - 1]) / g["dzh"][ks], bot, atol=1e-12)
+    assert np.allclose((f[ke] - f[ke - 1]) / g["dzh"][ke], top, atol=1e-12)
+    # "off": linear extrapolation to the top wall, exact for a linear profile on a uniform grid
+    du = po.dims(4, 3, 6, 3, 3, 1, 1.0, 1.0, 600.0)
+    gu = po.grid_arrays(du, (np.arange(6) + 0.5) * 100.0)
+    lin = np.broadcast_to((2.0 * gu["z"] + 1.0)[:, None, None], f.shape).copy()
+    atop = po.plane(du)
+    L.mo_ghost_cells_top_2nd(C.byref(du), ptr(lin), ptr(gu["dzh"]), 4, ptr(atop), None)
+    assert np.allclose(atop, 2.0 * 600.0 + 1.0, rtol=1e-14)
+
+
+# ------------------------------------------------------------ smag2, thermo
+def _uniform(ktot=6, dz=50.0, itot=5, jtot=4):
+    d = po.dims(itot, jtot, ktot, 3, 3, 1, 40.0 * itot, 30.0 * jtot, dz * ktot)
+    g = po.grid_arrays(d, (np.arange(ktot) + 0.5) * dz)
+    return d, g
+
+
+def test_N2_and_buoyancy_on_linear_theta():
+    d, g = _uniform()
+    gamma, thref0 = 0.006, 300.0
+    th = np.broadcast_to((thref0 + gamma * g["z"])[:, None, None], (d.kcells, d.jcells, d.icells)).copy()
+    thref = po.prof(d, thref0)
+    N2 = po.field(d)
+    po.lib().mo_calc_N2(C.byref(d), ptr(N2), ptr(th), ptr(g["dzi"]), ptr(thref))
+    ks, ke = d.kstart, d.kend
+    assert np.allclose(N2[ks:ke, d.jstart:d.jend, d.istart:d.iend], GRAV / thref0 * gamma, rtol=1e-12)
+    # buoyancy: g/threfh * (theta at the half level - threfh), from kstart+1
+    threfh = po.prof(d, thref0)
+    wt = po.field(d)
+    po.lib().mo_buoyancy_tend_2nd(C.byref(d), ptr(wt), ptr(th), ptr(threfh))
+    expect = GRAV / thref0 * (gamma * g["zh"])
+    inner = wt[:, d.jstart:d.jend, d.istart:d.iend]
+    assert np.allclose(inner[ks + 1:ke], expect[ks + 1:ke, None, None], rtol=1e-12, atol=1e-15)
+    assert not inner[ks].any()
+
+
+def test_strain2_on_linear_shear():
+    d, g = _uniform()
+    S = 0.01
+    u = np.broadcast_to((S * g["z"])[:, None, None], (d.kcells, d.jcells, d.icells)).copy()
+    zero = po.field(d)
+    s2 = po.field(d)
+    po.lib().mo_calc_strain2(C.byref(d), 0, ptr(s2), ptr(u), ptr(zero), ptr(zero), None, None,
+                             ptr(g["z"]), ptr(g["dzi"]), ptr(g["dzhi"]))
+    inner = s2[d.kstart:d.kend, d.jstart:d.jend, d.istart:d.iend]
+    assert np.allclose(inner, S * S + DSMALL, rtol=1e-12)
+
+
+@pytest.mark.parametrize("N2val", [0.0, 2e-5, -3e-5, 5.0])
+def test_evisc_closed_form(N2val):
+    d, g = _uniform()
+    cs, tPr, S2 = 0.23, 1.0 / 3.0, 4e-4
+    ev = po.field(d)
+    ev[d.kstart:d.kend] = S2
+    N2 = po.field(d, N2val)
+    po.lib().mo_calc_evisc(C.byref(d), 0, 0, ptr(ev), ptr(N2), None, ptr(g["z"]), ptr(g["dz"]), None, cs, tPr)
+    mlen = cs * (d.dx * d.dy * 50.0) ** (1.0 / 3.0)
+    ratio = min(N2val / (S2 * tPr), 1.0 - DSMALL)
+    expect = mlen ** 2 * np.sqrt(S2) * np.sqrt(1.0 - ratio)
+    inner = ev[d.kstart:d.kend, d.jstart:d.jend, d.istart:d.iend]
+    assert np.allclose(inner, expect, rtol=1e-12)
+    # mirrored walls
+    assert np.array_equal(ev[d.kstart - 1], ev[d.kstart]) and np.array_equal(ev[d.kend], ev[d.kend - 1])
+
+
+def test_evisc_mason_wall_damping_closed_form():
+    d, g = _uniform()
+    cs, tPr, S2 = 0.23, 1.0 / 3.0, 4e-4
+    ev = po.field(d)
+    ev[d.kstart:d.kend] = S2
+    z0m = po.plane(d, 0.1)
+    bgrad = po.plane(d, 0.0)
+    N2 = po.field(d, 0.0)
+    po.lib().mo_calc_evisc(C.byref(d), 1, 1, ptr(ev), ptr(N2), ptr(bgrad), ptr(g["z"]), ptr(g["dz"]),
+                           ptr(z0m), cs, tPr)
+    mlen0 = cs * (d.dx * d.dy * 50.0) ** (1.0 / 3.0)
+    for k in range(d.kstart, d.kend):
+        mlen = (mlen0 ** -2 + (KAPPA * (g["z"][k] + 0.1)) ** -2) ** -0.5
+        got = ev[k, d.jstart:d.jend, d.istart:d.iend]
+        assert np.allclose(got, mlen ** 2 * np.sqrt(S2), rtol=1e-12), k
+
+
+@pytest.mark.parametrize("mason", [0, 1])
+def test_evisc_neutral_surface_closed_form(mason):
+    d, g = _uniform()
+    cs, S2 = 0.23, 4e-4
+    ev = po.field(d)
+    ev[d.kstart:d.kend] = S2
+    z0m = po.plane(d, 0.1)
+    zero = po.field(d)
+    po.lib().mo_calc_evisc_neutral(C.byref(d), 1, mason, ptr(ev), ptr(zero), ptr(zero), ptr(g["z"]),
+                                   ptr(g["dz"]), ptr(g["dzhi"]), ptr(z0m), cs, 1e-5)
+    mlen0 = cs * (d.dx * d.dy * 50.0) ** (1.0 / 3.0)
+    for k in range(d.kstart, d.kend):
+        # Mason-Thomson blending with exponent 1 in the neutral case
+        mlen = 1.0 / (1.0 / mlen0 + 1.0 / (KAPPA * (g["z"][k] + 0.1))) if mason else mlen0
+        got = ev[k, d.jstart:d.jend, d.istart:d.iend]
+        assert np.allclose(got, mlen ** 2 * np.sqrt(S2), rtol=1e-12), k
+
+
+def test_evisc_neutral_van_driest_closed_form():
+    d, g = _uniform(ktot=8, dz=0.05)
+    cs, visc, S2 = 0.1, 1e-3, 1.0
+    ks, ke = d.kstart, d.kend
+    u = po.field(d)
+    v = po.field(d)
+    du_bot, du_top = 2.0, 0.5
+    u[ks] = 0.0
+    u[ks - 1] = -du_bot * g["dzh"][ks]
+    u[ke - 1] = 0.0
+    u[ke] = du_top * g["dzh"][ke]
+    ev = po.field(d)
+    ev[ks:ke] = S2
+    po.lib().mo_calc_evisc_neutral(C.byref(d), 0, 0, ptr(ev), ptr(u), ptr(v), ptr(g["z"]), ptr(g["dz"]),
+                                   ptr(g["dzhi"]), None, cs, visc)
+    mlen = cs * (d.dx * d.dy * 0.05) ** (1.0 / 3.0)
+    ut_b, ut_t = np.sqrt(visc * du_bot), np.sqrt(visc * du_top)
+    zsize = 8 * 0.05
+    for k in range(ks, ke):
+        z = g["z"][k]
+        fac = min(1 - np.exp(-z * ut_b / (26 * visc)), 1 - np.exp(-(zsize - z) * ut_t / (26 * visc)))
+        got = ev[k, d.jstart:d.jend, d.istart:d.iend]
+        assert np.allclose(got, (fac * mlen) ** 2 * np.sqrt(S2), rtol=1e-12), k
+
+
+# ------------------------------------------------------------------ rk3
+@pytest.mark.parametrize("lam", [-0.7, 0.3, -2.5])
+def test_rk3_reproduces_amplification_polynomial(lam):
+    d = po.dims(4, 3, 2, 3, 3, 1, 1.0, 1.0, 1.0)
+    rng = np.random.default_rng(0)
+    a0 = rng.standard_normal((d.kcells, d.jcells, d.icells))
+    a, at = a0.copy(), po.field(d)
+    dt = 0.9
+    inner = (slice(d.kstart, d.kend), slice(d.jstart, d.jend), slice(d.istart, d.iend))
+    for s in range(3):
+        at[inner] += lam * a[inner]
+        po.lib().mo_rk3(C.byref(d), ptr(a), ptr(at), s, dt)
+    z = lam * dt
+    R = 1 + z + z * z / 2 + z ** 3 / 6
+    assert np.allclose(a[inner], R * a0[inner], rtol=1e-14, atol=1e-15)
+    assert not at.any()
+    sub = [po.lib().mo_rk3_subdt(dt, s) for s in range(3)]
+    assert np.allclose(sub, [dt / 3, 15 * dt / 16, 8 * dt / 15], rtol=1e-15)
+    assert po.lib().mo_convert_to_itime(1.5) == 1500000000
+    assert po.lib().mo_convert_to_itime(0.1) == 100000000
+
+
+# ----------------------------------------------------------------- driver
+def _model(fixed_dt=None, noise=True, nx=12, ny=10, nz=12):
+    from microhh_amd import model as mm
+    case = mm.dry_cbl_case(itot=nx, jtot=ny, ktot=nz, xsize=50.0 * nx, ysize=50.0 * ny, zsize=1600.0,
+                           adaptive=fixed_dt is None, dt=fixed_dt or 10.0,
+                           rndamp_uvw=0.02 if noise else 0.0, rndamp_th=0.2 if noise else 0.0)
+    if not noise:
+        case["sbot"] = 300.0  # no surface heating: the stratified state is at rest
+    return po.Model(mm.oracle_case(case), *mm.profiles(case))
+
+
+def test_model_rest_state_stays_at_rest():
+    m = _model(fixed_dt=5.0, noise=False)
+    for _ in range(6):
+        m.substep()
+    for f in ("u", "v", "w"):
+        assert np.max(np.abs(m.field(f))) < 1e-12, f
+    m.close()
+
+
+def test_model_divergence_free_and_integer_time():
+    m = _model(fixed_dt=2.0)
+    L = po.lib()
+    for n in range(6):
+        m.substep()
+        assert L.mo_model_div(m.h) < 1e-13
+    assert L.mo_model_time(m.h) == 4.0 and L.mo_model_iteration(m.h) == 2
+    m.close()
+
+
+# ------------------------------------------------------- planted mistakes
+def test_pins_catch_planted_mistakes():
+    """Re-run two pins against deliberately wrong variants of the oracle's
+    results, to show the checks have teeth: a dropped vertical term in the
+    pressure operator and a flipped sign in the halfcomplex imaginary part."""
+    d, g, rho, rhoh, bmati, bmatj, a, c = _pres_setup(4, 3, 4)
+    rng = np.random.default_rng(1)
+    rhs = rng.standard_normal((d.ktot, d.jtot, d.itot))
+    p = po.field(d)
+    p.reshape(-1)[:rhs.size] = rhs.reshape(-1)
+    work, b = po.field(d), po.field(d)
+    c_wrong = 0.5 * c  # halves the upper-face coupling
+    po.lib().mo_pres_2_solve(C.byref(d), ptr(p), ptr(work), ptr(b), ptr(g["dz"]), ptr(rho),
+                             ptr(bmati), ptr(bmatj), ptr(a), ptr(c_wrong))
+    x = np.linalg.solve(_dense_operator(d, g, rho, rhoh), rhs.reshape(-1)).reshape(rhs.shape)
+    assert normwise(p[d.kstart:d.kend, d.jstart:d.jend, d.istart:d.iend], x) > 1e-3
+
+    n = 8
+    x = rng.standard_normal(n)
+    out = np.zeros(n)
+    po.lib().mo_fft_r2hc(n, ptr(x), ptr(out))
+    wrong = out.copy()
+    wrong[n // 2 + 1:] *= -1
+    assert np.max(np.abs(wrong - hc_numpy(x))) > 1e-3
