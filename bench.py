@@ -15,7 +15,8 @@ The line also carries
   substep_roofline : the whole substep's canonical 40 field passes per point
                  over the measured substep time;
   cpu_baseline : the FP64 CPU oracle (C restatement of the reference loops)
-                 timed on this host on a bounded sample of the same workload.
+                 timed on this host on a bounded sample of the same workload,
+                 serial and with OpenMP over the available cores.
 """
 import argparse
 import ctypes as C
@@ -52,6 +53,18 @@ PASSES = {
     "pres_out": 7, "pres_out_rk3": 13, "rk3": 4, "buoyancy": 3,
 }
 KERNELS = list(PASSES) + ["cyclic", "ghost_cells", "mirror_walls", "rk3_zero_ghosts", "cfl", "dnmul"]
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (scaled by the copy-kernel calibration: x2.000 / x1.000) on the default
+# workload; profiles/dycore_fp64_512x512x256_hbm_traffic.md holds the
+# commands and the full table.  Reported as roofline.traffic when the
+# workload and the dominant kernel match.
+MEASURED_TRAFFIC = {
+    (512, 512, 256, 8): {
+        "tend_2i5_smag2": 9053.3e6, "pres_out_rk3": 8322.1e6, "pres_in": 4687.4e6, "evisc": 4569.1e6,
+        "tdma": 2671.8e6, "rk3": 2483.0e6, "fft_x_bwd": 1091.7e6, "fft_y_fwd": 1078.1e6,
+        "fft_y_bwd": 1078.1e6, "fft_x_fwd": 1076.2e6,
+    },
+}
 # The canonical fused minimum per grid-point-substep (40 passes).
 B_ALG_PASSES = 40
 
@@ -60,23 +73,58 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(precision_bytes):
-    """Time the serial FP64 oracle on a bounded sample: one RK3 step (3 substeps)
-    of the same physics on a 192x192x256 slab at the workload's grid spacing."""
+def _time_oracle(n, ktot, nthreads):
     from microhh_amd import model as mm
     from oracle import pyoracle as po
-    n = 192
-    case = mm.dry_cbl_case(itot=n, jtot=n, ktot=256, xsize=50.0 * n, ysize=50.0 * n, zsize=3200.0)
-    o = po.Model(mm.oracle_case(case), *mm.profiles(case))
+    case = mm.dry_cbl_case(itot=n, jtot=n, ktot=ktot, xsize=50.0 * n, ysize=50.0 * n, zsize=12.5 * ktot)
+    oc = mm.oracle_case(case)
+    oc.nthreads = nthreads
+    o = po.Model(oc, *mm.profiles(case))
     t0 = time.perf_counter()
     for _ in range(3):
         o.substep()
     el = time.perf_counter() - t0
     o.close()
-    pts = n * n * 256 * 3
-    return {"value": round(pts / el / 1e6, 3), "unit": UNIT, "cores": 1, "kind": "port",
-            "sample": f"1 RK3 step (3 substeps) of the dry CBL on a {n}x{n}x256 slab (dx=dy=50 m, dz=12.5 m), "
-                      f"serial FP64 oracle (C restatement of the reference loops), {el:.1f} s"}
+    return n * n * ktot * 3 / el / 1e6, el
+
+
+def cpu_baseline():
+    """Time the FP64 oracle (C restatement of the reference loops) on a
+    bounded sample of the workload: one RK3 step (3 substeps) of the same
+    physics on a 256x256x256 slab at the workload's grid spacing.  Serial
+    (the reference's CPU semantics) and with OpenMP over the available cores."""
+    n, ktot = 256, 256
+    v1, el1 = _time_oracle(n, ktot, 1)
+    ncores = min(len(os.sched_getaffinity(0)), 16)
+    sample = (f"1 RK3 step (3 substeps) of the dry CBL on a {n}x{n}x{ktot} slab (dx=dy=50 m, dz=12.5 m), "
+              f"FP64 oracle (C restatement of the reference loops)")
+    out = {"value": round(v1, 3), "unit": UNIT, "cores": 1, "kind": "port",
+           "sample": f"{sample}, serial, {el1:.1f} s"}
+    if ncores > 1:
+        vn, eln = _time_oracle(n, ktot, ncores)
+        out["openmp"] = {"value": round(vn, 3), "unit": UNIT, "cores": ncores, "kind": "port",
+                         "sample": f"{sample}, OpenMP, {eln:.1f} s"}
+    return out
+
+
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """Device-to-device copies of a known size through the library's plain
+    streaming copy kernel: the reference point for the HBM traffic counters
+    (a copy of B bytes reads B and writes B) and a measured copy bandwidth."""
+    from microhh_amd import abi
+    L = abi.lib()
+    src, dst = C.c_void_p(), C.c_void_p()
+    abi.check(L.mhh_device_alloc(nbytes, C.byref(src)), "alloc")
+    abi.check(L.mhh_device_alloc(nbytes, C.byref(dst)), "alloc")
+    abi.check(L.mhh_copy_device(dst, src, nbytes), "copy")
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        abi.check(L.mhh_copy_device(dst, src, nbytes), "copy")
+    el = (time.perf_counter() - t0) / reps
+    L.mhh_device_free(src)
+    L.mhh_device_free(dst)
+    print(json.dumps({"mode": "copy_calibration", "bytes_per_copy": nbytes, "copies": reps + 1,
+                      "ms_per_copy": round(el * 1e3, 4), "gbs_read_plus_write": round(2 * nbytes / el / 1e9, 1)}))
 
 
 def main():
@@ -90,7 +138,12 @@ def main():
     ap.add_argument("--ktot", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel events in the timed steps")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the known-size copy kernel (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a "
@@ -149,8 +202,10 @@ def main():
     if kernels:
         dom = max((k for k in kernels if k in PASSES), key=lambda k: kernels[k]["ms_total"])
         e = kernels[dom]
+        traffic = MEASURED_TRAFFIC.get((itot, jtot, ktot, esize), {}).get(dom)
         roof = {"bound": "hbm", "achieved": e["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(e["gbs"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "frac": round(e["gbs"] / HBM_PEAK_GBS, 4),
+                "traffic": int(traffic) if traffic else None, "kernel": dom,
                 "alg_bytes_per_launch": e["alg_bytes"], "ms_per_launch": e["ms_mean"]}
     sub_ms = elapsed / nsub * 1e3
     substep_gbs = B_ALG_PASSES * esize * npts / (sub_ms * 1e-3) / 1e9
@@ -162,7 +217,7 @@ def main():
     if not args.no_cpu:
         log("bench: CPU baseline (serial oracle, bounded sample)")
         try:
-            cpu = cpu_baseline(esize)
+            cpu = cpu_baseline()
         except Exception as exc:  # the oracle library is test infrastructure; report, do not fail the bench
             cpu = {"error": str(exc)}
 

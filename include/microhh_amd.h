@@ -182,6 +182,9 @@ int  mhh_device_alloc(size_t bytes, void** ptr);
 int  mhh_device_free(void* ptr);
 int  mhh_copy_to_device(void* dst, const void* src, size_t bytes);
 int  mhh_copy_to_host(void* dst, const void* src, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel (16-byte aligned); also the
+ * known-size reference kernel for HBM traffic counters. */
+int  mhh_copy_device(void* dst, const void* src, size_t bytes);
 int  mhh_device_synchronize(void);
 
 /* ---- kernel timing ----

@@ -82,6 +82,7 @@ SIGNATURES = {
     "mhh_device_free": (C.c_int, [VP]),
     "mhh_copy_to_device": (C.c_int, [VP, VP, C.c_size_t]),
     "mhh_copy_to_host": (C.c_int, [VP, VP, C.c_size_t]),
+    "mhh_copy_device": (C.c_int, [VP, VP, C.c_size_t]),
     "mhh_device_synchronize": (C.c_int, []),
     "mhh_timing_enable": (C.c_int, [VP, C.c_int]),
     "mhh_timing_reset": (C.c_int, [VP]),

@@ -1,4 +1,6 @@
 // mhh_ctx.hip -- context lifetime, field binding, errors and max-reductions.
+#include <algorithm>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <limits>
@@ -278,6 +280,30 @@ int mhh_copy_to_host(void* dst, const void* src, size_t bytes)
 {
     if (bytes && (!dst || !src)) return mhh::fail(MHH_EINVAL, "mhh_copy_to_host: null pointer");
     return mhh::check_hip(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+}
+
+namespace
+{
+    // Plain streaming copy, 16 bytes per thread per iteration (grid-stride).
+    __global__ void __launch_bounds__(256) copy_g(double2* __restrict__ dst, const double2* __restrict__ src, size_t n)
+    {
+        const size_t stride = static_cast<size_t>(gridDim.x)*blockDim.x;
+        for (size_t i = static_cast<size_t>(blockIdx.x)*blockDim.x + threadIdx.x; i < n; i += stride)
+            dst[i] = src[i];
+    }
+}
+
+int mhh_copy_device(void* dst, const void* src, size_t bytes)
+{
+    if (bytes && (!dst || !src)) return mhh::fail(MHH_EINVAL, "mhh_copy_device: null pointer");
+    if (bytes % 16 || reinterpret_cast<uintptr_t>(dst) % 16 || reinterpret_cast<uintptr_t>(src) % 16)
+        return mhh::fail(MHH_EINVAL, "mhh_copy_device: pointers and size must be 16-byte aligned");
+    if (!bytes) return 0;
+    const size_t n = bytes / 16;
+    const unsigned int nb = static_cast<unsigned int>(std::min<size_t>((n + 255)/256, 8192));
+    copy_g<<<nb, 256>>>(static_cast<double2*>(dst), static_cast<const double2*>(src), n);
+    MHH_CHECK(mhh::check_launch("copy"));
+    return mhh::check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
 }
 
 int mhh_device_synchronize(void)

@@ -110,6 +110,7 @@ void mo_add_mean_prof(const mo_dims* d, double* data, const double* prof, double
 {
     /* src/fields.cxx:1082-1101: prof holds ktot values starting at kstart. */
     const int jj = d->icells, kk = d->ijcells;
+    #pragma omp parallel for
     for (int k=d->kstart; k<d->kend; ++k)
         for (int j=d->jstart; j<d->jend; ++j)
             for (int i=d->istart; i<d->iend; ++i)
@@ -123,10 +124,12 @@ void mo_boundary_cyclic(const mo_dims* d, double* a, int edge)
     const int jj = d->icells, kk = d->ijcells;
     if (edge == MO_EDGE_EW || edge == MO_EDGE_BOTH)
     {
+        #pragma omp parallel for
         for (int k=0; k<d->kcells; ++k)
             for (int j=0; j<d->jcells; ++j)
                 for (int i=0; i<d->igc; ++i)
                     a[i + j*jj + k*kk] = a[d->iend - d->igc + i + j*jj + k*kk];
+        #pragma omp parallel for
         for (int k=0; k<d->kcells; ++k)
             for (int j=0; j<d->jcells; ++j)
                 for (int i=0; i<d->igc; ++i)
@@ -269,6 +272,7 @@ void mo_calc_strain2(const mo_dims* d, int surface_model, double* strain2,
             }
     }
 
+    #pragma omp parallel for
     for (int k=d->kstart+k_offset; k<d->kend; ++k)
         for (int j=d->jstart; j<d->jend; ++j)
             for (int i=d->istart; i<d->iend; ++i)
@@ -284,6 +288,7 @@ void mo_calc_N2(const mo_dims* d, double* N2, const double* th, const double* dz
 {
     /* src/thermo_dry.cxx:66-79 */
     const int jj = d->icells, kk = d->ijcells;
+    #pragma omp parallel for
     for (int k=d->kstart; k<d->kend; ++k)
         for (int j=d->jstart; j<d->jend; ++j)
             for (int i=d->istart; i<d->iend; ++i)
@@ -314,6 +319,7 @@ void mo_calc_evisc(const mo_dims* d, int surface_model, int sw_mason, double* ev
 
     if (!surface_model)
     {
+        #pragma omp parallel for
         for (int k=d->kstart; k<d->kend; ++k)
         {
             const double mlen = cs*pow(dx*dy*dz[k], 1./3.);
@@ -348,6 +354,7 @@ void mo_calc_evisc(const mo_dims* d, int surface_model, int sw_mason, double* ev
                     evisc[ijk] = sq(mlen) * sqrt(evisc[ijk]) * sqrt(1.-RitPrratio);
                 }
         }
+        #pragma omp parallel for
         for (int k=d->kstart+1; k<d->kend; ++k)
         {
             const double mlen0 = cs*pow(dx*dy*dz[k], 1./3.);
@@ -382,6 +389,7 @@ void mo_calc_evisc_neutral(const mo_dims* d, int surface_model, int sw_mason, do
 
     if (!surface_model)
     {
+        #pragma omp parallel for
         for (int k=ks; k<ke; ++k)
         {
             const double mlen_smag = cs*pow(dx*dy*dz[k], 1./3.);
@@ -407,6 +415,7 @@ void mo_calc_evisc_neutral(const mo_dims* d, int surface_model, int sw_mason, do
     }
     else
     {
+        #pragma omp parallel for
         for (int k=ks; k<ke; ++k)
         {
             const double mlen0 = cs*pow(dx*dy*dz[k], 1./3.);
@@ -478,6 +487,7 @@ void mo_diff_u(const mo_dims* d, int surface_model, double* ut,
             }
     }
 
+    #pragma omp parallel for
     for (int k=ks+k_offset; k<ke-k_offset; ++k)
         for (int j=d->jstart; j<d->jend; ++j)
             for (int i=d->istart; i<d->iend; ++i)
@@ -551,6 +561,7 @@ void mo_diff_v(const mo_dims* d, int surface_model, double* vt,
             }
     }
 
+    #pragma omp parallel for
     for (int k=ks+k_offset; k<ke-k_offset; ++k)
         for (int j=d->jstart; j<d->jend; ++j)
             for (int i=d->istart; i<d->iend; ++i)
@@ -580,6 +591,7 @@ void mo_diff_w(const mo_dims* d, double* wt,
     /* include/diff_kernels.h:342-390 */
     const int ii = 1, jj = d->icells, kk = d->ijcells;
     const double dxi = 1./d->dx, dyi = 1./d->dy;
+    #pragma omp parallel for
     for (int k=d->kstart+1; k<d->kend; ++k)
         for (int j=d->jstart; j<d->jend; ++j)
             for (int i=d->istart; i<d->iend; ++i)
@@ -653,6 +665,7 @@ void mo_diff_c(const mo_dims* d, int surface_model, double* at, const double* a,
             }
     }
 
+    #pragma omp parallel for
     for (int k=ks+k_offset; k<ke-k_offset; ++k)
         for (int j=d->jstart; j<d->jend; ++j)
             for (int i=d->istart; i<d->iend; ++i)
@@ -681,6 +694,7 @@ double mo_calc_dnmul(const mo_dims* d, const double* evisc, const double* dzi, d
     const double dxidxi = 1./(d->dx*d->dx), dyidyi = 1./(d->dy*d->dy);
     const double tPrfac_i = 1./fmin2(1., tPr);
     double dnmul = 0;
+    #pragma omp parallel for reduction(max:dnmul)
     for (int k=d->kstart; k<d->kend; ++k)
         for (int j=d->jstart; j<d->jend; ++j)
             for (int i=d->istart; i<d->iend; ++i)
@@ -703,6 +717,7 @@ double mo_advec_2i5_cfl(const mo_dims* d, const double* u, const double* v, cons
     const int ks = d->kstart, ke = d->kend;
     double cfl = 0;
 
+    #pragma omp parallel for reduction(max:cfl)
     for (int k=ks; k<ke; ++k)
         for (int j=d->jstart; j<d->jend; ++j)
             for (int i=d->istart; i<d->iend; ++i)
@@ -802,6 +817,7 @@ void mo_advec_2i5_u(const mo_dims* d, double* ut, const double* u, const double*
     const double dxi = 1./d->dx, dyi = 1./d->dy;
     const int ks = d->kstart, ke = d->kend;
 
+    #pragma omp parallel for
     for (int k=ks; k<ke; ++k)
         for (int j=d->jstart; j<d->jend; ++j)
             for (int i=d->istart; i<d->iend; ++i)
@@ -811,6 +827,7 @@ void mo_advec_2i5_u(const mo_dims* d, double* ut, const double* u, const double*
                               hflux(i2(v[ijk-ii1+jj1], v[ijk+jj1]), i2(v[ijk-ii1], v[ijk]), u, ijk, jj1, dyi));
             }
 
+    #pragma omp parallel for
     for (int k=ks; k<ke; ++k)
     {
         const int lvl = level_class(k, ks, ke);
@@ -832,6 +849,7 @@ void mo_advec_2i5_v(const mo_dims* d, double* vt, const double* u, const double*
     const double dxi = 1./d->dx, dyi = 1./d->dy;
     const int ks = d->kstart, ke = d->kend;
 
+    #pragma omp parallel for
     for (int k=ks; k<ke; ++k)
         for (int j=d->jstart; j<d->jend; ++j)
             for (int i=d->istart; i<d->iend; ++i)
@@ -841,6 +859,7 @@ void mo_advec_2i5_v(const mo_dims* d, double* vt, const double* u, const double*
                               hflux(i2(v[ijk], v[ijk+jj1]), i2(v[ijk-jj1], v[ijk]), v, ijk, jj1, dyi));
             }
 
+    #pragma omp parallel for
     for (int k=ks; k<ke; ++k)
     {
         const int lvl = level_class(k, ks, ke);
@@ -862,6 +881,7 @@ void mo_advec_2i5_w(const mo_dims* d, double* wt, const double* u, const double*
     const double dxi = 1./d->dx, dyi = 1./d->dy;
     const int ks = d->kstart, ke = d->kend;
 
+    #pragma omp parallel for
     for (int k=ks+1; k<ke; ++k)
         for (int j=d->jstart; j<d->jend; ++j)
             for (int i=d->istart; i<d->iend; ++i)
@@ -871,6 +891,7 @@ void mo_advec_2i5_w(const mo_dims* d, double* wt, const double* u, const double*
                               hflux(i2(v[ijk+jj1-kk1], v[ijk+jj1]), i2(v[ijk-kk1], v[ijk]), w, ijk, jj1, dyi));
             }
 
+    #pragma omp parallel for
     for (int k=ks+1; k<ke; ++k)
     {
         for (int j=d->jstart; j<d->jend; ++j)
@@ -918,6 +939,7 @@ void mo_advec_2i5_s(const mo_dims* d, double* st, const double* s, const double*
     const double dxi = 1./d->dx, dyi = 1./d->dy;
     const int ks = d->kstart, ke = d->kend;
 
+    #pragma omp parallel for
     for (int k=ks; k<ke; ++k)
         for (int j=d->jstart; j<d->jend; ++j)
             for (int i=d->istart; i<d->iend; ++i)
@@ -927,6 +949,7 @@ void mo_advec_2i5_s(const mo_dims* d, double* st, const double* s, const double*
                               hflux(v[ijk+jj1], v[ijk], s, ijk, jj1, dyi));
             }
 
+    #pragma omp parallel for
     for (int k=ks; k<ke; ++k)
     {
         const int lvl = level_class(k, ks, ke);
@@ -947,6 +970,7 @@ double mo_advec_2_cfl(const mo_dims* d, const double* u, const double* v, const 
     const int ii = 1, jj = d->icells, kk = d->ijcells;
     const double dxi = 1./d->dx, dyi = 1./d->dy;
     double cfl = 0;
+    #pragma omp parallel for reduction(max:cfl)
     for (int k=d->kstart; k<d->kend; ++k)
         for (int j=d->jstart; j<d->jend; ++j)
             for (int i=d->istart; i<d->iend; ++i)
@@ -964,6 +988,7 @@ void mo_advec_2_u(const mo_dims* d, double* ut, const double* u, const double* v
     /* src/advec_2.cxx:82-108 */
     const int ii = 1, jj = d->icells, kk = d->ijcells;
     const double dxi = 1./d->dx, dyi = 1./d->dy;
+    #pragma omp parallel for
     for (int k=d->kstart; k<d->kend; ++k)
         for (int j=d->jstart; j<d->jend; ++j)
             for (int i=d->istart; i<d->iend; ++i)
@@ -985,6 +1010,7 @@ void mo_advec_2_v(const mo_dims* d, double* vt, const double* u, const double* v
     /* src/advec_2.cxx:110-136 */
     const int ii = 1, jj = d->icells, kk = d->ijcells;
     const double dxi = 1./d->dx, dyi = 1./d->dy;
+    #pragma omp parallel for
     for (int k=d->kstart; k<d->kend; ++k)
         for (int j=d->jstart; j<d->jend; ++j)
             for (int i=d->istart; i<d->iend; ++i)
@@ -1006,6 +1032,7 @@ void mo_advec_2_w(const mo_dims* d, double* wt, const double* u, const double* v
     /* src/advec_2.cxx:138-164 */
     const int ii = 1, jj = d->icells, kk = d->ijcells;
     const double dxi = 1./d->dx, dyi = 1./d->dy;
+    #pragma omp parallel for
     for (int k=d->kstart+1; k<d->kend; ++k)
         for (int j=d->jstart; j<d->jend; ++j)
             for (int i=d->istart; i<d->iend; ++i)
@@ -1028,6 +1055,7 @@ void mo_advec_2_s(const mo_dims* d, double* st, const double* s, const double* u
     /* src/advec_2.cxx:166-202 */
     const int ii = 1, jj = d->icells, kk = d->ijcells;
     const double dxi = 1./d->dx, dyi = 1./d->dy;
+    #pragma omp parallel for
     for (int k=d->kstart; k<d->kend; ++k)
         for (int j=d->jstart; j<d->jend; ++j)
             for (int i=d->istart; i<d->iend; ++i)
@@ -1050,6 +1078,7 @@ void mo_diff_2_c(const mo_dims* d, double* at, const double* a, double visc,
     /* src/diff_2.cxx:39-61 */
     const int ii = 1, jj = d->icells, kk = d->ijcells;
     const double dxidxi = 1/(d->dx*d->dx), dyidyi = 1/(d->dy*d->dy);
+    #pragma omp parallel for
     for (int k=d->kstart; k<d->kend; ++k)
         for (int j=d->jstart; j<d->jend; ++j)
             for (int i=d->istart; i<d->iend; ++i)
@@ -1071,6 +1100,7 @@ void mo_diff_2_w(const mo_dims* d, double* wt, const double* w, double visc,
     /* src/diff_2.cxx:63-85 */
     const int ii = 1, jj = d->icells, kk = d->ijcells;
     const double dxidxi = 1/(d->dx*d->dx), dyidyi = 1/(d->dy*d->dy);
+    #pragma omp parallel for
     for (int k=d->kstart+1; k<d->kend; ++k)
         for (int j=d->jstart; j<d->jend; ++j)
             for (int i=d->istart; i<d->iend; ++i)
@@ -1091,6 +1121,7 @@ void mo_buoyancy_tend_2nd(const mo_dims* d, double* wt, const double* th, const 
 {
     /* src/thermo_dry.cxx:163-180 */
     const int jj = d->icells, kk = d->ijcells;
+    #pragma omp parallel for
     for (int k=d->kstart+1; k<d->kend; ++k)
         for (int j=d->jstart; j<d->jend; ++j)
             for (int i=d->istart; i<d->iend; ++i)
@@ -1107,6 +1138,7 @@ void mo_rk3(const mo_dims* d, double* a, double* at, int substep, double dt)
     static const double cA[] = {0., -5./9., -153./128.};
     static const double cB[] = {1./3., 15./16., 8./15.};
     const int jj = d->icells, kk = d->ijcells;
+    #pragma omp parallel for
     for (int k=d->kstart; k<d->kend; ++k)
         for (int j=d->jstart; j<d->jend; ++j)
             for (int i=d->istart; i<d->iend; ++i)
@@ -1122,6 +1154,7 @@ void mo_rk3(const mo_dims* d, double* a, double* at, int substep, double dt)
     }
     else
     {
+        #pragma omp parallel for
         for (int k=d->kstart; k<d->kend; ++k)
             for (int j=d->jstart; j<d->jend; ++j)
                 for (int i=d->istart; i<d->iend; ++i)

@@ -75,24 +75,19 @@ static cplx* roots(int n)
     return W;
 }
 
-void mo_fft_r2hc(int n, const double* in, double* out)
+/* R2HC / HC2R with caller-provided roots W (from roots(n)) and scratch a, y
+   of n complex values each. */
+static void r2hc_w(int n, const double* in, double* out, const cplx* W, cplx* a, cplx* y)
 {
-    cplx* a = (cplx*)malloc(sizeof(cplx)*n);
-    cplx* y = (cplx*)malloc(sizeof(cplx)*n);
-    cplx* W = roots(n);
     for (int j=0; j<n; ++j) { a[j].re = in[j]; a[j].im = 0.; }
     fft_rec(y, a, n, 1, W, 1, n);
     for (int k=0; k<=n/2; ++k) out[k] = y[k].re;
     for (int k=1; k<(n+1)/2; ++k) out[n-k] = y[k].im;
-    free(a); free(y); free(W);
 }
 
-void mo_fft_hc2r(int n, const double* in, double* out)
+static void hc2r_w(int n, const double* in, double* out, const cplx* W, cplx* a, cplx* y)
 {
     /* x_j = sum_k Y_k exp(+2 pi i j k / n): conjugate input, forward DFT, real part. */
-    cplx* a = (cplx*)malloc(sizeof(cplx)*n);
-    cplx* y = (cplx*)malloc(sizeof(cplx)*n);
-    cplx* W = roots(n);
     a[0].re = in[0]; a[0].im = 0.;
     for (int k=1; k<(n+1)/2; ++k)
     {
@@ -102,6 +97,23 @@ void mo_fft_hc2r(int n, const double* in, double* out)
     if (n % 2 == 0) { a[n/2].re = in[n/2]; a[n/2].im = 0.; }
     fft_rec(y, a, n, 1, W, 1, n);
     for (int j=0; j<n; ++j) out[j] = y[j].re;
+}
+
+void mo_fft_r2hc(int n, const double* in, double* out)
+{
+    cplx* a = (cplx*)malloc(sizeof(cplx)*n);
+    cplx* y = (cplx*)malloc(sizeof(cplx)*n);
+    cplx* W = roots(n);
+    r2hc_w(n, in, out, W, a, y);
+    free(a); free(y); free(W);
+}
+
+void mo_fft_hc2r(int n, const double* in, double* out)
+{
+    cplx* a = (cplx*)malloc(sizeof(cplx)*n);
+    cplx* y = (cplx*)malloc(sizeof(cplx)*n);
+    cplx* W = roots(n);
+    hc2r_w(n, in, out, W, a, y);
     free(a); free(y); free(W);
 }
 
@@ -142,6 +154,7 @@ void mo_pres_2_input(const mo_dims* d, double* p, const double* u, const double*
     mo_boundary_cyclic(d, ut, MO_EDGE_EW);
     mo_boundary_cyclic(d, vt, MO_EDGE_NS);
 
+    #pragma omp parallel for
     for (int k=0; k<d->kmax; ++k)
         for (int j=0; j<d->jmax; ++j)
             for (int i=0; i<d->imax; ++i)
@@ -158,56 +171,73 @@ void mo_pres_2_input(const mo_dims* d, double* p, const double* u, const double*
 void mo_fft_forward(const mo_dims* d, double* data)
 {
     /* src/fft.cxx:338-394 (serial): per k-slice, R2HC along x for jmax rows,
-       then R2HC along y for iblock columns (stride iblock). */
+       then R2HC along y for iblock columns (stride iblock).  Slices are
+       independent; OpenMP threads take whole k-slices. */
     const int itot = d->itot, jtot = d->jtot, jmax = d->jmax, iblock = d->itot;
-    double* in  = (double*)malloc(sizeof(double)*(itot > jtot ? itot : jtot));
-    double* out = (double*)malloc(sizeof(double)*(itot > jtot ? itot : jtot));
-
-    int kk = itot*jmax;
-    for (int k=0; k<d->ktot; ++k)
-        for (int j=0; j<jmax; ++j)
-        {
-            double* row = data + j*itot + k*kk;
-            mo_fft_r2hc(itot, row, out);
-            memcpy(row, out, sizeof(double)*itot);
-        }
-
-    kk = iblock*jtot;
-    for (int k=0; k<d->ktot; ++k)
-        for (int i=0; i<iblock; ++i)
-        {
-            for (int j=0; j<jtot; ++j) in[j] = data[i + j*iblock + k*kk];
-            mo_fft_r2hc(jtot, in, out);
-            for (int j=0; j<jtot; ++j) data[i + j*iblock + k*kk] = out[j];
-        }
-    free(in); free(out);
+    const int nmax = itot > jtot ? itot : jtot;
+    #pragma omp parallel
+    {
+        double* in  = (double*)malloc(sizeof(double)*nmax);
+        double* out = (double*)malloc(sizeof(double)*nmax);
+        cplx* a = (cplx*)malloc(sizeof(cplx)*nmax);
+        cplx* y = (cplx*)malloc(sizeof(cplx)*nmax);
+        cplx* Wx = roots(itot);
+        cplx* Wy = roots(jtot);
+        const int kkx = itot*jmax;
+        #pragma omp for
+        for (int k=0; k<d->ktot; ++k)
+            for (int j=0; j<jmax; ++j)
+            {
+                double* row = data + j*itot + k*kkx;
+                r2hc_w(itot, row, out, Wx, a, y);
+                memcpy(row, out, sizeof(double)*itot);
+            }
+        const int kky = iblock*jtot;
+        #pragma omp for
+        for (int k=0; k<d->ktot; ++k)
+            for (int i=0; i<iblock; ++i)
+            {
+                for (int j=0; j<jtot; ++j) in[j] = data[i + j*iblock + k*kky];
+                r2hc_w(jtot, in, out, Wy, a, y);
+                for (int j=0; j<jtot; ++j) data[i + j*iblock + k*kky] = out[j];
+            }
+        free(in); free(out); free(a); free(y); free(Wx); free(Wy);
+    }
 }
 
 void mo_fft_backward(const mo_dims* d, double* data, double* tmp1)
 {
     /* src/fft.cxx:396-452 (serial): HC2R along y (/jtot) into data, then
-       HC2R along x (/itot) into tmp1. */
+       HC2R along x (/itot) into tmp1.  OpenMP threads take whole k-slices. */
     const int itot = d->itot, jtot = d->jtot, jmax = d->jmax, iblock = d->itot;
-    double* in  = (double*)malloc(sizeof(double)*(itot > jtot ? itot : jtot));
-    double* out = (double*)malloc(sizeof(double)*(itot > jtot ? itot : jtot));
-
-    int kk = iblock*jtot;
-    for (int k=0; k<d->ktot; ++k)
-        for (int i=0; i<iblock; ++i)
-        {
-            for (int j=0; j<jtot; ++j) in[j] = data[i + j*iblock + k*kk];
-            mo_fft_hc2r(jtot, in, out);
-            for (int j=0; j<jtot; ++j) data[i + j*iblock + k*kk] = out[j] / jtot;
-        }
-
-    kk = itot*jmax;
-    for (int k=0; k<d->ktot; ++k)
-        for (int j=0; j<jmax; ++j)
-        {
-            mo_fft_hc2r(itot, data + j*itot + k*kk, out);
-            for (int i=0; i<itot; ++i) tmp1[i + j*itot + k*kk] = out[i] / itot;
-        }
-    free(in); free(out);
+    const int nmax = itot > jtot ? itot : jtot;
+    #pragma omp parallel
+    {
+        double* in  = (double*)malloc(sizeof(double)*nmax);
+        double* out = (double*)malloc(sizeof(double)*nmax);
+        cplx* a = (cplx*)malloc(sizeof(cplx)*nmax);
+        cplx* y = (cplx*)malloc(sizeof(cplx)*nmax);
+        cplx* Wx = roots(itot);
+        cplx* Wy = roots(jtot);
+        const int kky = iblock*jtot;
+        #pragma omp for
+        for (int k=0; k<d->ktot; ++k)
+            for (int i=0; i<iblock; ++i)
+            {
+                for (int j=0; j<jtot; ++j) in[j] = data[i + j*iblock + k*kky];
+                hc2r_w(jtot, in, out, Wy, a, y);
+                for (int j=0; j<jtot; ++j) data[i + j*iblock + k*kky] = out[j] / jtot;
+            }
+        const int kkx = itot*jmax;
+        #pragma omp for
+        for (int k=0; k<d->ktot; ++k)
+            for (int j=0; j<jmax; ++j)
+            {
+                hc2r_w(itot, data + j*itot + k*kkx, out, Wx, a, y);
+                for (int i=0; i<itot; ++i) tmp1[i + j*itot + k*kkx] = out[i] / itot;
+            }
+        free(in); free(out); free(a); free(y); free(Wx); free(Wy);
+    }
 }
 
 void mo_tdma(double* a, double* b, double* c, double* p, double* work2d, double* work3d,
@@ -215,43 +245,56 @@ void mo_tdma(double* a, double* b, double* c, double* p, double* work2d, double*
 {
     /* src/pres_2.cxx:200-263 (Thomas algorithm, Numerical Recipes form). */
     const int jj = iblock, kk = iblock*jblock;
-    for (int j=0; j<jblock; ++j)
-        for (int i=0; i<iblock; ++i)
-            work2d[i + j*jj] = b[i + j*jj];
-    for (int j=0; j<jblock; ++j)
-        for (int i=0; i<iblock; ++i)
-            p[i + j*jj] /= work2d[i + j*jj];
-
-    for (int k=1; k<kmax; ++k)
+    /* Columns are independent: OpenMP threads split the j rows of every
+       level; the k recurrences stay sequential. */
+    #pragma omp parallel
     {
+        #pragma omp for
         for (int j=0; j<jblock; ++j)
             for (int i=0; i<iblock; ++i)
-            {
-                const int ij = i + j*jj, ijk = ij + k*kk;
-                work3d[ijk] = c[k-1] / work2d[ij];
-            }
+                work2d[i + j*jj] = b[i + j*jj];
+        #pragma omp for
         for (int j=0; j<jblock; ++j)
             for (int i=0; i<iblock; ++i)
-            {
-                const int ij = i + j*jj, ijk = ij + k*kk;
-                work2d[ij] = b[ijk] - a[k]*work3d[ijk];
-            }
-        for (int j=0; j<jblock; ++j)
-            for (int i=0; i<iblock; ++i)
-            {
-                const int ij = i + j*jj, ijk = ij + k*kk;
-                p[ijk] -= a[k]*p[ijk-kk];
-                p[ijk] /= work2d[ij];
-            }
-    }
+                p[i + j*jj] /= work2d[i + j*jj];
 
-    for (int k=kmax-2; k>=0; --k)
-        for (int j=0; j<jblock; ++j)
-            for (int i=0; i<iblock; ++i)
-            {
-                const int ijk = i + j*jj + k*kk;
-                p[ijk] -= work3d[ijk+kk]*p[ijk+kk];
-            }
+        for (int k=1; k<kmax; ++k)
+        {
+            #pragma omp for
+            for (int j=0; j<jblock; ++j)
+                for (int i=0; i<iblock; ++i)
+                {
+                    const int ij = i + j*jj, ijk = ij + k*kk;
+                    work3d[ijk] = c[k-1] / work2d[ij];
+                }
+            #pragma omp for
+            for (int j=0; j<jblock; ++j)
+                for (int i=0; i<iblock; ++i)
+                {
+                    const int ij = i + j*jj, ijk = ij + k*kk;
+                    work2d[ij] = b[ijk] - a[k]*work3d[ijk];
+                }
+            #pragma omp for
+            for (int j=0; j<jblock; ++j)
+                for (int i=0; i<iblock; ++i)
+                {
+                    const int ij = i + j*jj, ijk = ij + k*kk;
+                    p[ijk] -= a[k]*p[ijk-kk];
+                    p[ijk] /= work2d[ij];
+                }
+        }
+
+        for (int k=kmax-2; k>=0; --k)
+        {
+            #pragma omp for
+            for (int j=0; j<jblock; ++j)
+                for (int i=0; i<iblock; ++i)
+                {
+                    const int ijk = i + j*jj + k*kk;
+                    p[ijk] -= work3d[ijk+kk]*p[ijk+kk];
+                }
+        }
+    }
 }
 
 void mo_pres_2_solve(const mo_dims* d, double* p, double* work3d, double* b,
@@ -266,6 +309,7 @@ void mo_pres_2_solve(const mo_dims* d, double* p, double* work3d, double* b,
     mo_fft_forward(d, p);
 
     int jj = iblock, kk = iblock*jblock;
+    #pragma omp parallel for
     for (int k=0; k<kmax; ++k)
         for (int j=0; j<jblock; ++j)
             for (int i=0; i<iblock; ++i)
@@ -298,6 +342,7 @@ void mo_pres_2_solve(const mo_dims* d, double* p, double* work3d, double* b,
 
     jj = imax; kk = imax*jmax;
     const int jjp = d->icells, kkp = d->ijcells;
+    #pragma omp parallel for
     for (int k=0; k<kmax; ++k)
         for (int j=0; j<jmax; ++j)
             for (int i=0; i<imax; ++i)
@@ -319,6 +364,7 @@ void mo_pres_2_output(const mo_dims* d, double* ut, double* vt, double* wt,
     /* src/pres_2.cxx:364-387 */
     const int ii = 1, jj = d->icells, kk = d->ijcells;
     const double dxi = 1./d->dx, dyi = 1./d->dy;
+    #pragma omp parallel for
     for (int k=d->kstart; k<d->kend; ++k)
         for (int j=d->jstart; j<d->jend; ++j)
             for (int i=d->istart; i<d->iend; ++i)
@@ -337,6 +383,7 @@ double mo_pres_2_divergence(const mo_dims* d, const double* u, const double* v, 
     const int ii = 1, jj = d->icells, kk = d->ijcells;
     const double dxi = 1./d->dx, dyi = 1./d->dy;
     double divmax = 0.;
+    #pragma omp parallel for reduction(max:divmax)
     for (int k=d->kstart; k<d->kend; ++k)
         for (int j=d->jstart; j<d->jend; ++j)
             for (int i=d->istart; i<d->iend; ++i)

@@ -273,6 +273,18 @@ def test_rand_is_glibc_and_randomize_order():
     assert not data[kend:].any() and not data[:d.kstart].any()
 
 
+def test_add_mean_prof_adds_profile_on_interior_levels():
+    d = po.dims(4, 3, 5, 3, 3, 1, 1.0, 1.0, 1.0)
+    data = po.field(d, 0.25)
+    prof = np.arange(5, dtype=np.float64) * 2.0 + 300.0
+    po.lib().mo_add_mean_prof(C.byref(d), ptr(data), ptr(prof), 300.0)
+    inner = data[d.kstart:d.kend, d.jstart:d.jend, d.istart:d.iend]
+    assert np.array_equal(inner, np.broadcast_to((0.25 + prof - 300.0)[:, None, None], inner.shape))
+    ghost = data.copy()
+    ghost[d.kstart:d.kend, d.jstart:d.jend, d.istart:d.iend] = 0.25
+    assert np.all(ghost == 0.25)
+
+
 # -------------------------------------------------------------- boundary
 @pytest.mark.parametrize("edge", [0, 1, 2])
 @pytest.mark.parametrize("jtot", [4, 1])
