@@ -122,6 +122,11 @@ int  mhh_boundary_set_ghost_cells(mhh_ctx* ctx, const mhh_bc_desc* bc);
 /* ---- advection ---- */
 /* Advec_2i5::exec (src/advec_2i5.cxx:995-1063). */
 int  mhh_advec_2i5_exec(mhh_ctx* ctx);
+/* Advec_2i5 [advec] fluxlimit_list (src/advec_2i5.cxx:39-44, 935-945):
+ * limited[n] != 0 advects bound scalar n with the Koren flux limiter
+ * (include/advec_monotonic.h:97-201) in mhh_advec_2i5_exec.  The reference
+ * raises kgc to 2 when the list is not empty. */
+int  mhh_advec_2i5_set_fluxlimit(mhh_ctx* ctx, const int* limited, int nscalars);
 /* Advec_2i5::get_cfl (src/advec_2i5.cxx:954-971): max CFL times dt. */
 int  mhh_advec_2i5_cfl(mhh_ctx* ctx, double dt, double* cfl);
 /* Advec_2::exec / get_cfl (src/advec_2.cxx:312-350, 280-296). */

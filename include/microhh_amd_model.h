@@ -44,6 +44,7 @@ typedef struct
     double dt, dtmax, endtime, savetime;
     int precision;      /* MHH_FP64 or MHH_FP32 */
     int fused;          /* 1: fused kernels where the configuration allows (same results) */
+    int fluxlimit_th;   /* 1: [advec] fluxlimit_list = th (Koren limiter, kgc = 2) */
 } mhh_case;
 
 /* z_interior, u_prof, v_prof, th_prof: ktot host values each (the "init"

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "libmicrohh_amd.so")
 
 MHH_MAX_SCALARS = 8
 MHH_FP64, MHH_FP32 = 0, 1
+MHH_OK, MHH_EINVAL, MHH_EHIP, MHH_ESTATE, MHH_ENOMEM = 0, 1, 2, 3, 4
 BC = {"dirichlet": 0, "neumann": 1, "flux": 2, "ustar": 3, "off": 4, "noslip": 0, "freeslip": 1}
 EDGE_EW, EDGE_NS, EDGE_BOTH = 0, 1, 2
 
@@ -64,6 +65,7 @@ SIGNATURES = {
     "mhh_boundary_set_prognostic_cyclic_bcs": (C.c_int, [VP]),
     "mhh_boundary_set_ghost_cells": (C.c_int, [VP, C.POINTER(BcDesc)]),
     "mhh_advec_2i5_exec": (C.c_int, [VP]),
+    "mhh_advec_2i5_set_fluxlimit": (C.c_int, [VP, C.POINTER(C.c_int), C.c_int]),
     "mhh_advec_2i5_cfl": (C.c_int, [VP, C.c_double, DP]),
     "mhh_advec_2_exec": (C.c_int, [VP]),
     "mhh_advec_2_cfl": (C.c_int, [VP, C.c_double, DP]),

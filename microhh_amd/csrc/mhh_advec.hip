@@ -103,6 +103,39 @@ namespace
         a.at[ijk] = t;
     }
 
+    // Flux-limited scalar advection (include/advec_monotonic.h:97-201): Koren
+    // fluxes on all faces, the wall variants at the faces next to the walls
+    // and no flux through the walls.
+    template<typename TF>
+    __global__ void __launch_bounds__(BX*BY) advec_2i5_s_lim_g(Adv_args<TF> a, Gd g)
+    {
+        int i, j, k;
+        if (!point(g, g.kstart, i, j, k)) return;
+        const int ii = 1, jj = g.icells, kk = g.ijcells;
+        const int ijk = i + j*jj + k*kk;
+        const TF* __restrict__ s = a.s;
+        const TF fx = flux_lim(a.u[ijk+ii], s[ijk-ii], s[ijk], s[ijk+ii], s[ijk+2*ii])
+                    - flux_lim(a.u[ijk], s[ijk-2*ii], s[ijk-ii], s[ijk], s[ijk+ii]);
+        const TF fy = flux_lim(a.v[ijk+jj], s[ijk-jj], s[ijk], s[ijk+jj], s[ijk+2*jj])
+                    - flux_lim(a.v[ijk], s[ijk-2*jj], s[ijk-jj], s[ijk], s[ijk+jj]);
+        const TF* __restrict__ w = a.w;
+        TF fz;
+        if (k == g.kstart)
+            fz = a.rhoh[k+1] * flux_lim_bot(w[ijk+kk], s[ijk], s[ijk+kk], s[ijk+2*kk]);
+        else if (k == g.kstart+1)
+            fz = a.rhoh[k+1] * flux_lim(w[ijk+kk], s[ijk-kk], s[ijk], s[ijk+kk], s[ijk+2*kk])
+               - a.rhoh[k  ] * flux_lim_bot(w[ijk], s[ijk-kk], s[ijk], s[ijk+kk]);
+        else if (k == g.kend-2)
+            fz = a.rhoh[k+1] * flux_lim_top(w[ijk+kk], s[ijk-kk], s[ijk], s[ijk+kk])
+               - a.rhoh[k  ] * flux_lim(w[ijk], s[ijk-2*kk], s[ijk-kk], s[ijk], s[ijk+kk]);
+        else if (k == g.kend-1)
+            fz = - a.rhoh[k] * flux_lim_top(w[ijk], s[ijk-2*kk], s[ijk-kk], s[ijk]);
+        else
+            fz = a.rhoh[k+1] * flux_lim(w[ijk+kk], s[ijk-kk], s[ijk], s[ijk+kk], s[ijk+2*kk])
+               - a.rhoh[k  ] * flux_lim(w[ijk], s[ijk-2*kk], s[ijk-kk], s[ijk], s[ijk+kk]);
+        a.at[ijk] += - fx * a.dxi - fy * a.dyi - fz / a.rho[k] * a.dz[k];
+    }
+
     // Advec_2 (src/advec_2.cxx:82-202).
     template<typename TF>
     __global__ void __launch_bounds__(BX*BY) advec_2_u_g(Adv_args<TF> a, Gd g)
@@ -248,7 +281,12 @@ namespace
         { mhh::Timer _t(c, "advec_2i5_v"); advec_2i5_v_g<TF><<<grid3(g, g.kstart, g.kend), blk, 0, c->stream>>>(args<TF>(c, p, c->f.vt, nullptr, false), g); }
         { mhh::Timer _t(c, "advec_2i5_w"); advec_2i5_w_g<TF><<<grid3(g, g.kstart+1, g.kend), blk, 0, c->stream>>>(args<TF>(c, p, c->f.wt, nullptr, true), g); }
         for (int n = 0; n < c->f.nscalars; ++n)
-            { mhh::Timer _t(c, "advec_2i5_s"); advec_2i5_s_g<TF><<<grid3(g, g.kstart, g.kend), blk, 0, c->stream>>>(args<TF>(c, p, c->f.st[n], c->f.s[n], false), g); }
+        {
+            if (c->fluxlimit[n])
+                { mhh::Timer _t(c, "advec_2i5_s_lim"); advec_2i5_s_lim_g<TF><<<grid3(g, g.kstart, g.kend), blk, 0, c->stream>>>(args<TF>(c, p, c->f.st[n], c->f.s[n], false), g); }
+            else
+                { mhh::Timer _t(c, "advec_2i5_s"); advec_2i5_s_g<TF><<<grid3(g, g.kstart, g.kend), blk, 0, c->stream>>>(args<TF>(c, p, c->f.st[n], c->f.s[n], false), g); }
+        }
         return check_launch("advec_2i5");
     }
 
@@ -291,6 +329,16 @@ int mhh_advec_2i5_exec(mhh_ctx* c)
     if (c->gd.igc < 3 || c->gd.jgc < 3)
         return mhh::fail(MHH_EINVAL, "advec_2i5 needs igc = jgc >= 3 (src/advec_2i5.cxx:42-45)");
     return c->precision == MHH_FP64 ? exec_2i5<double>(c, c->pd) : exec_2i5<float>(c, c->pf);
+}
+
+int mhh_advec_2i5_set_fluxlimit(mhh_ctx* c, const int* limited, int nscalars)
+{
+    if (!c) return mhh::fail(MHH_EINVAL, "null context");
+    if (nscalars < 0 || nscalars > MHH_MAX_SCALARS || (nscalars > 0 && !limited))
+        return mhh::fail(MHH_EINVAL, "mhh_advec_2i5_set_fluxlimit: bad scalar list");
+    for (int n = 0; n < MHH_MAX_SCALARS; ++n)
+        c->fluxlimit[n] = n < nscalars ? (limited[n] != 0) : 0;
+    return 0;
 }
 
 int mhh_advec_2_exec(mhh_ctx* c)

@@ -60,6 +60,8 @@ struct mhh_ctx
     bool have_base_state = false;
     mhh_field_set f{};
     bool bound = false;
+    // Scalars advected with the Koren flux limiter ([advec] fluxlimit_list).
+    int fluxlimit[MHH_MAX_SCALARS] = {};
     // Reduction scratch: per-block partial maxima and one result slot.
     double* red = nullptr;
     int red_capacity = 0;

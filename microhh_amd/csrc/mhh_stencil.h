@@ -1,6 +1,7 @@
 // mhh_stencil.h -- interpolation and flux building blocks of the 2nd-order grid
 // and of the 2i5 scheme (coefficients of include/finite_difference.h:37-156).
 #pragma once
+#include <limits>
 
 #include <hip/hip_runtime.h>
 
@@ -175,5 +176,41 @@ namespace mhh
                        - ( rm * fabs(wm) * i3ws(m2, m1, s0, p1) ) / rc * dzc;
             }
         }
+    }
+
+    // Koren (1993) limited face fluxes of the flux-limited scalar advection
+    // (include/advec_monotonic.h:29-94).  koren_phi takes the numerator and
+    // the upwind difference of the ratio 2r.
+    template<typename TF>
+    __device__ __forceinline__ TF koren_phi(TF num, TF dsm)
+    {
+        const TF eps = std::numeric_limits<TF>::epsilon();
+        const TF denom = copysign(TF(1), dsm) * fmax(fabs(dsm), eps);
+        const TF two_r = TF(2.) * num / denom;
+        return fmax(TF(0.), fmin(two_r, fmin(TF(1./3.)*(TF(1.) + two_r), TF(2.))));
+    }
+
+    template<typename TF>
+    __device__ __forceinline__ TF flux_lim(TF u, TF sm2, TF sm1, TF sp1, TF sp2)
+    {
+        if (u >= TF(0.))
+            return u*(sm1 + TF(0.5)*koren_phi(sp1 - sm1, sm1 - sm2)*(sm1 - sm2));
+        return u*(sp1 + TF(0.5)*koren_phi(sm1 - sp1, sp1 - sp2)*(sp1 - sp2));
+    }
+
+    template<typename TF>
+    __device__ __forceinline__ TF flux_lim_bot(TF u, TF sm1, TF sp1, TF sp2)
+    {
+        if (u >= TF(0.))
+            return u*sm1;
+        return u*(sp1 + TF(0.5)*koren_phi(sm1 - sp1, sp1 - sp2)*(sp1 - sp2));
+    }
+
+    template<typename TF>
+    __device__ __forceinline__ TF flux_lim_top(TF u, TF sm2, TF sm1, TF sp1)
+    {
+        if (u >= TF(0.))
+            return u*(sm1 + TF(0.5)*koren_phi(sp1 - sm1, sm1 - sm2)*(sm1 - sm2));
+        return u*sp1;
     }
 }

@@ -335,6 +335,8 @@ int mhh_tendency_2i5_smag2(mhh_ctx* c, const mhh_smag_params* par, int buoyancy)
     if (!par) return mhh::fail(MHH_EINVAL, "null smag params");
     if (!c->f.evisc) return mhh::fail(MHH_ESTATE, "evisc field is not bound");
     if (buoyancy && !c->have_base_state) return mhh::fail(MHH_ESTATE, "buoyancy needs mhh_set_base_state()");
+    if (c->fluxlimit[0])
+        return mhh::fail(MHH_ESTATE, "the fused tendency has no flux limiter: use mhh_advec_2i5_exec for limited scalars");
     return c->precision == MHH_FP64 ? tend<double>(c, c->pd, *par, buoyancy) : tend<float>(c, c->pf, *par, buoyancy);
 }
 

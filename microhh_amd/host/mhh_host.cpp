@@ -599,8 +599,8 @@ namespace mhh_host
                      const double* th_prof, void* stream) : cs(c)
     {
         grid = std::make_unique<Grid<TF>>(c.itot, c.jtot, c.ktot, c.xsize, c.ysize, c.zsize);
-        if (c.advec == 0)
-            grid->set_minimum_ghost_cells(3, 3, 1);   // src/advec_2i5.cxx:42-45
+        if (c.advec == 0)   // src/advec_2i5.cxx:42-45: kgc 2 with a flux-limited scalar
+            grid->set_minimum_ghost_cells(3, 3, (c.fluxlimit_th && c.thermo == 0) ? 2 : 1);
         grid->init();
         grid->create(z);
 
@@ -643,7 +643,14 @@ namespace mhh_host
         timeloop = std::make_unique<Timeloop<TF>>(ctx_, c);
         // The fused tendency kernel covers 2i5 + smag2 + dry thermo with the
         // default boundary: one pass instead of the three operator calls.
-        fused_tendency = c.fused && c.advec == 0 && c.diff == 0 && c.thermo == 0;
+        fused_tendency = c.fused && c.advec == 0 && c.diff == 0 && c.thermo == 0 && !c.fluxlimit_th;
+        if (c.advec == 0 && c.thermo == 0)
+        {
+            // Advec_2i5::create splits the scalars into limited and plain
+            // (src/advec_2i5.cxx:935-945); th is scalar 0.
+            const int limited[1] = {c.fluxlimit_th ? 1 : 0};
+            check(mhh_advec_2i5_set_fluxlimit(ctx_, limited, 1), "mhh_advec_2i5_set_fluxlimit");
+        }
     }
 
     template<typename TF>

@@ -30,7 +30,7 @@ class Case(C.Structure):
         ("rndamp_th", C.c_double), ("rndz", C.c_double), ("rndexp", C.c_double),
         ("adaptivestep", C.c_int),
         ("dt", C.c_double), ("dtmax", C.c_double), ("endtime", C.c_double), ("savetime", C.c_double),
-        ("precision", C.c_int), ("fused", C.c_int),
+        ("precision", C.c_int), ("fused", C.c_int), ("fluxlimit_th", C.c_int),
     ]
 
 
@@ -71,11 +71,12 @@ CASE_KEYS = [f[0] for f in Case._fields_]
 
 def dry_cbl_case(itot=512, jtot=512, ktot=256, xsize=25600.0, ysize=25600.0, zsize=3200.0,
                  precision=MHH_FP64, adaptive=True, dt=10.0, dtmax=60.0, rndamp_uvw=0.02, rndamp_th=0.2,
-                 rndz=None, rndexp=0.0, advec=0, diff=0, rndseed=2, fused=True):
+                 rndz=None, rndexp=0.0, advec=0, diff=0, rndseed=2, fused=True, fluxlimit=False):
     """The dry convective boundary layer of the hot path: advec_2i5 + smag2 +
     pres_2, RK3, doubly periodic, Boussinesq theta with 300 K to 1000 m and a
     0.006 K/m lapse rate above.  The surface is a fixed warm theta (Dirichlet,
-    301 K) because the "default" boundary switch has no surface layer."""
+    301 K) because the "default" boundary switch has no surface layer.
+    fluxlimit=True advects theta with the Koren flux limiter (two ghost levels)."""
     return dict(
         itot=itot, jtot=jtot, ktot=ktot, xsize=xsize, ysize=ysize, zsize=zsize,
         advec=advec, diff=diff, thermo=0, cflmax=1.2, dnmax=0.4, cs=0.23, tPr=1.0 / 3.0, swmason=1,
@@ -85,7 +86,7 @@ def dry_cbl_case(itot=512, jtot=512, ktot=256, xsize=25600.0, ysize=25600.0, zsi
         rndseed=rndseed, rndamp_u=rndamp_uvw, rndamp_v=rndamp_uvw, rndamp_w=rndamp_uvw, rndamp_th=rndamp_th,
         rndz=zsize if rndz is None else rndz, rndexp=rndexp,
         adaptivestep=1 if adaptive else 0, dt=dt, dtmax=dtmax, endtime=1.0e9, savetime=1.0e8,
-        precision=precision, fused=1 if fused else 0,
+        precision=precision, fused=1 if fused else 0, fluxlimit_th=1 if fluxlimit else 0,
     )
 
 

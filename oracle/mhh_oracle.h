@@ -120,6 +120,12 @@ void mo_advec_2i5_s(const mo_dims* d, double* st, const double* s, const double*
                     const double* v, const double* w, const double* dzi,
                     const double* rhoref, const double* rhorefh);
 
+/* Flux-limited scalar advection for the scalars in [advec] fluxlimit_list
+ * (include/advec_monotonic.h:29-201, called from src/advec_2i5.cxx:1047-1056). */
+void mo_advec_2i5_s_lim(const mo_dims* d, double* st, const double* s, const double* u,
+                        const double* v, const double* w, const double* dzi,
+                        const double* rhoref, const double* rhorefh);
+
 /* ---- advection 2 (src/advec_2.cxx:50-202) ---- */
 double mo_advec_2_cfl(const mo_dims* d, const double* u, const double* v, const double* w,
                       const double* dzi, double dt);
@@ -197,6 +203,7 @@ typedef struct
     int adaptivestep;
     double dt, dtmax, endtime, savetime;
     int nthreads;       /* OpenMP threads for the timed baseline (1 = serial reference semantics) */
+    int fluxlimit_th;   /* [advec] fluxlimit_list = th (src/advec_2i5.cxx:39-44) */
 } mo_case;
 
 mo_model* mo_model_create(const mo_case* c, const double* z_interior,

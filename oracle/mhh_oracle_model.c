@@ -67,7 +67,8 @@ mo_model* mo_model_create(const mo_case* c, const double* z_interior,
     /* advec_2i5 raises the horizontal ghost cells to 3 (src/advec_2i5.cxx:42-45);
        advec_2 and the 2nd-order grid need 1. */
     const int gc = (c->advec == 0) ? 3 : 1;
-    mo_dims_init(&m->d, c->itot, c->jtot, c->ktot, gc, gc, 1, c->xsize, c->ysize, c->zsize);
+    const int kgc = (c->advec == 0 && c->fluxlimit_th && c->thermo == 0) ? 2 : 1;
+    mo_dims_init(&m->d, c->itot, c->jtot, c->ktot, gc, gc, kgc, c->xsize, c->ysize, c->zsize);
     const mo_dims* d = &m->d;
     const size_t n3 = (size_t)d->ncells, n2 = (size_t)d->ijcells, n1 = (size_t)d->kcells;
 
@@ -292,7 +293,9 @@ int mo_model_substep(mo_model* m)
         mo_advec_2i5_u(d, m->ut, m->u, m->v, m->w, m->dzi, m->rhoref, m->rhorefh);
         mo_advec_2i5_v(d, m->vt, m->u, m->v, m->w, m->dzi, m->rhoref, m->rhorefh);
         mo_advec_2i5_w(d, m->wt, m->u, m->v, m->w, m->dzhi, m->rhoref, m->rhorefh);
-        if (c->thermo == 0)
+        if (c->thermo == 0 && c->fluxlimit_th)
+            mo_advec_2i5_s_lim(d, m->tht, m->th, m->u, m->v, m->w, m->dzi, m->rhoref, m->rhorefh);
+        else if (c->thermo == 0)
             mo_advec_2i5_s(d, m->tht, m->th, m->u, m->v, m->w, m->dzi, m->rhoref, m->rhorefh);
     }
     else

@@ -5,6 +5,7 @@
  * follows one reference loop nest; the expression trees keep the reference
  * order of operations so that the FP64 results agree to rounding.
  */
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -963,6 +964,80 @@ void mo_advec_2i5_s(const mo_dims* d, double* st, const double* s, const double*
 }
 
 /* -------------------------------------------------------------- advec_2 */
+/* Koren (1993) limited face flux, include/advec_monotonic.h:29-94: the
+   general form, and the bottom/top variants that take the plain upwind value
+   for flow out of / into the wall-adjacent face. */
+static inline double koren_phi(double num, double dsm)
+{
+    const double eps = DBL_EPSILON;
+    const double denom = copysign(1., dsm) * fmax2(fabs(dsm), eps);
+    const double two_r = 2. * num / denom;
+    return fmax2(0., fmin2(two_r, fmin2((1./3.)*(1. + two_r), 2.)));
+}
+static inline double flux_lim(double u, double sm2, double sm1, double sp1, double sp2)
+{
+    if (u >= 0.)
+    {
+        const double phi = koren_phi(sp1 - sm1, sm1 - sm2);
+        return u*(sm1 + 0.5*phi*(sm1 - sm2));
+    }
+    const double phi = koren_phi(sm1 - sp1, sp1 - sp2);
+    return u*(sp1 + 0.5*phi*(sp1 - sp2));
+}
+static inline double flux_lim_bot(double u, double sm2, double sm1, double sp1, double sp2)
+{
+    if (u >= 0.)
+        return u*sm1;
+    const double phi = koren_phi(sm1 - sp1, sp1 - sp2);
+    return u*(sp1 + 0.5*phi*(sp1 - sp2));
+}
+static inline double flux_lim_top(double u, double sm2, double sm1, double sp1, double sp2)
+{
+    if (u >= 0.)
+    {
+        const double phi = koren_phi(sp1 - sm1, sm1 - sm2);
+        return u*(sm1 + 0.5*phi*(sm1 - sm2));
+    }
+    return u*sp1;
+}
+
+void mo_advec_2i5_s_lim(const mo_dims* d, double* st, const double* s, const double* u,
+                        const double* v, const double* w, const double* dzi,
+                        const double* rhoref, const double* rhorefh)
+{
+    /* include/advec_monotonic.h:97-201: interior levels, then kstart,
+       kstart+1, kend-2, kend-1 with the wall variants of the vertical flux. */
+    const int ii1 = 1, ii2 = 2, jj1 = d->icells, jj2 = 2*d->icells, kk1 = d->ijcells, kk2 = 2*d->ijcells;
+    const double dxi = 1./d->dx, dyi = 1./d->dy;
+    const int ks = d->kstart, ke = d->kend;
+    #pragma omp parallel for
+    for (int k=ks; k<ke; ++k)
+        for (int j=d->jstart; j<d->jend; ++j)
+            for (int i=d->istart; i<d->iend; ++i)
+            {
+                const int ijk = i + j*jj1 + k*kk1;
+                const double fx = ( flux_lim(u[ijk+ii1], s[ijk-ii1], s[ijk    ], s[ijk+ii1], s[ijk+ii2])
+                                  - flux_lim(u[ijk    ], s[ijk-ii2], s[ijk-ii1], s[ijk    ], s[ijk+ii1]) );
+                const double fy = ( flux_lim(v[ijk+jj1], s[ijk-jj1], s[ijk    ], s[ijk+jj1], s[ijk+jj2])
+                                  - flux_lim(v[ijk    ], s[ijk-jj2], s[ijk-jj1], s[ijk    ], s[ijk+jj1]) );
+                double fz;
+                if (k == ks)
+                    fz = ( rhorefh[k+1] * flux_lim_bot(w[ijk+kk1], s[ijk-kk1], s[ijk], s[ijk+kk1], s[ijk+kk2]) );
+                else if (k == ks+1)
+                    fz = ( rhorefh[k+1] * flux_lim    (w[ijk+kk1], s[ijk-kk1], s[ijk    ], s[ijk+kk1], s[ijk+kk2])
+                         - rhorefh[k  ] * flux_lim_bot(w[ijk    ], s[ijk-kk2], s[ijk-kk1], s[ijk    ], s[ijk+kk1]) );
+                else if (k == ke-2)
+                    fz = ( rhorefh[k+1] * flux_lim_top(w[ijk+kk1], s[ijk-kk1], s[ijk    ], s[ijk+kk1], s[ijk+kk2])
+                         - rhorefh[k  ] * flux_lim    (w[ijk    ], s[ijk-kk2], s[ijk-kk1], s[ijk    ], s[ijk+kk1]) );
+                else if (k == ke-1)
+                    fz = ( - rhorefh[k] * flux_lim_top(w[ijk], s[ijk-kk2], s[ijk-kk1], s[ijk], s[ijk+kk1]) );
+                else
+                    fz = ( rhorefh[k+1] * flux_lim(w[ijk+kk1], s[ijk-kk1], s[ijk    ], s[ijk+kk1], s[ijk+kk2])
+                         - rhorefh[k  ] * flux_lim(w[ijk    ], s[ijk-kk2], s[ijk-kk1], s[ijk    ], s[ijk+kk1]) );
+                st[ijk] += - fx * dxi - fy * dyi - fz / rhoref[k] * dzi[k];
+            }
+}
+
 double mo_advec_2_cfl(const mo_dims* d, const double* u, const double* v, const double* w,
                       const double* dzi, double dt)
 {

@@ -43,7 +43,7 @@ class Case(C.Structure):
         ("rndamp_th", C.c_double), ("rndz", C.c_double), ("rndexp", C.c_double),
         ("adaptivestep", C.c_int),
         ("dt", C.c_double), ("dtmax", C.c_double), ("endtime", C.c_double), ("savetime", C.c_double),
-        ("nthreads", C.c_int),
+        ("nthreads", C.c_int), ("fluxlimit_th", C.c_int),
     ]
 
 
@@ -91,6 +91,7 @@ def _declare(L):
         "mo_advec_2i5_v": (None, [P, DP, DP, DP, DP, DP, DP, DP]),
         "mo_advec_2i5_w": (None, [P, DP, DP, DP, DP, DP, DP, DP]),
         "mo_advec_2i5_s": (None, [P, DP, DP, DP, DP, DP, DP, DP, DP]),
+        "mo_advec_2i5_s_lim": (None, [P, DP, DP, DP, DP, DP, DP, DP, DP]),
         "mo_advec_2_cfl": (d, [P, DP, DP, DP, DP, d]),
         "mo_advec_2_u": (None, [P, DP, DP, DP, DP, DP, DP, DP]),
         "mo_advec_2_v": (None, [P, DP, DP, DP, DP, DP, DP, DP]),

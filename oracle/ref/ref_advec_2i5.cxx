@@ -1,6 +1,8 @@
 /*
  * ref_advec_2i5.cxx -- wraps the reference's own src/advec_2i5.cxx kernels
- * (anonymous-namespace templates, reached by including the translation unit).
+ * (anonymous-namespace templates, reached by including the translation unit)
+ * and the flux-limited scalar advection of include/advec_monotonic.h that the
+ * translation unit includes.
  * TEST INFRASTRUCTURE ONLY: builds oracle/_ref, which writes golden vectors.
  */
 #include "advec_2i5.cxx"
@@ -45,6 +47,17 @@ namespace
                         d.istart, d.iend, d.jstart, d.jend, d.kstart, d.kend, d.icells, d.ijcells);
         r.save("st", st);
     }
+    void op_s_lim(const Ref_run& r)
+    {
+        Ref_dims d(r); const size_t n = d.ncells, nk = d.kcells;
+        auto st = r.load("st", n); auto s = r.load("s", n);
+        auto u = r.load("u", n); auto v = r.load("v", n); auto w = r.load("w", n);
+        auto dzi = r.load("dzi", nk); auto rho = r.load("rhoref", nk); auto rhoh = r.load("rhorefh", nk);
+        advec_s_lim<double>(st.data(), s.data(), u.data(), v.data(), w.data(), dzi.data(), d.dx, d.dy,
+                            rho.data(), rhoh.data(),
+                            d.istart, d.iend, d.jstart, d.jend, d.kstart, d.kend, d.icells, d.ijcells);
+        r.save("st", st);
+    }
     void op_cfl(const Ref_run& r)
     {
         Ref_dims d(r); const size_t n = d.ncells, nk = d.kcells;
@@ -62,5 +75,6 @@ void ref_register_advec_2i5()
     ref_register("advec_2i5_v", op_v);
     ref_register("advec_2i5_w", op_w);
     ref_register("advec_2i5_s", op_s);
+    ref_register("advec_2i5_s_lim", op_s_lim);
     ref_register("advec_2i5_cfl", op_cfl);
 }

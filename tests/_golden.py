@@ -4,7 +4,7 @@ import os
 import numpy as np
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-GRIDS = ("g1", "g2")
+GRIDS = ("g1", "g2", "g3")
 
 
 def load(grid):

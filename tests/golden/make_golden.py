@@ -27,6 +27,8 @@ GRIDS = {
     # name: itot, jtot, ktot, stretched z, variable density, dx, dy, seed
     "g1": dict(itot=8, jtot=6, ktot=8, stretch=True, anelastic=True, dx=50.0, dy=40.0, seed=11),
     "g2": dict(itot=5, jtot=7, ktot=7, stretch=False, anelastic=False, dx=25.0, dy=25.0, seed=12),
+    # two ghost levels, as Advec_2i5 sets them when a scalar is flux limited (src/advec_2i5.cxx:44)
+    "g3": dict(itot=6, jtot=5, ktot=9, stretch=True, anelastic=True, dx=30.0, dy=45.0, seed=13, kgc=2),
 }
 
 OPS = [
@@ -35,6 +37,7 @@ OPS = [
     ("advec_2i5_v", "vt", ["vt", "u", "v", "w", "dzi", "rhoref", "rhorefh"]),
     ("advec_2i5_w", "wt", ["wt", "u", "v", "w", "dzhi", "rhoref", "rhorefh"]),
     ("advec_2i5_s", "st", ["st", "s", "u", "v", "w", "dzi", "rhoref", "rhorefh"]),
+    ("advec_2i5_s_lim", "st", ["st", "s", "u", "v", "w", "dzi", "rhoref", "rhorefh"]),
     ("advec_2i5_cfl", "cfl", ["u", "v", "w", "dzi"]),
     ("advec_2_u", "ut", ["ut", "u", "v", "w", "dzi", "rhoref", "rhorefh"]),
     ("advec_2_v", "vt", ["vt", "u", "v", "w", "dzi", "rhoref", "rhorefh"]),
@@ -62,7 +65,7 @@ def make_inputs(g):
     rng = np.random.default_rng(g["seed"])
     itot, jtot, ktot = g["itot"], g["jtot"], g["ktot"]
     igc = jgc = 3
-    kgc = 1
+    kgc = g.get("kgc", 1)
     ic, jc, kc = itot + 2 * igc, jtot + 2 * jgc, ktot + 2 * kgc
     shape3, shape2 = (kc, jc, ic), (jc, ic)
 
@@ -90,8 +93,9 @@ def make_inputs(g):
     dz[1:kc - 1] = zh[2:] - zh[1:kc - 1]
     dz[kgc - 1] = dz[kgc]
     dz[kgc + ktot] = dz[kgc + ktot - 1]
-    dzi = 1.0 / dz
-    dzhi = np.zeros(kc); dzhi[1:] = 1.0 / dzh[1:]; dzhi[kgc - 1] = dzhi[kgc + 1]
+    # levels the 2nd-order grid leaves unset (kgc = 2) stay zero, as in the reference's vectors
+    dzi = np.zeros(kc); dzi[dz != 0] = 1.0 / dz[dz != 0]
+    dzhi = np.zeros(kc); nz = np.nonzero(dzh[1:])[0] + 1; dzhi[nz] = 1.0 / dzh[nz]; dzhi[kgc - 1] = dzhi[kgc + 1]
 
     if g["anelastic"]:
         rhoref = 1.2 * np.exp(-z / 8000.0)

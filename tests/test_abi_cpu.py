@@ -39,6 +39,7 @@ def test_error_paths_without_gpu():
     assert L.mhh_ctx_create(None, 0, None, None) == 1
     assert b"null" in L.mhh_last_error()
     assert L.mhh_advec_2i5_exec(None) == 1
+    assert L.mhh_advec_2i5_set_fluxlimit(None, None, 0) == 1
     gd = abi.GridDesc(8, 8, 4, 3, 3, 1, 1.0, 1.0, 1.0)
     h = C.c_void_p()
     assert L.mhh_ctx_create(C.byref(gd), 0, None, C.byref(h)) == 1   # ktot < 6

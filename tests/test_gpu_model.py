@@ -41,10 +41,13 @@ def test_initial_fields_identical():
     g.close(); o.close()
 
 
-@pytest.mark.parametrize("advec,diff", [(0, 0), (1, 1), (0, 1), (1, 0)])
-def test_substep_tendencies(advec, diff):
+@pytest.mark.parametrize("advec,diff,fluxlimit", [(0, 0, False), (1, 1, False), (0, 1, False), (1, 0, False),
+                                                  (0, 0, True)])
+def test_substep_tendencies(advec, diff, fluxlimit):
     case, g, o = _pair(itot=48, jtot=32, ktot=24, xsize=4800.0, ysize=3200.0, zsize=1200.0,
-                       adaptive=False, dt=2.0, advec=advec, diff=diff, rndamp_uvw=0.5, rndamp_th=0.5)
+                       adaptive=False, dt=2.0, advec=advec, diff=diff, rndamp_uvw=0.5, rndamp_th=0.5,
+                       fluxlimit=fluxlimit)
+    assert o.d.kgc == (2 if fluxlimit else 1)
     sl = _interior(o)
     # spin up a few substeps on the GPU so that the state is not the noise field
     for _ in range(4):

@@ -1,7 +1,8 @@
 """Per-operator parity: the HIP kernels (through the C ABI) against the oracle.
 
-Inputs are the seeded fields of the golden fixtures (grids g1, g2) plus a
-larger seeded grid.  Metric: normwise max|gpu - oracle| / max|oracle| per
+Inputs are the seeded fields of the golden fixtures (grids g1, g2 and g3,
+the last with two ghost levels as flux-limited runs use) plus a larger
+seeded grid.  Metric: normwise max|gpu - oracle| / max|oracle| per
 field, tolerance 1e-12 (the per-substep tendency criterion); both are FP64.
 """
 import ctypes as C
@@ -17,7 +18,7 @@ TOL = 1e-12
 
 
 def _case(name):
-    if name in ("g1", "g2"):
+    if name in ("g1", "g2", "g3"):
         p, i, _ = load(name)
         return p, i
     # a larger seeded case, built the same way as the fixtures
@@ -26,7 +27,7 @@ def _case(name):
     return p, i
 
 
-CASES = ["g1", "g2", "big"]
+CASES = ["g1", "g2", "g3", "big"]
 
 
 @pytest.fixture(params=CASES)
@@ -70,6 +71,38 @@ def test_advec_exec(setup, scheme):
             fn(d, po.ptr(ref), *args, dz, rho, rhoh)
         err = normwise(g.get(gname), ref)
         assert err < TOL, f"{scheme} {comp}: {err:.3e}"
+
+
+def test_advec_2i5_flux_limited_scalar(setup):
+    """[advec] fluxlimit_list = th: the scalar goes through the Koren limiter
+    (include/advec_monotonic.h:97-201), momentum through the 2i5 kernels."""
+    from microhh_amd import abi
+    p, i, d, g, *_ = setup
+    _load_common(g, i)
+    g.call("mhh_advec_2i5_set_fluxlimit", (C.c_int * 1)(1), 1)
+    g.call("mhh_advec_2i5_exec")
+    L = po.lib()
+    ref = _c(i["st"])
+    L.mo_advec_2i5_s_lim(d, po.ptr(ref), po.ptr(_c(i["s"])), *[po.ptr(_c(i[k])) for k in ("u", "v", "w")],
+                         po.ptr(_c(i["dzi"])), po.ptr(_c(i["rhoref"])), po.ptr(_c(i["rhorefh"])))
+    err = normwise(g.get("st0"), ref)
+    assert err < TOL, f"s_lim: {err:.3e}"
+    uref = _c(i["ut"])
+    L.mo_advec_2i5_u(d, po.ptr(uref), *[po.ptr(_c(i[k])) for k in ("u", "v", "w")],
+                     po.ptr(_c(i["dzi"])), po.ptr(_c(i["rhoref"])), po.ptr(_c(i["rhorefh"])))
+    assert normwise(g.get("ut"), uref) < TOL
+    # the fused tendency kernel has no limiter and refuses (no launch)
+    par = abi.SmagParams(0.23, 1.0 / 3.0, 1.5e-5, (C.c_double * 8)(*([2e-5] * 8)), 1, 0, 1)
+    rc = abi.lib().mhh_tendency_2i5_smag2(g.ctx, C.byref(par), 1)
+    assert rc == abi.MHH_ESTATE
+    # clearing the list restores the unlimited scheme
+    g.call("mhh_advec_2i5_set_fluxlimit", (C.c_int * 1)(0), 1)
+    _load_common(g, i)
+    g.call("mhh_advec_2i5_exec")
+    ref2 = _c(i["st"])
+    L.mo_advec_2i5_s(d, po.ptr(ref2), po.ptr(_c(i["s"])), *[po.ptr(_c(i[k])) for k in ("u", "v", "w")],
+                     po.ptr(_c(i["dzi"])), po.ptr(_c(i["rhoref"])), po.ptr(_c(i["rhorefh"])))
+    assert normwise(g.get("st0"), ref2) < TOL
 
 
 @pytest.mark.parametrize("scheme", ["2i5", "2"])

@@ -57,6 +57,19 @@ def test_advec(case, comp, scheme):
     assert np.array_equal(t == i[tname], ref == i[tname])
 
 
+def test_advec_2i5_s_lim(case):
+    d, p, i, o = case
+    t = _c(i["st"])
+    po.lib().mo_advec_2i5_s_lim(d, po.ptr(t), po.ptr(_c(i["s"])), po.ptr(_c(i["u"])), po.ptr(_c(i["v"])),
+                                po.ptr(_c(i["w"])), po.ptr(_c(i["dzi"])), po.ptr(_c(i["rhoref"])),
+                                po.ptr(_c(i["rhorefh"])))
+    ref = o["advec_2i5_s_lim_st"]
+    assert normwise(t, ref) < TOL
+    assert np.array_equal(t == i["st"], ref == i["st"])
+    # the limiter differs from the unlimited scheme on these rough fields
+    assert normwise(ref, o["advec_2i5_s_st"]) > 1e-3
+
+
 @pytest.mark.parametrize("scheme", ["2i5", "2"])
 def test_cfl(case, scheme):
     d, p, i, o = case

@@ -478,11 +478,12 @@ def test_rk3_reproduces_amplification_polynomial(lam):
 
 
 # ----------------------------------------------------------------- driver
-def _model(fixed_dt=None, noise=True, nx=12, ny=10, nz=12):
+def _model(fixed_dt=None, noise=True, nx=12, ny=10, nz=12, fluxlimit=False):
     from microhh_amd import model as mm
     case = mm.dry_cbl_case(itot=nx, jtot=ny, ktot=nz, xsize=50.0 * nx, ysize=50.0 * ny, zsize=1600.0,
                            adaptive=fixed_dt is None, dt=fixed_dt or 10.0,
-                           rndamp_uvw=0.02 if noise else 0.0, rndamp_th=0.2 if noise else 0.0)
+                           rndamp_uvw=0.02 if noise else 0.0, rndamp_th=0.2 if noise else 0.0,
+                           fluxlimit=fluxlimit)
     if not noise:
         case["sbot"] = 300.0  # no surface heating: the stratified state is at rest
     return po.Model(mm.oracle_case(case), *mm.profiles(case))
@@ -497,8 +498,10 @@ def test_model_rest_state_stays_at_rest():
     m.close()
 
 
-def test_model_divergence_free_and_integer_time():
-    m = _model(fixed_dt=2.0)
+@pytest.mark.parametrize("fluxlimit", [False, True])
+def test_model_divergence_free_and_integer_time(fluxlimit):
+    m = _model(fixed_dt=2.0, fluxlimit=fluxlimit)
+    assert m.d.kgc == (2 if fluxlimit else 1)
     L = po.lib()
     for n in range(6):
         m.substep()
