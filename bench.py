@@ -132,7 +132,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "mixed"])
     ap.add_argument("--itot", type=int, default=512)
     ap.add_argument("--jtot", type=int, default=512)
     ap.add_argument("--ktot", type=int, default=256)
@@ -151,7 +151,7 @@ def main():
         sys.exit(2)
 
     from microhh_amd import abi, model as mm
-    prec = mm.MHH_FP64 if args.precision == "fp64" else mm.MHH_FP32
+    prec = {"fp64": mm.MHH_FP64, "fp32": mm.MHH_FP32, "mixed": mm.MHH_MIXED}[args.precision]
     esize = 8 if prec == mm.MHH_FP64 else 4
     itot, jtot, ktot = args.itot, args.jtot, args.ktot
     case = mm.dry_cbl_case(itot=itot, jtot=jtot, ktot=ktot, xsize=50.0 * itot, ysize=50.0 * jtot,

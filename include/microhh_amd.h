@@ -33,7 +33,10 @@ extern "C" {
 #define MHH_MAX_SCALARS 8
 
 enum { MHH_OK = 0, MHH_EINVAL = 1, MHH_EHIP = 2, MHH_ESTATE = 3, MHH_ENOMEM = 4 };
-enum { MHH_FP64 = 0, MHH_FP32 = 1 };
+/* MHH_MIXED: FP32 fields, stencils and transforms; the tridiagonal solve of the
+ * pressure projection and the divergence residual in FP64.  Fields are float
+ * storage as with MHH_FP32. */
+enum { MHH_FP64 = 0, MHH_FP32 = 1, MHH_MIXED = 2 };
 /* Boundary types: include/boundary.h:51 (Boundary_type). */
 enum { MHH_BC_DIRICHLET = 0, MHH_BC_NEUMANN = 1, MHH_BC_FLUX = 2, MHH_BC_USTAR = 3, MHH_BC_OFF = 4 };
 /* Halo edges: include/boundary_cyclic.h (Edge). */
@@ -106,6 +109,12 @@ int  mhh_ctx_create(const mhh_grid_desc* grid, int precision, void* hip_stream, 
 int  mhh_ctx_destroy(mhh_ctx* ctx);
 int  mhh_fields_bind(mhh_ctx* ctx, const mhh_field_set* fields);
 int  mhh_set_base_state(mhh_ctx* ctx, const double* thref, const double* threfh);
+/* Declare that bound scalar n is stored relative to a constant offset
+ * (field = theta - offset; the offset argument of Fields::add_mean_prof and
+ * Boundary::set_bc, src/fields.cxx:1082-1101, src/boundary.cxx:51-91).  The
+ * buoyancy then subtracts threfh - offset.  Default 0.  Used by MHH_MIXED
+ * callers to keep FP32 resolution on the perturbation of theta. */
+int  mhh_set_scalar_offset(mhh_ctx* ctx, int n, double offset);
 int  mhh_sync(mhh_ctx* ctx);
 const char* mhh_last_error(void);
 /* Bytes per element of the context (8 or 4). */
@@ -159,6 +168,11 @@ int  mhh_pres_2_exec(mhh_ctx* ctx, double dt);
  * and rkdt those of the RK3 update.  The scalars are integrated with
  * mhh_timeloop_rk3_scalars. */
 int  mhh_pres_2_exec_rk3(mhh_ctx* ctx, double dt, int substep, double rkdt);
+/* MHH_MIXED only: the pressure field then holds p minus its horizontal mean,
+ * and the mean (solved in FP64) is kept per level; out receives kcells values
+ * (p = field + out[k]).  For FP64 and FP32 contexts out is zero.  No
+ * counterpart in the reference, whose p always holds the full pressure. */
+int  mhh_pres_2_mean_profile(mhh_ctx* ctx, double* out);
 /* Pres_2::check_divergence (src/pres_2.cxx:389-422). */
 int  mhh_pres_2_divergence(mhh_ctx* ctx, double* divmax);
 

@@ -42,7 +42,7 @@ typedef struct
     double rndz, rndexp;
     int adaptivestep;
     double dt, dtmax, endtime, savetime;
-    int precision;      /* MHH_FP64 or MHH_FP32 */
+    int precision;      /* MHH_FP64, MHH_FP32 or MHH_MIXED (theta then held relative to thref0) */
     int fused;          /* 1: fused kernels where the configuration allows (same results) */
     int fluxlimit_th;   /* 1: [advec] fluxlimit_list = th (Koren limiter, kgc = 2) */
 } mhh_case;

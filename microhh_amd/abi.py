@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmicrohh_amd.so")
 
 MHH_MAX_SCALARS = 8
-MHH_FP64, MHH_FP32 = 0, 1
+MHH_FP64, MHH_FP32, MHH_MIXED = 0, 1, 2
 MHH_OK, MHH_EINVAL, MHH_EHIP, MHH_ESTATE, MHH_ENOMEM = 0, 1, 2, 3, 4
 BC = {"dirichlet": 0, "neumann": 1, "flux": 2, "ustar": 3, "off": 4, "noslip": 0, "freeslip": 1}
 EDGE_EW, EDGE_NS, EDGE_BOTH = 0, 1, 2
@@ -57,6 +57,7 @@ SIGNATURES = {
     "mhh_ctx_create": (C.c_int, [C.POINTER(GridDesc), C.c_int, VP, C.POINTER(VP)]),
     "mhh_ctx_destroy": (C.c_int, [VP]),
     "mhh_fields_bind": (C.c_int, [VP, C.POINTER(FieldSet)]),
+    "mhh_set_scalar_offset": (C.c_int, [VP, C.c_int, C.c_double]),
     "mhh_set_base_state": (C.c_int, [VP, DP, DP]),
     "mhh_sync": (C.c_int, [VP]),
     "mhh_last_error": (C.c_char_p, []),
@@ -75,6 +76,7 @@ SIGNATURES = {
     "mhh_diff_2_exec": (C.c_int, [VP, C.c_double, DP]),
     "mhh_thermo_dry_exec": (C.c_int, [VP]),
     "mhh_pres_2_exec": (C.c_int, [VP, C.c_double]),
+    "mhh_pres_2_mean_profile": (C.c_int, [VP, DP]),
     "mhh_pres_2_divergence": (C.c_int, [VP, DP]),
     "mhh_tendency_2i5_smag2": (C.c_int, [VP, C.POINTER(SmagParams), C.c_int]),
     "mhh_timeloop_rk3": (C.c_int, [VP, C.c_int, C.c_double]),

@@ -82,17 +82,19 @@ namespace
             a[ij + ke*kk] = agradtop[ij]*dzh_top + a[ij + (ke-1)*kk];
     }
 
-    // calc_buoyancy_tend_2nd (src/thermo_dry.cxx:163-180).
+    // calc_buoyancy_tend_2nd (src/thermo_dry.cxx:163-180); threfh_rel is
+    // threfh minus the storage offset of th (equal to threfh without one).
     template<typename TF>
     __global__ void __launch_bounds__(BX*BY) buoyancy_g(TF* __restrict__ wt, const TF* __restrict__ th,
-                                                        const TF* __restrict__ threfh, Gd g)
+                                                        const TF* __restrict__ threfh,
+                                                        const TF* __restrict__ threfh_rel, Gd g)
     {
         const int i = blockIdx.x*BX + threadIdx.x + g.istart;
         const int j = blockIdx.y*BY + threadIdx.y + g.jstart;
         const int k = blockIdx.z + g.kstart + 1;
         if (i >= g.iend || j >= g.jend) return;
         const int ijk = i + j*g.icells + k*g.ijcells;
-        wt[ijk] += TF(grav)/threfh[k] * (i2(th[ijk-g.ijcells], th[ijk]) - threfh[k]);
+        wt[ijk] += TF(grav)/threfh[k] * (i2(th[ijk-g.ijcells], th[ijk]) - threfh_rel[k]);
     }
 
     // rk3 (src/timeloop.cxx:251-286): a += cB[s]*dt*at, then at *= cA[s+1]
@@ -221,7 +223,7 @@ namespace
     {
         const Gd& g = c->gd;
         { mhh::Timer _t(c, "buoyancy"); buoyancy_g<TF><<<grid3(g, g.kstart+1, g.kend), dim3(BX, BY), 0, c->stream>>>(
-                static_cast<TF*>(c->f.wt), static_cast<const TF*>(c->f.s[0]), p.threfh, g); }
+                static_cast<TF*>(c->f.wt), static_cast<const TF*>(c->f.s[0]), p.threfh, p.threfh_rel, g); }
         return check_launch("buoyancy");
     }
 

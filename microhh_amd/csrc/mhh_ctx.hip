@@ -24,7 +24,7 @@ namespace
     template<typename TF>
     void free_prof(mhh::Prof<TF>& p)
     {
-        TF* all[] = {p.z, p.zh, p.dz, p.dzh, p.dzi, p.dzhi, p.rhoref, p.rhorefh, p.thref, p.threfh};
+        TF* all[] = {p.z, p.zh, p.dz, p.dzh, p.dzi, p.dzhi, p.rhoref, p.rhorefh, p.thref, p.threfh, p.threfh_rel};
         for (TF* a : all) if (a) (void)hipFree(a);
         p = mhh::Prof<TF>();
     }
@@ -128,6 +128,29 @@ namespace mhh
     }
 }
 
+namespace
+{
+    // Device copies of the base state for the element type of the context.
+    template<typename TF>
+    int upload_base_state(mhh_ctx* c, mhh::Prof<TF>& p)
+    {
+        if (p.thref) { (void)hipFree(p.thref); (void)hipFree(p.threfh); (void)hipFree(p.threfh_rel); }
+        p.thref = p.threfh = p.threfh_rel = nullptr;
+        std::vector<double> rel(c->h_threfh);
+        for (double& x : rel) x -= c->s_offset[0];
+        MHH_CHECK(upload(&p.thref, c->h_thref));
+        MHH_CHECK(upload(&p.threfh, c->h_threfh));
+        return upload(&p.threfh_rel, rel);
+    }
+
+    int refresh_base_state(mhh_ctx* c)
+    {
+        if (!c->have_base_state) return 0;
+        if (c->precision == MHH_FP64) return upload_base_state(c, c->pd);
+        return upload_base_state(c, c->pf);
+    }
+}
+
 extern "C" {
 
 const char* mhh_last_error(void) { return last_error.c_str(); }
@@ -142,8 +165,8 @@ int mhh_ctx_create(const mhh_grid_desc* g, int precision, void* hip_stream, mhh_
     using mhh::fail;
     if (!g || !out) return fail(MHH_EINVAL, "mhh_ctx_create: null argument");
     *out = nullptr;
-    if (precision != MHH_FP64 && precision != MHH_FP32)
-        return fail(MHH_EINVAL, "mhh_ctx_create: precision must be MHH_FP64 or MHH_FP32");
+    if (precision != MHH_FP64 && precision != MHH_FP32 && precision != MHH_MIXED)
+        return fail(MHH_EINVAL, "mhh_ctx_create: precision must be MHH_FP64, MHH_FP32 or MHH_MIXED");
     if (g->itot < 1 || g->jtot < 1 || g->ktot < 6)
         return fail(MHH_EINVAL, "mhh_ctx_create: need itot, jtot >= 1 and ktot >= 6 "
                                 "(the 2i5 wall-level variants of src/advec_2i5.cxx:219-299 must not overlap)");
@@ -160,7 +183,8 @@ int mhh_ctx_create(const mhh_grid_desc* g, int precision, void* hip_stream, mhh_
         return fail(MHH_EINVAL, "mhh_ctx_create: ncells exceeds the 32-bit index range");
 
     mhh_ctx* c = new mhh_ctx();
-    c->precision = precision;
+    c->mixed = precision == MHH_MIXED;
+    c->precision = c->mixed ? MHH_FP32 : precision;
     c->stream = static_cast<hipStream_t>(hip_stream);
     (void)hipGetDevice(&c->device);
 
@@ -180,7 +204,9 @@ int mhh_ctx_create(const mhh_grid_desc* g, int precision, void* hip_stream, mhh_
     c->h_dzi.assign(g->dzi, g->dzi + nk);   c->h_dzhi.assign(g->dzhi, g->dzhi + nk);
     c->h_rhoref.assign(g->rhoref, g->rhoref + nk); c->h_rhorefh.assign(g->rhorefh, g->rhorefh + nk);
 
-    int rc = (precision == MHH_FP64) ? upload_profiles(c, c->pd) : upload_profiles(c, c->pf);
+    // FP64 profiles for FP64 fields and for the FP64 parts of the mixed mode.
+    int rc = (c->precision == MHH_FP64 || c->mixed) ? upload_profiles(c, c->pd) : 0;
+    if (!rc && c->precision == MHH_FP32) rc = upload_profiles(c, c->pf);
     if (!rc) rc = mhh::check_hip(hipHostMalloc(reinterpret_cast<void**>(&c->h_red), sizeof(double)), "hipHostMalloc");
     if (!rc) rc = mhh::ensure_reduction(c, 1024);
     if (!rc) rc = mhh::pres_init(c);
@@ -233,20 +259,16 @@ int mhh_set_base_state(mhh_ctx* c, const double* thref, const double* threfh)
     const size_t nk = static_cast<size_t>(c->gd.kcells);
     c->h_thref.assign(thref, thref + nk);
     c->h_threfh.assign(threfh, threfh + nk);
-    if (c->precision == MHH_FP64)
-    {
-        if (c->pd.thref) { (void)hipFree(c->pd.thref); (void)hipFree(c->pd.threfh); }
-        MHH_CHECK(upload(&c->pd.thref, c->h_thref));
-        MHH_CHECK(upload(&c->pd.threfh, c->h_threfh));
-    }
-    else
-    {
-        if (c->pf.thref) { (void)hipFree(c->pf.thref); (void)hipFree(c->pf.threfh); }
-        MHH_CHECK(upload(&c->pf.thref, c->h_thref));
-        MHH_CHECK(upload(&c->pf.threfh, c->h_threfh));
-    }
     c->have_base_state = true;
-    return 0;
+    return refresh_base_state(c);
+}
+
+int mhh_set_scalar_offset(mhh_ctx* c, int n, double offset)
+{
+    if (!c) return mhh::fail(MHH_EINVAL, "null context");
+    if (n < 0 || n >= MHH_MAX_SCALARS) return mhh::fail(MHH_EINVAL, "mhh_set_scalar_offset: scalar index out of range");
+    c->s_offset[n] = offset;
+    return refresh_base_state(c);
 }
 
 int mhh_sync(mhh_ctx* c)

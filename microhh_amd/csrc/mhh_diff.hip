@@ -110,12 +110,9 @@ namespace
     // evisc for the buoyant case (calc_evisc, src/diff_smag2.cxx:148-269) or the
     // neutral case (calc_evisc_neutral, :47-146), strain2 and N2 fused.
     template<typename TF, bool sfc, bool thermo, bool sw_mason>
-    __global__ void __launch_bounds__(BX*BY) evisc_g(Visc_args<TF> a, Gd g)
+    __device__ __forceinline__ TF evisc_point(const Visc_args<TF>& a, const Gd& g, int ij, int k)
     {
-        int i, j, k;
-        if (!point(g, g.kstart, i, j, k)) return;
         const int ii = 1, jj = g.icells, kk = g.ijcells;
-        const int ij = i + j*jj;
         const int ijk = ij + k*kk;
         const TF* __restrict__ u = a.u; const TF* __restrict__ v = a.v; const TF* __restrict__ w = a.w;
 
@@ -158,7 +155,19 @@ namespace
             const TF fac = fmin(fac_bot, fac_top);
             ev = sq(fac * a.mlen0[k]) * sqrt(s2);
         }
-        a.evisc[ijk] = ev;
+        return ev;
+    }
+
+    // evisc for the buoyant case (calc_evisc, src/diff_smag2.cxx:148-269) or the
+    // neutral case (calc_evisc_neutral, :47-146), strain2 and N2 fused; one
+    // point per thread.
+    template<typename TF, bool sfc, bool thermo, bool sw_mason>
+    __global__ void __launch_bounds__(BX*BY) evisc_g(Visc_args<TF> a, Gd g)
+    {
+        int i, j, k;
+        if (!point(g, g.kstart, i, j, k)) return;
+        const int ij = i + j*g.icells;
+        a.evisc[ij + k*g.ijcells] = evisc_point<TF, sfc, thermo, sw_mason>(a, g, ij, k);
     }
 
     // Mirror evisc over both walls (src/diff_smag2.cxx:193-205) for the interior

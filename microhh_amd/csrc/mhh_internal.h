@@ -30,6 +30,7 @@ namespace mhh
         TF* z = nullptr; TF* zh = nullptr; TF* dz = nullptr; TF* dzh = nullptr;
         TF* dzi = nullptr; TF* dzhi = nullptr; TF* rhoref = nullptr; TF* rhorefh = nullptr;
         TF* thref = nullptr; TF* threfh = nullptr;
+        TF* threfh_rel = nullptr;   // threfh - offset of scalar 0 (the buoyancy difference)
     };
 
     // Pressure-solver tables (Pres_2::set_values, src/pres_2.cxx:124-153) and
@@ -41,6 +42,9 @@ namespace mhh
         TF* bmatj = nullptr;    // jtot
         TF* a = nullptr;        // kmax
         TF* c = nullptr;        // kmax
+        TF* ac = nullptr;       // kmax: a + c
+        TF* dz2 = nullptr;      // kmax: dz*dz
+        TF* dz2rho = nullptr;   // kmax: dz*dz*rhoref
         double* twx = nullptr;  // itot complex roots exp(-2 pi i t / itot), (re, im) pairs
         double* twy = nullptr;  // jtot complex roots
     };
@@ -48,7 +52,15 @@ namespace mhh
 
 struct mhh_ctx
 {
-    int precision = MHH_FP64;
+    int precision = MHH_FP64;   // element type of the fields: MHH_FP64 or MHH_FP32
+    bool mixed = false;         // MHH_MIXED: FP32 fields, FP64 tridiagonal and divergence
+    double s_offset[MHH_MAX_SCALARS] = {};
+    // Mixed mode: the horizontal mean of the pressure, solved in FP64.
+    double* pm_part = nullptr;   // ktot x PM_CHUNKS partial plane sums of the RHS
+    double* pm_mean = nullptr;   // ktot: plane mean of the RHS
+    double* pm_work = nullptr;   // ktot: Thomas coefficients of the mean mode
+    double* pbar = nullptr;      // kcells: mean pressure per level
+    float* dpbar = nullptr;      // kcells: pbar[k] - pbar[k-1], the mean's part of the z gradient
     hipStream_t stream = nullptr;
     int device = 0;
     mhh::Gd gd{};
@@ -121,6 +133,9 @@ namespace mhh
 
     // Zero the ghost cells of a tendency field (mhh_boundary.hip).
     template<typename TF> int zero_tendency_ghosts(mhh_ctx* c, TF* at);
+
+    // Partial sums per level of the mixed mode's plane mean.
+    constexpr int PM_CHUNKS = 16;
 
     // Pressure solver setup (mhh_pres.hip).
     int pres_init(mhh_ctx* ctx);

@@ -82,7 +82,8 @@ namespace
     // rows of P at level k; writes C[k][kx][j] for kx <= itot/2.
     template<typename TF>
     __global__ void __launch_bounds__(256) fft_x_fwd_g(const TF* __restrict__ P, cplx<TF>* __restrict__ C,
-                                                       Plan plan, const double2* __restrict__ tw, int RB, Gd g)
+                                                       Plan plan, const double2* __restrict__ tw, int RB, Gd g,
+                                                       const double* __restrict__ pmean)
     {
         using T = cplx<TF>;
         extern __shared__ unsigned char smem_raw[];
@@ -96,13 +97,26 @@ namespace
         T* A = rows + row*re;
         if (plan.fast) load_twiddles<TF>(tws, tw, n);
         const TF* Pk = P + static_cast<size_t>(k)*n*M;
-        for (int x = lane; x < n; x += 64)
+        if (pmean)
         {
-            T z;
-            z.x = (ja < M) ? Pk[x + ja*n] : TF(0);
-            z.y = (jb < M) ? Pk[x + jb*n] : TF(0);
-            A[x] = z;
+            // mixed mode: the plane mean is solved separately in FP64
+            const double m = pmean[k];
+            for (int x = lane; x < n; x += 64)
+            {
+                T z;
+                z.x = (ja < M) ? TF(double(Pk[x + ja*n]) - m) : TF(0);
+                z.y = (jb < M) ? TF(double(Pk[x + jb*n]) - m) : TF(0);
+                A[x] = z;
+            }
         }
+        else
+            for (int x = lane; x < n; x += 64)
+            {
+                T z;
+                z.x = (ja < M) ? Pk[x + ja*n] : TF(0);
+                z.y = (jb < M) ? Pk[x + jb*n] : TF(0);
+                A[x] = z;
+            }
         __syncthreads();
         T* Z = fft_row<TF>(A, tws, tw, plan, -1, lane);
         // Stage the two half-spectra packed in n slots: in the row buffer itself
@@ -190,59 +204,69 @@ namespace
 
     // Tridiagonal solve per (kx, ky) mode (Pres_2::solve + tdma,
     // src/pres_2.cxx:266-324, 200-263), one thread per mode marching in k.
-    template<typename TF>
-    __global__ void __launch_bounds__(256) tdma_g(cplx<TF>* __restrict__ C, TF* __restrict__ gam,
-                                                  const TF* __restrict__ bmati, const TF* __restrict__ bmatj,
-                                                  const TF* __restrict__ a, const TF* __restrict__ c,
-                                                  const TF* __restrict__ dz, const TF* __restrict__ rho,
-                                                  int nx, Gd g)
+    // TF is the element type of the spectrum, TC the arithmetic type of the
+    // sweep (double in the mixed mode).
+    template<typename TF, typename TC>
+    __global__ void __launch_bounds__(256) tdma_g(cplx<TF>* __restrict__ C, TC* __restrict__ gam,
+                                                  const TC* __restrict__ bmati, const TC* __restrict__ bmatj,
+                                                  const TC* __restrict__ a, const TC* __restrict__ c,
+                                                  const TC* __restrict__ ac, const TC* __restrict__ dz2,
+                                                  const TC* __restrict__ dz2rho, int nx, Gd g, bool skip_mean)
     {
-        using T = cplx<TF>;
-        const int M = g.jtot, K = g.ktot, kgc = g.kgc;
+        const int M = g.jtot, K = g.ktot;
         const int mode = blockIdx.x*blockDim.x + threadIdx.x;
         const int nmodes = nx*M;
         if (mode >= nmodes) return;
         const int kx = mode / M, ky = mode % M;
-        const TF lam = bmati[kx] + bmatj[ky];
-        const size_t kstride = static_cast<size_t>(nmodes);
+        const TC lam = bmati[kx] + bmatj[ky];
+        const size_t ks = static_cast<size_t>(nmodes);
+        const bool mean_mode = (kx == 0 && ky == 0);
+        if (mean_mode && skip_mean)
+        {
+            // the mixed mode solves the horizontal mean in FP64 (mean_mode_g)
+            for (int k = 0; k < K; ++k) C[k*ks] = cplx<TF>{TF(0), TF(0)};
+            return;
+        }
 
-        auto bcoef = [&](int k) -> TF {
-            TF b = dz[k+kgc]*dz[k+kgc] * rho[k+kgc]*lam - (a[k] + c[k]);
+        // b of src/pres_2.cxx:306-324: dz*dz*rho*(bmati+bmatj) - (a+c), the
+        // bottom Neumann row and the top row (Dirichlet for the mean mode).
+        auto bcoef = [&](int k) -> TC {
+            TC b = dz2rho[k]*lam - ac[k];
             if (k == 0) b += a[0];
-            if (k == K-1) { if (kx == 0 && ky == 0) b -= c[K-1]; else b += c[K-1]; }
+            if (k == K-1) { if (mean_mode) b -= c[K-1]; else b += c[K-1]; }
             return b;
         };
 
-        // forward elimination
-        TF work2d = bcoef(0);
-        T p = C[mode];
-        TF f = dz[kgc]*dz[kgc];
-        p.x = f*p.x; p.y = f*p.y;
-        p.x /= work2d; p.y /= work2d;
-        C[mode] = p;
-        T pprev = p;
+        // forward elimination (src/pres_2.cxx:200-252)
+        TC w2 = bcoef(0);
+        TC px, py;
+        {
+            const cplx<TF> q = C[mode];
+            px = dz2[0]*TC(q.x); py = dz2[0]*TC(q.y);
+            px /= w2; py /= w2;
+            C[mode] = cplx<TF>{TF(px), TF(py)};
+        }
         for (int k = 1; k < K; ++k)
         {
-            const TF gk = c[k-1] / work2d;
-            gam[k*kstride + mode] = gk;
-            work2d = bcoef(k) - a[k]*gk;
-            T q = C[k*kstride + mode];
-            const TF fk = dz[k+kgc]*dz[k+kgc];
-            q.x = fk*q.x; q.y = fk*q.y;
-            q.x -= a[k]*pprev.x; q.y -= a[k]*pprev.y;
-            q.x /= work2d; q.y /= work2d;
-            C[k*kstride + mode] = q;
-            pprev = q;
+            const cplx<TF> q = C[k*ks + mode];
+            const TC gk = c[k-1] / w2;
+            gam[k*ks + mode] = gk;
+            w2 = bcoef(k) - a[k]*gk;
+            TC qx = dz2[k]*TC(q.x), qy = dz2[k]*TC(q.y);
+            qx -= a[k]*px; qy -= a[k]*py;
+            qx /= w2; qy /= w2;
+            C[k*ks + mode] = cplx<TF>{TF(qx), TF(qy)};
+            px = qx; py = qy;
         }
-        // back substitution
-        T pn = pprev;
+        // back substitution (src/pres_2.cxx:254-262)
         for (int k = K-2; k >= 0; --k)
         {
-            T q = C[k*kstride + mode];
-            const TF gk1 = gam[(k+1)*kstride + mode];
-            q.x -= gk1*pn.x; q.y -= gk1*pn.y;
-            C[k*kstride + mode] = q;
-            pn = q;
+            const cplx<TF> q = C[k*ks + mode];
+            const TC gk1 = gam[(k+1)*ks + mode];
+            TC qx = TC(q.x), qy = TC(q.y);
+            qx -= gk1*px; qy -= gk1*py;
+            C[k*ks + mode] = cplx<TF>{TF(qx), TF(qy)};
+            px = qx; py = qy;
         }
     }
 
@@ -358,11 +382,82 @@ namespace
         }
     }
 
+    // Mixed mode: partial plane sums of the RHS P in FP64, PM_CHUNKS per level
+    // (a fixed summation order, so the result is reproducible).
+    template<typename TF>
+    __global__ void __launch_bounds__(256) plane_sum_g(const TF* __restrict__ P, double* __restrict__ part, int nplane)
+    {
+        __shared__ double sm[256];
+        const int k = blockIdx.y, chunk = blockIdx.x, tid = threadIdx.x;
+        const int len = (nplane + PM_CHUNKS - 1)/PM_CHUNKS;
+        const int b = chunk*len, e = min(b + len, nplane);
+        const TF* Pk = P + static_cast<size_t>(k)*nplane;
+        double s = 0.;
+        for (int x = b + tid; x < e; x += 256) s += double(Pk[x]);
+        sm[tid] = s;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1)
+        {
+            if (tid < w) sm[tid] += sm[tid + w];
+            __syncthreads();
+        }
+        if (tid == 0) part[k*PM_CHUNKS + chunk] = sm[0];
+    }
+
+    // Mixed mode: the horizontal-mean mode of the projection in FP64.  The
+    // plane means of P are its (0,0) Fourier coefficient divided by
+    // itot*jtot, so the same tridiagonal system (lambda = 0, Neumann bottom,
+    // p = 0 at the top face, src/pres_2.cxx:306-324) with the means as RHS gives
+    // the mean pressure directly.  One workgroup; thread 0 runs the sweep.
+    __global__ void __launch_bounds__(256) mean_mode_g(const double* __restrict__ part, double* __restrict__ pmean,
+                                                       double* __restrict__ gam, double* __restrict__ pbar,
+                                                       float* __restrict__ dpbar,
+                                                       const double* __restrict__ a, const double* __restrict__ c,
+                                                       const double* __restrict__ ac, const double* __restrict__ dz2,
+                                                       double inv_nplane, Gd g)
+    {
+        const int K = g.ktot;
+        for (int k = threadIdx.x; k < K; k += blockDim.x)
+        {
+            double s = 0.;
+            for (int q = 0; q < PM_CHUNKS; ++q) s += part[k*PM_CHUNKS + q];
+            pmean[k] = s*inv_nplane;
+        }
+        __syncthreads();
+        if (threadIdx.x != 0) return;
+        auto bcoef = [&](int k) -> double {
+            double b = -ac[k];
+            if (k == 0) b += a[0];
+            if (k == K-1) b -= c[K-1];
+            return b;
+        };
+        double w2 = bcoef(0);
+        double pk = dz2[0]*pmean[0] / w2;
+        pbar[g.kstart] = pk;
+        for (int k = 1; k < K; ++k)
+        {
+            const double gk = c[k-1] / w2;
+            gam[k] = gk;
+            w2 = bcoef(k) - a[k]*gk;
+            pk = (dz2[k]*pmean[k] - a[k]*pk) / w2;
+            pbar[g.kstart + k] = pk;
+        }
+        for (int k = K-2; k >= 0; --k)
+        {
+            pk = pbar[g.kstart + k] - gam[k+1]*pk;
+            pbar[g.kstart + k] = pk;
+        }
+        pbar[g.kstart-1] = pbar[g.kstart];
+        dpbar[g.kstart] = 0.f;
+        for (int k = g.kstart+1; k < g.kend; ++k)
+            dpbar[k] = float(pbar[k] - pbar[k-1]);
+    }
+
     // Pres_2::output (src/pres_2.cxx:364-387).
     template<typename TF>
     __global__ void __launch_bounds__(BX*BY) pres_out_g(TF* __restrict__ ut, TF* __restrict__ vt, TF* __restrict__ wt,
                                                         const TF* __restrict__ p, const TF* __restrict__ dzhi,
-                                                        TF dxi, TF dyi, Gd g)
+                                                        TF dxi, TF dyi, Gd g, const TF* __restrict__ dpbar)
     {
         const int i = blockIdx.x*BX + threadIdx.x + g.istart;
         const int j = blockIdx.y*BY + threadIdx.y + g.jstart;
@@ -370,9 +465,11 @@ namespace
         if (i >= g.iend || j >= g.jend) return;
         const int ijk = i + j*g.icells + k*g.ijcells;
         const TF pc = p[ijk];
+        TF dpz = pc - p[ijk-g.ijcells];
+        if (dpbar) dpz += dpbar[k];   // mixed mode: the FP64 mean's part
         ut[ijk] -= (pc - p[ijk-1]) * dxi;
         vt[ijk] -= (pc - p[ijk-g.icells]) * dyi;
-        wt[ijk] -= (pc - p[ijk-g.ijcells]) * dzhi[k];
+        wt[ijk] -= dpz * dzhi[k];
     }
 
     // Pres_2::output followed by the RK3 update of u, v, w (src/pres_2.cxx:364-387,
@@ -382,7 +479,7 @@ namespace
             TF* __restrict__ u, TF* __restrict__ v, TF* __restrict__ w,
             TF* __restrict__ ut, TF* __restrict__ vt, TF* __restrict__ wt,
             const TF* __restrict__ p, const TF* __restrict__ dzhi, TF dxi, TF dyi,
-            TF cBdt, TF cAn, int zero, Gd g)
+            TF cBdt, TF cAn, int zero, Gd g, const TF* __restrict__ dpbar)
     {
         const int i = blockIdx.x*BX + threadIdx.x + g.istart;
         const int j = blockIdx.y*BY + threadIdx.y + g.jstart;
@@ -390,20 +487,24 @@ namespace
         if (i >= g.iend || j >= g.jend) return;
         const int ijk = i + j*g.icells + k*g.ijcells;
         const TF pc = p[ijk];
+        TF dpz = pc - p[ijk-g.ijcells];
+        if (dpbar) dpz += dpbar[k];   // mixed mode: the FP64 mean's part
         TF tu = ut[ijk]; tu -= (pc - p[ijk-1]) * dxi;
         TF tv = vt[ijk]; tv -= (pc - p[ijk-g.icells]) * dyi;
-        TF tw = wt[ijk]; tw -= (pc - p[ijk-g.ijcells]) * dzhi[k];
+        TF tw = wt[ijk]; tw -= dpz * dzhi[k];
         u[ijk] += cBdt*tu; ut[ijk] = zero ? TF(0) : tu*cAn;
         v[ijk] += cBdt*tv; vt[ijk] = zero ? TF(0) : tv*cAn;
         w[ijk] += cBdt*tw; wt[ijk] = zero ? TF(0) : tw*cAn;
     }
 
-    // Pres_2::calc_divergence (src/pres_2.cxx:389-422), per-block maxima.
-    template<typename TF>
+    // Pres_2::calc_divergence (src/pres_2.cxx:389-422), per-block maxima.  TC
+    // is the arithmetic type (double in the mixed mode: the FP64 residual of
+    // the FP32 fields).
+    template<typename TF, typename TC>
     __global__ void __launch_bounds__(BX*BY) divergence_g(const TF* __restrict__ u, const TF* __restrict__ v,
-                                                          const TF* __restrict__ w, const TF* __restrict__ dzi,
-                                                          const TF* __restrict__ rho, const TF* __restrict__ rhoh,
-                                                          TF dxi, TF dyi, Gd g, double* __restrict__ red)
+                                                          const TF* __restrict__ w, const TC* __restrict__ dzi,
+                                                          const TC* __restrict__ rho, const TC* __restrict__ rhoh,
+                                                          TC dxi, TC dyi, Gd g, double* __restrict__ red)
     {
         __shared__ double sm[BX*BY];
         const int tid = threadIdx.x + threadIdx.y*BX;
@@ -417,8 +518,8 @@ namespace
             {
                 const int jj = g.icells, kk = g.ijcells;
                 const int ijk = i + j*jj + k*kk;
-                const TF div = rho[k]*((u[ijk+1]-u[ijk])*dxi + (v[ijk+jj]-v[ijk])*dyi)
-                             + (rhoh[k+1]*w[ijk+kk]-rhoh[k]*w[ijk])*dzi[k];
+                const TC div = rho[k]*((TC(u[ijk+1])-TC(u[ijk]))*dxi + (TC(v[ijk+jj])-TC(v[ijk]))*dyi)
+                             + (rhoh[k+1]*TC(w[ijk+kk])-rhoh[k]*TC(w[ijk]))*dzi[k];
                 m = fmax(m, static_cast<double>(fabs(div)));
             }
         sm[tid] = m;
@@ -452,7 +553,7 @@ namespace
         const Gd& g = c->gd;
         const int N = g.itot, M = g.jtot, K = g.ktot;
         // Pres_2::set_values (src/pres_2.cxx:124-153) in the element type.
-        std::vector<TF> bi(N), bj(M), av(K), cv(K);
+        std::vector<TF> bi(N), bj(M), av(K), cv(K), acv(K), dz2v(K), dz2rhov(K);
         const TF dxidxi = TF(1.)/(TF(g.dx)*TF(g.dx));
         const TF dyidyi = TF(1.)/(TF(g.dy)*TF(g.dy));
         const TF pi = std::acos(TF(-1.));
@@ -464,6 +565,10 @@ namespace
         {
             av[k] = TF(c->h_dz[k+g.kgc]) * TF(c->h_rhorefh[k+g.kgc  ])*TF(c->h_dzhi[k+g.kgc  ]);
             cv[k] = TF(c->h_dz[k+g.kgc]) * TF(c->h_rhorefh[k+g.kgc+1])*TF(c->h_dzhi[k+g.kgc+1]);
+            // the factors of b in the order of src/pres_2.cxx:311-312
+            acv[k] = av[k] + cv[k];
+            dz2v[k] = TF(c->h_dz[k+g.kgc])*TF(c->h_dz[k+g.kgc]);
+            dz2rhov[k] = dz2v[k]*TF(c->h_rhoref[k+g.kgc]);
         }
         auto up = [](TF** d, const std::vector<TF>& h) -> int {
             MHH_CHECK(check_hip(hipMalloc(d, sizeof(TF)*h.size()), "hipMalloc(pres table)"));
@@ -473,6 +578,9 @@ namespace
         MHH_CHECK(up(&t.bmatj, bj));
         MHH_CHECK(up(&t.a, av));
         MHH_CHECK(up(&t.c, cv));
+        MHH_CHECK(up(&t.ac, acv));
+        MHH_CHECK(up(&t.dz2, dz2v));
+        MHH_CHECK(up(&t.dz2rho, dz2rhov));
         // Twiddles exp(-2 pi i t/n) in long double, rounded once.
         auto roots = [](double** d, int n) -> int {
             std::vector<double> h(2*static_cast<size_t>(n));
@@ -498,6 +606,9 @@ namespace
         if (t.bmatj) (void)hipFree(t.bmatj);
         if (t.a) (void)hipFree(t.a);
         if (t.c) (void)hipFree(t.c);
+        if (t.ac) (void)hipFree(t.ac);
+        if (t.dz2) (void)hipFree(t.dz2);
+        if (t.dz2rho) (void)hipFree(t.dz2rho);
         if (t.twx) (void)hipFree(t.twx);
         if (t.twy) (void)hipFree(t.twy);
         t = Pres_tables<TF>();
@@ -530,13 +641,23 @@ namespace
                     static_cast<const TF*>(c->f.ut), static_cast<const TF*>(c->f.vt), static_cast<const TF*>(c->f.wt),
                     p.dzi, p.rhoref, p.rhorefh, dxi, dyi, dti, g); }
             MHH_CHECK(check_launch("pres_in"));
+            if (c->mixed)
+            {
+                { mhh::Timer _t(c, "pres_mean"); plane_sum_g<TF><<<dim3(PM_CHUNKS, K), 256, 0, c->stream>>>(P, c->pm_part, N*M); }
+                MHH_CHECK(check_launch("plane_sum"));
+                const Pres_tables<double>& d = c->td;
+                { mhh::Timer _t(c, "pres_mean"); mean_mode_g<<<1, 256, 0, c->stream>>>(c->pm_part, c->pm_mean, c->pm_work,
+                        c->pbar, c->dpbar, d.a, d.c, d.ac, d.dz2, 1./(double(N)*double(M)), g); }
+                MHH_CHECK(check_launch("mean_mode"));
+            }
         }
         // 2. forward x transform
         {
             const int RB = rows_per_block(px, sizeof(T));
             const dim3 grd((M + 2*RB - 1)/(2*RB), K);
             const size_t lds = lds_bytes(px, RB, sizeof(T));
-            { mhh::Timer _t(c, "fft_x_fwd"); fft_x_fwd_g<TF><<<grd, dim3(64, RB), lds, c->stream>>>(P, C, px, reinterpret_cast<const double2*>(t.twx), RB, g); }
+            { mhh::Timer _t(c, "fft_x_fwd"); fft_x_fwd_g<TF><<<grd, dim3(64, RB), lds, c->stream>>>(P, C, px, reinterpret_cast<const double2*>(t.twx), RB, g,
+                    c->mixed ? c->pm_mean : nullptr); }
             MHH_CHECK(check_launch("fft_x_fwd"));
         }
         // 3. forward y transform
@@ -549,8 +670,19 @@ namespace
         // 4. tridiagonal per mode
         {
             const int nmodes = nx*M;
-            { mhh::Timer _t(c, "tdma"); tdma_g<TF><<<(nmodes + 255)/256, 256, 0, c->stream>>>(C, gam, t.bmati, t.bmatj, t.a, t.c,
-                                                                  p.dz, p.rhoref, nx, g); }
+            if (c->mixed)
+            {
+                const Pres_tables<double>& d = c->td;
+                mhh::Timer _t(c, "tdma");
+                tdma_g<TF, double><<<(nmodes + 255)/256, 256, 0, c->stream>>>(C, reinterpret_cast<double*>(c->f.tmp2),
+                        d.bmati, d.bmatj, d.a, d.c, d.ac, d.dz2, d.dz2rho, nx, g, true);
+            }
+            else
+            {
+                mhh::Timer _t(c, "tdma");
+                tdma_g<TF, TF><<<(nmodes + 255)/256, 256, 0, c->stream>>>(C, gam, t.bmati, t.bmatj, t.a, t.c,
+                        t.ac, t.dz2, t.dz2rho, nx, g, false);
+            }
             MHH_CHECK(check_launch("tdma"));
         }
         // 5. inverse y transform (divides by jtot)
@@ -565,12 +697,14 @@ namespace
             { mhh::Timer _t(c, "fft_x_bwd"); fft_x_bwd_g<TF><<<grd, dim3(64, RB), lds, c->stream>>>(C, P, px, reinterpret_cast<const double2*>(t.twx), RB, g); }
             MHH_CHECK(check_launch("fft_x_bwd"));
         }
-        // 7. pressure gradient on the tendencies
+        // 7. pressure gradient on the tendencies (with the FP64 mean's part in the
+        // mixed mode, whose fields are float)
+        const TF* dpb = c->mixed ? reinterpret_cast<const TF*>(c->dpbar) : nullptr;
         if (rk3 < 0)
         {
             { mhh::Timer _t(c, "pres_out"); pres_out_g<TF><<<grid3(g, g.kstart, g.kend), dim3(BX, BY), 0, c->stream>>>(
                     static_cast<TF*>(c->f.ut), static_cast<TF*>(c->f.vt), static_cast<TF*>(c->f.wt),
-                    P, p.dzhi, dxi, dyi, g); }
+                    P, p.dzhi, dxi, dyi, g, dpb); }
             return check_launch("pres_out");
         }
         // Williamson low-storage coefficients (src/timeloop.cxx:255-256).
@@ -580,7 +714,7 @@ namespace
         { mhh::Timer _t(c, "pres_out_rk3"); pres_out_rk3_g<TF><<<grid3(g, g.kstart, g.kend), dim3(BX, BY), 0, c->stream>>>(
                 static_cast<TF*>(c->f.u), static_cast<TF*>(c->f.v), static_cast<TF*>(c->f.w),
                 static_cast<TF*>(c->f.ut), static_cast<TF*>(c->f.vt), static_cast<TF*>(c->f.wt),
-                P, p.dzhi, dxi, dyi, cB[rk3]*TF(rkdt), cA[substepn], substepn == 0, g); }
+                P, p.dzhi, dxi, dyi, cB[rk3]*TF(rkdt), cA[substepn], substepn == 0, g, dpb); }
         MHH_CHECK(check_launch("pres_out_rk3"));
         if (substepn == 0)
         {
@@ -598,9 +732,21 @@ namespace
         const dim3 grd = grid_red(g);
         const int nb = static_cast<int>(grd.x*grd.y*grd.z);
         MHH_CHECK(ensure_reduction(c, nb));
-        { mhh::Timer _t(c, "divergence"); divergence_g<TF><<<grd, dim3(BX, BY), 0, c->stream>>>(
-                static_cast<const TF*>(c->f.u), static_cast<const TF*>(c->f.v), static_cast<const TF*>(c->f.w),
-                p.dzi, p.rhoref, p.rhorefh, TF(1.)/TF(g.dx), TF(1.)/TF(g.dy), g, c->red); }
+        if (c->mixed)
+        {
+            const Prof<double>& d = c->pd;
+            mhh::Timer _t(c, "divergence");
+            divergence_g<TF, double><<<grd, dim3(BX, BY), 0, c->stream>>>(
+                    static_cast<const TF*>(c->f.u), static_cast<const TF*>(c->f.v), static_cast<const TF*>(c->f.w),
+                    d.dzi, d.rhoref, d.rhorefh, 1./g.dx, 1./g.dy, g, c->red);
+        }
+        else
+        {
+            mhh::Timer _t(c, "divergence");
+            divergence_g<TF, TF><<<grd, dim3(BX, BY), 0, c->stream>>>(
+                    static_cast<const TF*>(c->f.u), static_cast<const TF*>(c->f.v), static_cast<const TF*>(c->f.w),
+                    p.dzi, p.rhoref, p.rhorefh, TF(1.)/TF(g.dx), TF(1.)/TF(g.dy), g, c->red);
+        }
         MHH_CHECK(check_launch("divergence"));
         return reduce_max_partials(c, nb, out);
     }
@@ -613,13 +759,31 @@ namespace mhh
         Plan px, py;
         if (!make_plan(c->gd.itot, px) || !make_plan(c->gd.jtot, py))
             return fail(MHH_EINVAL, "itot and jtot must factor into primes <= 13 (FFT radices)");
-        return c->precision == MHH_FP64 ? setup_tables<double>(c, c->td) : setup_tables<float>(c, c->tf);
+        // FP64 tables for FP64 fields and for the FP64 tridiagonal of the mixed mode.
+        if (c->precision == MHH_FP64 || c->mixed) MHH_CHECK(setup_tables<double>(c, c->td));
+        if (c->precision == MHH_FP32) MHH_CHECK(setup_tables<float>(c, c->tf));
+        if (c->mixed)
+        {
+            const size_t K = c->gd.ktot, nk = c->gd.kcells;
+            MHH_CHECK(check_hip(hipMalloc(&c->pm_part, sizeof(double)*K*PM_CHUNKS), "hipMalloc(pm_part)"));
+            MHH_CHECK(check_hip(hipMalloc(&c->pm_mean, sizeof(double)*K), "hipMalloc(pm_mean)"));
+            MHH_CHECK(check_hip(hipMalloc(&c->pm_work, sizeof(double)*K), "hipMalloc(pm_work)"));
+            MHH_CHECK(check_hip(hipMalloc(&c->pbar, sizeof(double)*nk), "hipMalloc(pbar)"));
+            MHH_CHECK(check_hip(hipMalloc(&c->dpbar, sizeof(float)*nk), "hipMalloc(dpbar)"));
+            MHH_CHECK(check_hip(hipMemset(c->pbar, 0, sizeof(double)*nk), "hipMemset(pbar)"));
+            MHH_CHECK(check_hip(hipMemset(c->dpbar, 0, sizeof(float)*nk), "hipMemset(dpbar)"));
+            MHH_CHECK(check_hip(hipMemset(c->pm_work, 0, sizeof(double)*K), "hipMemset(pm_work)"));
+        }
+        return 0;
     }
 
     void pres_free(mhh_ctx* c)
     {
         free_tables(c->td);
         free_tables(c->tf);
+        for (void* b : {static_cast<void*>(c->pm_part), static_cast<void*>(c->pm_mean), static_cast<void*>(c->pm_work),
+                        static_cast<void*>(c->pbar), static_cast<void*>(c->dpbar)})
+            if (b) (void)hipFree(b);
     }
 }
 
@@ -639,6 +803,19 @@ int mhh_pres_2_exec_rk3(mhh_ctx* c, double dt, int substep, double rkdt)
     if (substep < 0 || substep > 2) return mhh::fail(MHH_EINVAL, "rk3 substep must be 0, 1 or 2");
     return c->precision == MHH_FP64 ? exec<double>(c, c->pd, c->td, dt, substep, rkdt)
                                     : exec<float>(c, c->pf, c->tf, dt, substep, rkdt);
+}
+
+int mhh_pres_2_mean_profile(mhh_ctx* c, double* out)
+{
+    if (!c || !out) return mhh::fail(MHH_EINVAL, "mhh_pres_2_mean_profile: null argument");
+    const size_t nk = static_cast<size_t>(c->gd.kcells);
+    if (!c->mixed)
+    {
+        std::fill(out, out + nk, 0.);
+        return 0;
+    }
+    MHH_CHECK(mhh::check_hip(c->stream ? hipStreamSynchronize(c->stream) : hipDeviceSynchronize(), "sync"));
+    return mhh::check_hip(hipMemcpy(out, c->pbar, sizeof(double)*nk, hipMemcpyDeviceToHost), "hipMemcpy(pbar)");
 }
 
 int mhh_pres_2_divergence(mhh_ctx* c, double* out)

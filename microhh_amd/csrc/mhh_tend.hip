@@ -35,6 +35,7 @@ namespace
         const TF* __restrict__ s; const TF* __restrict__ evisc;
         const TF* __restrict__ dzi; const TF* __restrict__ dzhi;
         const TF* __restrict__ rho; const TF* __restrict__ rhoh; const TF* __restrict__ threfh;
+        const TF* __restrict__ threfh_rel;
         TF dxi, dyi, dxidxi, dyidyi, visc, svisc, tPr_i;
         int buoyancy;
         int nti, ntj, nkc;
@@ -220,7 +221,7 @@ namespace
                 {
                     TF t = a.wt[ijk];
                     if (a.buoyancy)
-                        t += TF(grav)/a.threfh[k] * (i2(qs[2], qs[3]) - a.threfh[k]);
+                        t += TF(grav)/a.threfh[k] * (i2(qs[2], qs[3]) - a.threfh_rel[k]);
                     t += hsum(hflux_2i5(i2(Up[c+1], Uc[c+1]), i2(Up[c], Uc[c]), Wc, c, 1, dxi),
                               hflux_2i5(i2(Vp[c+PW], Vc[c+PW]), i2(Vp[c], Vc[c]), Wc, c, PW, dyi));
                     t += vflux_2i5_w(level_class_w(k, g.kstart, g.kend), a.rho[k], a.rho[k-1], rhohk, dzhik,
@@ -312,7 +313,7 @@ namespace
         a.u = static_cast<const TF*>(c->f.u); a.v = static_cast<const TF*>(c->f.v);
         a.w = static_cast<const TF*>(c->f.w); a.s = static_cast<const TF*>(c->f.s[0]);
         a.evisc = static_cast<const TF*>(c->f.evisc);
-        a.dzi = p.dzi; a.dzhi = p.dzhi; a.rho = p.rhoref; a.rhoh = p.rhorefh; a.threfh = p.threfh;
+        a.dzi = p.dzi; a.dzhi = p.dzhi; a.rho = p.rhoref; a.rhoh = p.rhorefh; a.threfh = p.threfh; a.threfh_rel = p.threfh_rel;
         a.dxi = TF(1.)/TF(g.dx); a.dyi = TF(1.)/TF(g.dy);
         a.dxidxi = TF(1./(g.dx*g.dx)); a.dyidyi = TF(1./(g.dy*g.dy));
         a.visc = TF(par.visc); a.svisc = TF(par.svisc[0]); a.tPr_i = TF(1)/TF(par.tPr);

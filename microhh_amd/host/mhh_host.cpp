@@ -225,8 +225,9 @@ namespace mhh_host
         };
         add_prof(u, u_prof, c.utrans);
         add_prof(v, v_prof, c.vtrans);
+        const double th_offset = has_th ? sp.at("th")->offset : 0.;
         if (has_th)
-            add_prof(th, th_prof, 0.);
+            add_prof(th, th_prof, th_offset);
         for (int l = 0; l < gd.ijcells; ++l)
         {
             w[gd.kstart*gd.ijcells + l] = 0.;
@@ -238,7 +239,7 @@ namespace mhh_host
         if (has_th)
         {
             upload(sp.at("th")->fld, th);
-            upload(sp.at("th")->fld_bot, std::vector<double>(gd.ijcells, th_prof[0]));
+            upload(sp.at("th")->fld_bot, std::vector<double>(gd.ijcells, th_prof[0] - th_offset));
         }
     }
 
@@ -291,7 +292,11 @@ namespace mhh_host
     void Fields<TF>::set_host(const std::string& name, const double* data)
     {
         const auto& gd = grid.get_grid_data();
-        upload(any(name).fld, std::vector<double>(data, data + gd.ncells));
+        const Field3d<TF>& f = any(name);
+        std::vector<double> h(data, data + gd.ncells);
+        if (f.offset != 0. && (mp.count(name) || sp.count(name)))
+            for (double& x : h) x -= f.offset;
+        upload(f.fld, h);
     }
 
     template<typename TF>
@@ -299,7 +304,10 @@ namespace mhh_host
     {
         const auto& gd = grid.get_grid_data();
         std::vector<double> h;
-        download(h, any(name).fld, static_cast<size_t>(gd.ncells));
+        const Field3d<TF>& f = any(name);
+        download(h, f.fld, static_cast<size_t>(gd.ncells));
+        if (f.offset != 0. && (mp.count(name) || sp.count(name)))
+            for (double& x : h) x += f.offset;
         std::memcpy(data, h.data(), sizeof(double)*h.size());
     }
 
@@ -347,8 +355,8 @@ namespace mhh_host
         for (auto& it : fields.sp)
         {
             auto& s = *it.second;
-            set_bc(s.fld_bot, s.grad_bot, s.flux_bot, sbcbot, sbot, s.visc, 0.);
-            set_bc(s.fld_top, s.grad_top, s.flux_top, sbctop, stop, s.visc, 0.);
+            set_bc(s.fld_bot, s.grad_bot, s.flux_bot, sbcbot, sbot, s.visc, s.offset);
+            set_bc(s.fld_top, s.grad_top, s.flux_top, sbctop, stop, s.visc, s.offset);
         }
     }
 
@@ -623,8 +631,15 @@ namespace mhh_host
         g.z = gd.z.data(); g.zh = gd.zh.data(); g.dz = gd.dz.data(); g.dzh = gd.dzh.data();
         g.dzi = gd.dzi.data(); g.dzhi = gd.dzhi.data();
         g.rhoref = fields->rhoref.data(); g.rhorefh = fields->rhorefh.data();
-        const int prec = std::is_same<TF, double>::value ? MHH_FP64 : MHH_FP32;
+        const int prec = std::is_same<TF, double>::value ? MHH_FP64 : (c.precision == MHH_MIXED ? MHH_MIXED : MHH_FP32);
         check(mhh_ctx_create(&g, prec, stream, &ctx_), "mhh_ctx_create");
+        if (c.precision == MHH_MIXED && c.thermo == 0)
+        {
+            // theta is held as theta - thref0 in FP32 so that its perturbation
+            // keeps the resolution of the arithmetic
+            fields->sp.at("th")->offset = c.thref0;
+            check(mhh_set_scalar_offset(ctx_, 0, c.thref0), "mhh_set_scalar_offset");
+        }
 
         if (c.thermo == 0)
         {
@@ -726,6 +741,16 @@ namespace mhh_host
     {
         check(mhh_sync(ctx_), "mhh_sync");
         fields->get_host(name, host);
+        if (name == "p" && cs.precision == MHH_MIXED)
+        {
+            // the mixed mode holds the horizontal mean of p separately (FP64)
+            const auto& gd = grid->get_grid_data();
+            std::vector<double> pbar(gd.kcells);
+            check(mhh_pres_2_mean_profile(ctx_, pbar.data()), "mhh_pres_2_mean_profile");
+            for (int k = 0; k < gd.kcells; ++k)
+                for (int ij = 0; ij < gd.ijcells; ++ij)
+                    host[ij + static_cast<size_t>(k)*gd.ijcells] += pbar[k];
+        }
     }
 
     template<typename TF>

@@ -67,6 +67,7 @@ namespace mhh_host
     {
         std::string name;
         double visc = 0.;
+        double offset = 0.;   // the device field holds value - offset (mixed precision)
         TF* fld = nullptr;
         TF* fld_bot = nullptr; TF* fld_top = nullptr;
         TF* grad_bot = nullptr; TF* grad_top = nullptr;

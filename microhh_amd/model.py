@@ -10,7 +10,7 @@ import numpy as np
 
 from . import abi
 
-MHH_FP64, MHH_FP32 = abi.MHH_FP64, abi.MHH_FP32
+MHH_FP64, MHH_FP32, MHH_MIXED = abi.MHH_FP64, abi.MHH_FP32, abi.MHH_MIXED
 
 
 class Case(C.Structure):

@@ -106,3 +106,61 @@ def test_adaptive_dt_sequence():
             g.substep(); o.substep()
         assert g.time()[1] == o.dt, f"step {step}: dt {g.time()[1]} vs {o.dt}"
     g.close(); o.close()
+
+
+def _state_pair(precision, **kw):
+    """An FP64 model advanced 2 steps, and a model of the given precision
+    started from that same state (both at substep 0)."""
+    a = mm.Model(mm.dry_cbl_case(precision=mm.MHH_FP64, **kw))
+    a.steps(2)
+    b = mm.Model(mm.dry_cbl_case(precision=precision, **kw))
+    for n in ("u", "v", "w", "th", "ut", "vt", "wt", "tht"):
+        b.set_field(n, a.field(n))
+    return a, b
+
+
+MIXED_KW = dict(itot=64, jtot=64, ktot=48, xsize=3200.0, ysize=3200.0, zsize=1200.0,
+                adaptive=False, dt=2.0, rndamp_uvw=0.5, rndamp_th=0.5)
+
+
+def test_mixed_precision_one_substep_against_fp64():
+    """MHH_MIXED (FP32 fields and transforms; FP64 tridiagonal, mean pressure
+    and divergence residual; theta held relative to thref0) against FP64 from
+    the same state, one substep.  Tolerances: FP32 resolution times the
+    conditioning of one substep (measured 3e-6 in u at 512x512x256)."""
+    a, b = _state_pair(mm.MHH_MIXED, **MIXED_KW)
+    sl = np.s_[1:-1, 3:-3, 3:-3]
+    for s in range(3):
+        a.substep(); b.substep()
+        for n, tol in (("u", 5e-5), ("v", 5e-5), ("w", 5e-5), ("th", 1e-6)):
+            err = _nw(b.field(n)[sl], a.field(n)[sl])
+            assert err < tol, f"substep {s} {n}: {err:.3e}"
+        assert b.div() < 1e-7, f"substep {s}: divergence {b.div():.3e}"
+    # the pressure read back includes the FP64 horizontal mean
+    err = _nw(b.field("p")[sl], a.field("p")[sl])
+    assert err < 1e-4, f"p: {err:.3e}"
+    a.close(); b.close()
+
+
+def test_mixed_precision_initial_theta():
+    """theta = (theta - thref0) in FP32 plus thref0: the perturbation keeps FP32
+    resolution (|error| <= 2 ulp of the largest perturbation)."""
+    kw = dict(MIXED_KW)
+    a = mm.Model(mm.dry_cbl_case(precision=mm.MHH_FP64, **kw))
+    b = mm.Model(mm.dry_cbl_case(precision=mm.MHH_MIXED, **kw))
+    sl = np.s_[1:-1, 3:-3, 3:-3]
+    ta, tb = a.field("th")[sl], b.field("th")[sl]
+    dev = np.max(np.abs(ta - 300.0))
+    assert np.max(np.abs(tb - ta)) <= 2.0 * np.finfo(np.float32).eps * dev
+    a.close(); b.close()
+
+
+def test_fp32_runs_and_projects():
+    """Plain FP32 (the reference's single-precision build analogue): finite
+    fields and an FP32-level divergence after a few steps."""
+    m = mm.Model(mm.dry_cbl_case(precision=mm.MHH_FP32, **MIXED_KW))
+    m.steps(3)
+    for n in ("u", "v", "w", "th"):
+        assert np.all(np.isfinite(m.field(n))), n
+    assert m.div() < 1e-3
+    m.close()
