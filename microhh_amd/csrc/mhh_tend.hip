@@ -59,19 +59,32 @@ namespace
     constexpr int NQ3 = (PN + TX*TY - 1)/(TX*TY);
     constexpr int NQ1 = (EN + TX*TY - 1)/(TX*TY);
 
-    template<typename TF, int NQ>
-    __device__ __forceinline__ void fetch_plane(TF (&r)[NQ], const TF* __restrict__ src, int k,
-                                                int i0, int j0, int halo, int w, int n, const Gd& g, int tid)
+    // Offsets (relative to the start of a plane) of the thread's share of a
+    // halo plane, -1 where the element lies outside the field; the same for
+    // every level, so they are computed once per chunk.
+    template<int NQ>
+    __device__ __forceinline__ void plane_offsets(int (&o)[NQ], int i0, int j0, int halo, int w, int n,
+                                                  const Gd& g, int tid)
     {
-        const TF* __restrict__ sk = src + static_cast<size_t>(k)*g.ijcells;
         #pragma unroll
         for (int q = 0; q < NQ; ++q)
         {
             const int idx = tid + q*TX*TY;
             const int pj = idx / w, pi = idx - pj*w;
             const int gi = i0 - halo + pi, gj = j0 - halo + pj;
-            r[q] = (idx < n && k < g.kcells && gi < g.icells && gj < g.jcells) ? sk[gi + gj*g.icells] : TF(0);
+            o[q] = (idx < n && gi < g.icells && gj < g.jcells) ? gi + gj*g.icells : -1;
         }
+    }
+
+    template<typename TF, int NQ>
+    __device__ __forceinline__ void fetch_plane(TF (&r)[NQ], const TF* __restrict__ src, int k,
+                                                const int (&o)[NQ], const Gd& g)
+    {
+        const TF* __restrict__ sk = src + static_cast<size_t>(k)*g.ijcells;
+        const bool kin = k < g.kcells;
+        #pragma unroll
+        for (int q = 0; q < NQ; ++q)
+            r[q] = (kin && o[q] >= 0) ? sk[o[q]] : TF(0);
     }
 
     template<typename TF, int NQ>
@@ -85,8 +98,52 @@ namespace
         }
     }
 
+    // Vertical 2i5 face products of a cell-centred field at w level f (the face
+    // between cells f-1 and f), s0..s5 = s[f-3 .. f+2]: the centred part
+    // rf*wf*I and the upwind part rf*|wf|*D with the interpolation order of the
+    // face (src/advec_2i5.cxx:215-299): zero at the walls, 2nd order next to
+    // them, 4th/3rd order one further, 6th/5th order inside.  Both cells that
+    // share a face use the same products, so each face is computed once.
     template<typename TF>
-    __global__ void __launch_bounds__(TX*TY) tend_2i5_smag2_g(Tend_args<TF> a, Gd g)
+    __device__ __forceinline__ void vface_c(int f, int ks, int ke, TF rf, TF wf,
+                                            TF s0, TF s1, TF s2, TF s3, TF s4, TF s5, TF& fc, TF& fu)
+    {
+        if (f == ks || f == ke) { fc = TF(0); fu = TF(0); }
+        else if (f == ks+1 || f == ke-1) { fc = rf*wf*i2(s2, s3); fu = TF(0); }
+        else if (f == ks+2 || f == ke-2)
+        {
+            fc = rf*wf*i4ws(s1, s2, s3, s4);
+            fu = rf*fabs(wf)*i3ws(s1, s2, s3, s4);
+        }
+        else
+        {
+            fc = rf*wf*i6ws(s0, s1, s2, s3, s4, s5);
+            fu = rf*fabs(wf)*i5ws(s0, s1, s2, s3, s4, s5);
+        }
+    }
+
+    // The same for w at cell-centre level c (the face between w levels c and
+    // c+1), w0..w5 = w[c-2 .. c+3] (src/advec_2i5.cxx:501-578).
+    template<typename TF>
+    __device__ __forceinline__ void vface_w(int c, int ks, int ke, TF rf,
+                                            TF w0, TF w1, TF w2, TF w3, TF w4, TF w5, TF& fc, TF& fu)
+    {
+        const TF wf = i2(w2, w3);
+        if (c == ks || c == ke-1) { fc = rf*wf*i2(w2, w3); fu = TF(0); }
+        else if (c == ks+1 || c == ke-2)
+        {
+            fc = rf*wf*i4ws(w1, w2, w3, w4);
+            fu = rf*fabs(wf)*i3ws(w1, w2, w3, w4);
+        }
+        else
+        {
+            fc = rf*wf*i6ws(w0, w1, w2, w3, w4, w5);
+            fu = rf*fabs(wf)*i5ws(w0, w1, w2, w3, w4, w5);
+        }
+    }
+
+    template<typename TF>
+    __global__ void __launch_bounds__(TX*TY, 2) tend_2i5_smag2_g(Tend_args<TF> a, Gd g)
     {
         __shared__ TF lds[7*PN + 3*EN];
 
@@ -147,24 +204,50 @@ namespace
         load_plane(Sc, a.s, kb, i0, j0, H, PW, PN, g, tid);
         load_plane(En, a.evisc, kb+1, i0, j0, 1, EW, EN, g, tid);
         TF ru[NQ3], rv[NQ3], rw[NQ3], rs[NQ3], re[NQ1];
+        int o3[NQ3], o1[NQ1];
+        plane_offsets(o3, i0, j0, H, PW, PN, g, tid);
+        plane_offsets(o1, i0, j0, 1, EW, EN, g, tid);
+
+        // Bottom-face terms of the first level of the chunk; every later level
+        // takes them from the top face of the level below.
+        TF bu_c, bu_u, bv_c, bv_u, bw_c, bw_u, bs_c, bs_u;   // advective
+        TF du_b, dv_b, dw_b, ds_b;                          // diffusive
+        __syncthreads();
+        if (active)
+        {
+            vface_c(kb, g.kstart, g.kend, a.rhoh[kb], i2(Wc[c-1], Wc[c]),
+                    qu[0], qu[1], qu[2], qu[3], qu[4], qu[5], bu_c, bu_u);
+            vface_c(kb, g.kstart, g.kend, a.rhoh[kb], i2(Wc[c-PW], Wc[c]),
+                    qv[0], qv[1], qv[2], qv[3], qv[4], qv[5], bv_c, bv_u);
+            vface_c(kb, g.kstart, g.kend, a.rhoh[kb], Wc[c],
+                    qs[0], qs[1], qs[2], qs[3], qs[4], qs[5], bs_c, bs_u);
+            vface_w(kb-1, g.kstart, g.kend, a.rho[kb-1],
+                    qw[0], qw[1], qw[2], qw[3], qw[4], qw[5], bw_c, bw_u);
+            const TF eu = TF(0.25)*(Ep[ce-1] + Ep[ce] + Ec[ce-1] + Ec[ce]) + visc;
+            du_b = a.rhoh[kb] * eu*((qu[3]-qu[2])*a.dzhi[kb] + (Wc[c]-Wc[c-1])*dxi);
+            const TF ev = TF(0.25)*(Ep[ce-EW] + Ep[ce] + Ec[ce-EW] + Ec[ce]) + visc;
+            dv_b = a.rhoh[kb] * ev*((qv[3]-qv[2])*a.dzhi[kb] + (Wc[c]-Wc[c-PW])*dyi);
+            dw_b = a.rho[kb-1] * (Ep[ce] + visc)*(qw[3]-qw[2])*a.dzi[kb-1];
+            const TF es = TF(0.5)*(Ep[ce]+Ec[ce]) * a.tPr_i + a.svisc;
+            ds_b = a.rhoh[kb] * es*(qs[3]-qs[2])*a.dzhi[kb];
+        }
 
         for (int k = kb; k < ke; ++k)
         {
             const bool more = k + 1 < ke;
             if (more)
             {
-                fetch_plane(ru, a.u, k+1, i0, j0, H, PW, PN, g, tid);
-                fetch_plane(rv, a.v, k+1, i0, j0, H, PW, PN, g, tid);
-                fetch_plane(rw, a.w, k+2, i0, j0, H, PW, PN, g, tid);
-                fetch_plane(rs, a.s, k+1, i0, j0, H, PW, PN, g, tid);
-                fetch_plane(re, a.evisc, k+2, i0, j0, 1, EW, EN, g, tid);
+                fetch_plane(ru, a.u, k+1, o3, g);
+                fetch_plane(rv, a.v, k+1, o3, g);
+                fetch_plane(rw, a.w, k+2, o3, g);
+                fetch_plane(rs, a.s, k+1, o3, g);
+                fetch_plane(re, a.evisc, k+2, o1, g);
             }
             __syncthreads();
 
             if (active)
             {
                 const int ijk = ij + k*kk;
-                const int lvl = level_class_c(k, g.kstart, g.kend);
                 const TF rhok = a.rho[k], rhohk = a.rhoh[k], rhohkp = a.rhoh[k+1];
                 const TF dzik = a.dzi[k], dzhik = a.dzhi[k], dzhikp = a.dzhi[k+1];
 
@@ -173,22 +256,24 @@ namespace
                     TF t = a.ut[ijk];
                     t += hsum(hflux_2i5(i2(Uc[c], Uc[c+1]), i2(Uc[c-1], Uc[c]), Uc, c, 1, dxi),
                               hflux_2i5(i2(Vc[c-1+PW], Vc[c+PW]), i2(Vc[c-1], Vc[c]), Uc, c, PW, dyi));
-                    t += vflux_2i5_c(lvl, i2(Wn[c-1], Wn[c]), i2(Wc[c-1], Wc[c]),
-                                     rhohkp, rhohk, rhok, dzik, qu + 3, 0, 1);
+                    TF tc, tu;
+                    vface_c(k+1, g.kstart, g.kend, rhohkp, i2(Wn[c-1], Wn[c]),
+                            qu[1], qu[2], qu[3], qu[4], qu[5], qu[6], tc, tu);
+                    t += - ( tc - bu_c ) / rhok * dzik + ( tu - bu_u ) / rhok * dzik;
+                    bu_c = tc; bu_u = tu;
                     const TF evisce = Ec[ce] + visc;
                     const TF eviscw = Ec[ce-1] + visc;
                     const TF eviscn = TF(0.25)*(Ec[ce-1] + Ec[ce] + Ec[ce-1+EW] + Ec[ce+EW]) + visc;
                     const TF eviscs = TF(0.25)*(Ec[ce-1-EW] + Ec[ce-EW] + Ec[ce-1] + Ec[ce]) + visc;
                     const TF evisct = TF(0.25)*(Ec[ce-1] + Ec[ce] + En[ce-1] + En[ce]) + visc;
-                    const TF eviscb = TF(0.25)*(Ep[ce-1] + Ep[ce] + Ec[ce-1] + Ec[ce]) + visc;
                     const TF hor =
                              + ( evisce*(Uc[c+1]-Uc[c  ])*dxi
                                - eviscw*(Uc[c  ]-Uc[c-1])*dxi ) * TF(2.)*dxi
                              + ( eviscn*((Uc[c+PW]-Uc[c   ])*dyi + (Vc[c+PW]-Vc[c-1+PW])*dxi)
                                - eviscs*((Uc[c   ]-Uc[c-PW])*dyi + (Vc[c   ]-Vc[c-1   ])*dxi) ) * dyi;
-                    const TF ver =
-                             ( rhohkp * evisct*((qu[4]-qu[3])*dzhikp + (Wn[c]-Wn[c-1])*dxi)
-                             - rhohk  * eviscb*((qu[3]-qu[2])*dzhik  + (Wc[c]-Wc[c-1])*dxi) ) / rhok * dzik;
+                    const TF dt = rhohkp * evisct*((qu[4]-qu[3])*dzhikp + (Wn[c]-Wn[c-1])*dxi);
+                    const TF ver = ( dt - du_b ) / rhok * dzik;
+                    du_b = dt;
                     t += hor + ver;
                     a.ut[ijk] = t;
                 }
@@ -197,70 +282,78 @@ namespace
                     TF t = a.vt[ijk];
                     t += hsum(hflux_2i5(i2(Uc[c+1-PW], Uc[c+1]), i2(Uc[c-PW], Uc[c]), Vc, c, 1, dxi),
                               hflux_2i5(i2(Vc[c], Vc[c+PW]), i2(Vc[c-PW], Vc[c]), Vc, c, PW, dyi));
-                    t += vflux_2i5_c(lvl, i2(Wn[c-PW], Wn[c]), i2(Wc[c-PW], Wc[c]),
-                                     rhohkp, rhohk, rhok, dzik, qv + 3, 0, 1);
+                    TF tc, tu;
+                    vface_c(k+1, g.kstart, g.kend, rhohkp, i2(Wn[c-PW], Wn[c]),
+                            qv[1], qv[2], qv[3], qv[4], qv[5], qv[6], tc, tu);
+                    t += - ( tc - bv_c ) / rhok * dzik + ( tu - bv_u ) / rhok * dzik;
+                    bv_c = tc; bv_u = tu;
                     const TF evisce = TF(0.25)*(Ec[ce-EW] + Ec[ce] + Ec[ce+1-EW] + Ec[ce+1]) + visc;
                     const TF eviscw = TF(0.25)*(Ec[ce-1-EW] + Ec[ce-1] + Ec[ce-EW] + Ec[ce]) + visc;
                     const TF eviscn = Ec[ce] + visc;
                     const TF eviscs = Ec[ce-EW] + visc;
                     const TF evisct = TF(0.25)*(Ec[ce-EW] + Ec[ce] + En[ce-EW] + En[ce]) + visc;
-                    const TF eviscb = TF(0.25)*(Ep[ce-EW] + Ep[ce] + Ec[ce-EW] + Ec[ce]) + visc;
                     const TF hor =
                              + ( evisce*((Vc[c+1]-Vc[c  ])*dxi + (Uc[c+1]-Uc[c+1-PW])*dyi)
                                - eviscw*((Vc[c  ]-Vc[c-1])*dxi + (Uc[c  ]-Uc[c  -PW])*dyi) ) * dxi
                              + ( eviscn*(Vc[c+PW]-Vc[c   ])*dyi
                                - eviscs*(Vc[c   ]-Vc[c-PW])*dyi ) * TF(2.)*dyi;
-                    const TF ver =
-                             ( rhohkp * evisct*((qv[4]-qv[3])*dzhikp + (Wn[c]-Wn[c-PW])*dyi)
-                             - rhohk  * eviscb*((qv[3]-qv[2])*dzhik  + (Wc[c]-Wc[c-PW])*dyi) ) / rhok * dzik;
+                    const TF dt = rhohkp * evisct*((qv[4]-qv[3])*dzhikp + (Wn[c]-Wn[c-PW])*dyi);
+                    const TF ver = ( dt - dv_b ) / rhok * dzik;
+                    dv_b = dt;
                     t += hor + ver;
                     a.vt[ijk] = t;
                 }
                 // ---------------------------------------------------------- w
-                if (k > g.kstart)
                 {
-                    TF t = a.wt[ijk];
-                    if (a.buoyancy)
-                        t += TF(grav)/a.threfh[k] * (i2(qs[2], qs[3]) - a.threfh_rel[k]);
-                    t += hsum(hflux_2i5(i2(Up[c+1], Uc[c+1]), i2(Up[c], Uc[c]), Wc, c, 1, dxi),
-                              hflux_2i5(i2(Vp[c+PW], Vc[c+PW]), i2(Vp[c], Vc[c]), Wc, c, PW, dyi));
-                    t += vflux_2i5_w(level_class_w(k, g.kstart, g.kend), a.rho[k], a.rho[k-1], rhohk, dzhik,
-                                     qw + 3, 0, 1);
-                    const TF evisce = TF(0.25)*(Ep[ce] + Ec[ce] + Ep[ce+1] + Ec[ce+1]) + visc;
-                    const TF eviscw = TF(0.25)*(Ep[ce-1] + Ec[ce-1] + Ep[ce] + Ec[ce]) + visc;
-                    const TF eviscn = TF(0.25)*(Ep[ce] + Ec[ce] + Ep[ce+EW] + Ec[ce+EW]) + visc;
-                    const TF eviscs = TF(0.25)*(Ep[ce-EW] + Ec[ce-EW] + Ep[ce] + Ec[ce]) + visc;
-                    const TF evisct = Ec[ce] + visc;
-                    const TF eviscb = Ep[ce] + visc;
-                    t += + ( evisce*((Wc[c+1]-Wc[c  ])*dxi + (Uc[c+1]-Up[c+1])*dzhik)
-                           - eviscw*((Wc[c  ]-Wc[c-1])*dxi + (Uc[c  ]-Up[c  ])*dzhik) ) * dxi
-                         + ( eviscn*((Wc[c+PW]-Wc[c   ])*dyi + (Vc[c+PW]-Vp[c+PW])*dzhik)
-                           - eviscs*((Wc[c   ]-Wc[c-PW])*dyi + (Vc[c   ]-Vp[c   ])*dzhik) ) * dyi
-                         + ( a.rho[k  ] * evisct*(qw[4]-qw[3])*dzik
-                           - a.rho[k-1] * eviscb*(qw[3]-qw[2])*a.dzi[k-1] ) / rhohk * TF(2.)*dzhik;
-                    a.wt[ijk] = t;
+                    // faces at the cell centres k (top) and k-1 (bottom, carried)
+                    TF tc, tu;
+                    vface_w(k, g.kstart, g.kend, rhok, qw[1], qw[2], qw[3], qw[4], qw[5], qw[6], tc, tu);
+                    const TF dt = rhok * (Ec[ce] + visc)*(qw[4]-qw[3])*dzik;
+                    if (k > g.kstart)
+                    {
+                        TF t = a.wt[ijk];
+                        if (a.buoyancy)
+                            t += TF(grav)/a.threfh[k] * (i2(qs[2], qs[3]) - a.threfh_rel[k]);
+                        t += hsum(hflux_2i5(i2(Up[c+1], Uc[c+1]), i2(Up[c], Uc[c]), Wc, c, 1, dxi),
+                                  hflux_2i5(i2(Vp[c+PW], Vc[c+PW]), i2(Vp[c], Vc[c]), Wc, c, PW, dyi));
+                        t += - ( tc - bw_c ) / rhohk * dzhik + ( tu - bw_u ) / rhohk * dzhik;
+                        const TF evisce = TF(0.25)*(Ep[ce] + Ec[ce] + Ep[ce+1] + Ec[ce+1]) + visc;
+                        const TF eviscw = TF(0.25)*(Ep[ce-1] + Ec[ce-1] + Ep[ce] + Ec[ce]) + visc;
+                        const TF eviscn = TF(0.25)*(Ep[ce] + Ec[ce] + Ep[ce+EW] + Ec[ce+EW]) + visc;
+                        const TF eviscs = TF(0.25)*(Ep[ce-EW] + Ec[ce-EW] + Ep[ce] + Ec[ce]) + visc;
+                        t += + ( evisce*((Wc[c+1]-Wc[c  ])*dxi + (Uc[c+1]-Up[c+1])*dzhik)
+                               - eviscw*((Wc[c  ]-Wc[c-1])*dxi + (Uc[c  ]-Up[c  ])*dzhik) ) * dxi
+                             + ( eviscn*((Wc[c+PW]-Wc[c   ])*dyi + (Vc[c+PW]-Vp[c+PW])*dzhik)
+                               - eviscs*((Wc[c   ]-Wc[c-PW])*dyi + (Vc[c   ]-Vp[c   ])*dzhik) ) * dyi
+                             + ( dt - dw_b ) / rhohk * TF(2.)*dzhik;
+                        a.wt[ijk] = t;
+                    }
+                    bw_c = tc; bw_u = tu; dw_b = dt;
                 }
                 // ---------------------------------------------------------- s
                 {
                     TF t = a.st[ijk];
                     t += hsum(hflux_2i5(Uc[c+1], Uc[c], Sc, c, 1, dxi),
                               hflux_2i5(Vc[c+PW], Vc[c], Sc, c, PW, dyi));
-                    t += vflux_2i5_c(lvl, Wn[c], Wc[c], rhohkp, rhohk, rhok, dzik, qs + 3, 0, 1);
+                    TF tc, tu;
+                    vface_c(k+1, g.kstart, g.kend, rhohkp, Wn[c],
+                            qs[1], qs[2], qs[3], qs[4], qs[5], qs[6], tc, tu);
+                    t += - ( tc - bs_c ) / rhok * dzik + ( tu - bs_u ) / rhok * dzik;
+                    bs_c = tc; bs_u = tu;
                     const TF ti = a.tPr_i, sv = a.svisc;
                     const TF evisce = TF(0.5)*(Ec[ce   ]+Ec[ce+1 ]) * ti + sv;
                     const TF eviscw = TF(0.5)*(Ec[ce-1 ]+Ec[ce   ]) * ti + sv;
                     const TF eviscn = TF(0.5)*(Ec[ce   ]+Ec[ce+EW]) * ti + sv;
                     const TF eviscs = TF(0.5)*(Ec[ce-EW]+Ec[ce   ]) * ti + sv;
                     const TF evisct = TF(0.5)*(Ec[ce   ]+En[ce   ]) * ti + sv;
-                    const TF eviscb = TF(0.5)*(Ep[ce   ]+Ec[ce   ]) * ti + sv;
                     const TF hor =
                              + ( evisce*(Sc[c+1]-Sc[c  ])
                                - eviscw*(Sc[c  ]-Sc[c-1]) ) * a.dxidxi
                              + ( eviscn*(Sc[c+PW]-Sc[c   ])
                                - eviscs*(Sc[c   ]-Sc[c-PW]) ) * a.dyidyi;
-                    const TF ver =
-                             ( rhohkp * evisct*(qs[4]-qs[3])*dzhikp
-                             - rhohk  * eviscb*(qs[3]-qs[2])*dzhik ) / rhok * dzik;
+                    const TF dt = rhohkp * evisct*(qs[4]-qs[3])*dzhikp;
+                    const TF ver = ( dt - ds_b ) / rhok * dzik;
+                    ds_b = dt;
                     t += hor + ver;
                     a.st[ijk] = t;
                 }
