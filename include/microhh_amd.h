@@ -146,6 +146,14 @@ int  mhh_advec_2_cfl(mhh_ctx* ctx, double dt, double* cfl);
 /* Diff_smag2::exec_viscosity (src/diff_smag2.cxx:473-607): strain2, N2, evisc,
  * wall mirror, cyclic. */
 int  mhh_diff_smag2_exec_viscosity(mhh_ctx* ctx, const mhh_smag_params* par);
+/* Diff_smag2::exec_viscosity followed by the two time-step inputs of
+ * Model::set_time_step (src/model.cxx:764-786) in the same pass: the 2i5 CFL
+ * per unit time step, max(|u|dxi + |v|dyi + |w|dzi) of Advec_2i5::get_cfl
+ * (src/advec_2i5.cxx:954-971, times dt gives the CFL number), and dnmul of
+ * Diff_smag2::get_time_limit (include/diff_kernels.h:486-511).  Default
+ * boundary (no surface model) and igc = jgc >= 3 only. */
+int  mhh_diff_smag2_exec_viscosity_limits(mhh_ctx* ctx, const mhh_smag_params* par,
+                                          double* cfl_rate, double* dnmul);
 /* Diff_smag2::exec (src/diff_smag2.cxx:380-471). */
 int  mhh_diff_smag2_exec(mhh_ctx* ctx, const mhh_smag_params* par);
 /* Diff_smag2::get_dn without the dt factor: calc_dnmul (include/diff_kernels.h:486-511). */

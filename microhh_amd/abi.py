@@ -71,6 +71,7 @@ SIGNATURES = {
     "mhh_advec_2_exec": (C.c_int, [VP]),
     "mhh_advec_2_cfl": (C.c_int, [VP, C.c_double, DP]),
     "mhh_diff_smag2_exec_viscosity": (C.c_int, [VP, C.POINTER(SmagParams)]),
+    "mhh_diff_smag2_exec_viscosity_limits": (C.c_int, [VP, C.POINTER(SmagParams), DP, DP]),
     "mhh_diff_smag2_exec": (C.c_int, [VP, C.POINTER(SmagParams)]),
     "mhh_diff_smag2_dnmul": (C.c_int, [VP, C.c_double, DP]),
     "mhh_diff_2_exec": (C.c_int, [VP, C.c_double, DP]),

@@ -116,15 +116,20 @@ namespace mhh
         if (stop) (void)hipEventRecord(stop, c->stream);
     }
 
-    int reduce_max_partials(mhh_ctx* ctx, int npartials, double* out)
+    int reduce_max_partials_at(mhh_ctx* ctx, double* base, int npartials, double* out)
     {
-        fold_max<<<1, 256, 0, ctx->stream>>>(ctx->red, npartials);
+        fold_max<<<1, 256, 0, ctx->stream>>>(base, npartials);
         MHH_CHECK(check_launch("fold_max"));
-        MHH_CHECK(check_hip(hipMemcpyAsync(ctx->h_red, ctx->red, sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+        MHH_CHECK(check_hip(hipMemcpyAsync(ctx->h_red, base, sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
                             "hipMemcpyAsync(reduction)"));
         MHH_CHECK(check_hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize"));
         *out = *ctx->h_red;
         return 0;
+    }
+
+    int reduce_max_partials(mhh_ctx* ctx, int npartials, double* out)
+    {
+        return reduce_max_partials_at(ctx, ctx->red, npartials, out);
     }
 }
 

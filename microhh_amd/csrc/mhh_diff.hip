@@ -170,6 +170,218 @@ namespace
         a.evisc[ij + k*g.ijcells] = evisc_point<TF, sfc, thermo, sw_mason>(a, g, ij, k);
     }
 
+    // ---- plane-ring viscosity kernel (no surface model) ----
+    // A block owns a 64 x 4 column tile and marches up a chunk of levels with
+    // u, v at k-1, k, k+1 and w at k, k+1 in LDS (halo 1), so every plane is
+    // read from HBM once per chunk; theta comes from a register queue.  With
+    // `limits` it also returns the block maxima of the 2i5 CFL rate
+    // (src/advec_2i5.cxx:59-148, without the dt factor) and of dnmul
+    // (include/diff_kernels.h:486-511) for the time step.
+    constexpr int VTX = 64, VTY = 4;
+    constexpr int VPW = VTX + 2, VPH = VTY + 2, VPN = VPW*VPH;
+    constexpr int VKCHUNK = 64;
+    constexpr int VNQ = (VPN + VTX*VTY - 1)/(VTX*VTY);
+
+    template<typename TF>
+    __device__ __forceinline__ TF strain2_planes(const TF* __restrict__ Um, const TF* __restrict__ Uc,
+                                                 const TF* __restrict__ Up, const TF* __restrict__ Vm,
+                                                 const TF* __restrict__ Vc, const TF* __restrict__ Vp,
+                                                 const TF* __restrict__ Wc, const TF* __restrict__ Wp,
+                                                 int c, TF dxi, TF dyi, TF dzik, TF dzhik, TF dzhikp)
+    {
+        // strain2_interior with u, v at k-1/k/k+1 and w at k/k+1 from LDS planes
+        // (ii = 1, jj = VPW); same terms in the same order.
+        constexpr int ii = 1, jj = VPW;
+        TF s2 = TF(2.)*(
+               + sq((Uc[c+ii]-Uc[c])*dxi)
+               + sq((Vc[c+jj]-Vc[c])*dyi)
+               + sq((Wp[c]-Wc[c])*dzik)
+               + TF(0.125)*sq((Uc[c      ]-Uc[c   -jj])*dyi  + (Vc[c      ]-Vc[c-ii   ])*dxi)
+               + TF(0.125)*sq((Uc[c+ii   ]-Uc[c+ii-jj])*dyi  + (Vc[c+ii   ]-Vc[c      ])*dxi)
+               + TF(0.125)*sq((Uc[c   +jj]-Uc[c      ])*dyi  + (Vc[c   +jj]-Vc[c-ii+jj])*dxi)
+               + TF(0.125)*sq((Uc[c+ii+jj]-Uc[c+ii   ])*dyi  + (Vc[c+ii+jj]-Vc[c   +jj])*dxi)
+               + TF(0.125)*sq((Uc[c      ]-Um[c      ])*dzhik  + (Wc[c      ]-Wc[c-ii   ])*dxi)
+               + TF(0.125)*sq((Uc[c+ii   ]-Um[c+ii   ])*dzhik  + (Wc[c+ii   ]-Wc[c      ])*dxi)
+               + TF(0.125)*sq((Up[c      ]-Uc[c      ])*dzhikp + (Wp[c      ]-Wp[c-ii   ])*dxi)
+               + TF(0.125)*sq((Up[c+ii   ]-Uc[c+ii   ])*dzhikp + (Wp[c+ii   ]-Wp[c      ])*dxi)
+               + TF(0.125)*sq((Vc[c      ]-Vm[c      ])*dzhik  + (Wc[c      ]-Wc[c   -jj])*dyi)
+               + TF(0.125)*sq((Vc[c+jj   ]-Vm[c+jj   ])*dzhik  + (Wc[c+jj   ]-Wc[c      ])*dyi)
+               + TF(0.125)*sq((Vp[c      ]-Vc[c      ])*dzhikp + (Wp[c      ]-Wp[c   -jj])*dyi)
+               + TF(0.125)*sq((Vp[c+jj   ]-Vc[c+jj   ])*dzhikp + (Wp[c+jj   ]-Wp[c      ])*dyi) );
+        s2 += TF(dsmall);
+        return s2;
+    }
+
+    template<typename TF>
+    __device__ __forceinline__ void vplane_load(TF* __restrict__ dst, const TF* __restrict__ src, int k,
+                                                const int (&o)[VNQ], int tid, const Gd& g)
+    {
+        const TF* __restrict__ sk = src + static_cast<size_t>(k)*g.ijcells;
+        #pragma unroll
+        for (int q = 0; q < VNQ; ++q)
+        {
+            const int idx = tid + q*VTX*VTY;
+            if (idx < VPN) dst[idx] = o[q] >= 0 ? sk[o[q]] : TF(0);
+        }
+    }
+
+    template<typename TF>
+    __device__ __forceinline__ void vplane_fetch(TF (&r)[VNQ], const TF* __restrict__ src, int k,
+                                                 const int (&o)[VNQ], const Gd& g)
+    {
+        const TF* __restrict__ sk = src + static_cast<size_t>(k)*g.ijcells;
+        #pragma unroll
+        for (int q = 0; q < VNQ; ++q) r[q] = o[q] >= 0 ? sk[o[q]] : TF(0);
+    }
+
+    template<typename TF>
+    __device__ __forceinline__ void vplane_store(TF* __restrict__ dst, const TF (&r)[VNQ], int tid)
+    {
+        #pragma unroll
+        for (int q = 0; q < VNQ; ++q)
+        {
+            const int idx = tid + q*VTX*VTY;
+            if (idx < VPN) dst[idx] = r[q];
+        }
+    }
+
+    template<typename TF, bool thermo, bool limits>
+    __global__ void __launch_bounds__(VTX*VTY) visc_tile_g(Visc_args<TF> a, Gd g, int nti, int ntj, int nkc,
+                                                           double* __restrict__ red_cfl, double* __restrict__ red_dn,
+                                                           TF tPrfac_i, TF dxidxi, TF dyidyi)
+    {
+        __shared__ TF lds[8*VPN];
+        __shared__ double smc[limits ? VTX*VTY : 1], smd[limits ? VTX*VTY : 1];
+        const int tid = threadIdx.x + VTX*threadIdx.y;
+        // XCD-aware tile order, as in the fused tendency kernel
+        const int nb = gridDim.x;
+        const int tile = (blockIdx.x % 8)*(nb/8) + blockIdx.x/8;
+        if (tile >= nti*ntj*nkc)
+        {
+            if (limits && tid == 0) { red_cfl[1 + blockIdx.x] = 0.; red_dn[1 + blockIdx.x] = 0.; }
+            return;
+        }
+        const int ti = tile % nti, tj = (tile / nti) % ntj, kc = tile / (nti*ntj);
+        const int i0 = g.istart + ti*VTX, j0 = g.jstart + tj*VTY;
+        const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+        const bool active = i < g.iend && j < g.jend;
+        const int kb = g.kstart + kc*VKCHUNK, ke = min(kb + VKCHUNK, g.kend);
+        const int ij = i + j*g.icells, kk = g.ijcells;
+
+        int o[VNQ];
+        #pragma unroll
+        for (int q = 0; q < VNQ; ++q)
+        {
+            const int idx = tid + q*VTX*VTY;
+            const int pj = idx / VPW, pi = idx - pj*VPW;
+            const int gi = i0 - 1 + pi, gj = j0 - 1 + pj;
+            o[q] = (idx < VPN && gi < g.icells && gj < g.jcells) ? gi + gj*g.icells : -1;
+        }
+        TF* Um = lds;        TF* Uc = Um + VPN;   TF* Up = Uc + VPN;
+        TF* Vm = Up + VPN;   TF* Vc = Vm + VPN;   TF* Vp = Vc + VPN;
+        TF* Wc = Vp + VPN;   TF* Wp = Wc + VPN;
+        vplane_load(Um, a.u, kb-1, o, tid, g); vplane_load(Uc, a.u, kb, o, tid, g); vplane_load(Up, a.u, kb+1, o, tid, g);
+        vplane_load(Vm, a.v, kb-1, o, tid, g); vplane_load(Vc, a.v, kb, o, tid, g); vplane_load(Vp, a.v, kb+1, o, tid, g);
+        vplane_load(Wc, a.w, kb, o, tid, g);   vplane_load(Wp, a.w, kb+1, o, tid, g);
+
+        // theta at k-1, k+1 of the own column (N2, src/thermo_dry.cxx:66-79)
+        TF t0 = TF(0), t1 = TF(0), t2 = TF(0);
+        if (thermo && active) { t0 = a.th[ij + (kb-1)*kk]; t1 = a.th[ij + kb*kk]; t2 = a.th[ij + (kb+1)*kk]; }
+        // van Driest wall friction velocities of the column (neutral case)
+        TF u_tau_bot = TF(0), u_tau_top = TF(0);
+        if (!thermo && active)
+        {
+            const TF* __restrict__ u = a.u; const TF* __restrict__ v = a.v;
+            const int kbot = ij + g.kstart*kk, ktop = ij + g.kend*kk;
+            u_tau_bot = pow(sq(a.visc*(u[kbot] - u[kbot-kk])*a.dzhi[g.kstart])
+                          + sq(a.visc*(v[kbot] - v[kbot-kk])*a.dzhi[g.kstart]), TF(0.25));
+            u_tau_top = pow(sq(a.visc*(u[ktop] - u[ktop-kk])*a.dzhi[g.kend])
+                          + sq(a.visc*(v[ktop] - v[ktop-kk])*a.dzhi[g.kend]), TF(0.25));
+        }
+        const int c = (threadIdx.y + 1)*VPW + (threadIdx.x + 1);
+        double cm = 0., dm = 0.;
+        TF ru[VNQ], rv[VNQ], rw[VNQ];
+        TF tn = TF(0);
+
+        for (int k = kb; k < ke; ++k)
+        {
+            const bool more = k + 1 < ke;
+            if (more)
+            {
+                vplane_fetch(ru, a.u, k+2, o, g);
+                vplane_fetch(rv, a.v, k+2, o, g);
+                vplane_fetch(rw, a.w, k+2, o, g);
+                if (thermo && active) tn = a.th[ij + (k+2)*kk];
+            }
+            __syncthreads();
+            if (active)
+            {
+                const int ijk = ij + k*kk;
+                const TF s2 = strain2_planes(Um, Uc, Up, Vm, Vc, Vp, Wc, Wp, c, a.dxi, a.dyi,
+                                             a.dzi[k], a.dzhi[k], a.dzhi[k+1]);
+                TF ev;
+                if (thermo)
+                {
+                    const TF num = TF(grav)/a.thref[k]*TF(0.5)*(t2 - t0)*a.dzi[k];
+                    TF RitPrratio = num / s2 / a.tPr;
+                    RitPrratio = fmin(RitPrratio, TF(1.-dsmall));
+                    const TF mlen = a.mlen0[k];
+                    ev = mlen*mlen * sqrt(s2) * sqrt(TF(1.)-RitPrratio);
+                }
+                else
+                {
+                    const TF A_vandriest = TF(26.);
+                    const TF fac_bot = TF(1.) - exp(-(        a.z[k] *u_tau_bot) / (A_vandriest*a.visc));
+                    const TF fac_top = TF(1.) - exp(-((a.zsize-a.z[k])*u_tau_top) / (A_vandriest*a.visc));
+                    const TF fac = fmin(fac_bot, fac_top);
+                    ev = sq(fac * a.mlen0[k]) * sqrt(s2);
+                }
+                a.evisc[ijk] = ev;
+                if (limits)
+                {
+                    dm = fmax(dm, static_cast<double>(fabs(ev*tPrfac_i*(dxidxi + dyidyi + a.dzi[k]*a.dzi[k]))));
+                    // the 2i5 CFL rate as in cfl_g (the horizontal stencils from L2)
+                    const TF* __restrict__ u = a.u; const TF* __restrict__ v = a.v; const TF* __restrict__ w = a.w;
+                    const int ii = 1, jj = g.icells;
+                    TF wpart;
+                    if (k == g.kstart || k == g.kend-1)
+                        wpart = fabs(i2(w[ijk], w[ijk+kk]))*a.dzi[k];
+                    else if (k == g.kstart+1 || k == g.kend-2)
+                        wpart = fabs(i4ws(w[ijk-kk], w[ijk], w[ijk+kk], w[ijk+2*kk]))*a.dzi[k];
+                    else
+                        wpart = fabs(i6ws(w[ijk-2*kk], w[ijk-kk], w[ijk], w[ijk+kk], w[ijk+2*kk], w[ijk+3*kk]))*a.dzi[k];
+                    const TF val = fabs(i6ws(u[ijk-2*ii], u[ijk-ii], u[ijk], u[ijk+ii], u[ijk+2*ii], u[ijk+3*ii]))*a.dxi
+                                 + fabs(i6ws(v[ijk-2*jj], v[ijk-jj], v[ijk], v[ijk+jj], v[ijk+2*jj], v[ijk+3*jj]))*a.dyi
+                                 + wpart;
+                    cm = fmax(cm, static_cast<double>(val));
+                }
+            }
+            __syncthreads();
+            TF* x;
+            x = Um; Um = Uc; Uc = Up; Up = x;
+            x = Vm; Vm = Vc; Vc = Vp; Vp = x;
+            x = Wc; Wc = Wp; Wp = x;
+            if (more)
+            {
+                vplane_store(Up, ru, tid);
+                vplane_store(Vp, rv, tid);
+                vplane_store(Wp, rw, tid);
+            }
+            t0 = t1; t1 = t2; t2 = tn;
+        }
+        if (limits)
+        {
+            smc[tid] = cm; smd[tid] = dm;
+            __syncthreads();
+            for (int st = VTX*VTY/2; st > 0; st >>= 1)
+            {
+                if (tid < st) { smc[tid] = fmax(smc[tid], smc[tid + st]); smd[tid] = fmax(smd[tid], smd[tid + st]); }
+                __syncthreads();
+            }
+            if (tid == 0) { red_cfl[1 + blockIdx.x] = smc[0]; red_dn[1 + blockIdx.x] = smd[0]; }
+        }
+    }
+
     // Mirror evisc over both walls (src/diff_smag2.cxx:193-205) for the interior
     // columns; the cyclic update that follows fills the horizontal ghosts.
     template<typename TF>
@@ -447,8 +659,10 @@ namespace
     template<typename TF>
     const TF* dptr(void* p) { return static_cast<const TF*>(p); }
 
+    // lim: also leave the CFL-rate and dnmul block maxima in the reduction
+    // buffer (c->red[1..*nred] and c->red[*nred+2 ..]); no surface model only.
     template<typename TF>
-    int viscosity(mhh_ctx* c, const Prof<TF>& p, const mhh_smag_params& par)
+    int viscosity(mhh_ctx* c, const Prof<TF>& p, const mhh_smag_params& par, bool lim = false, int* nred = nullptr)
     {
         const Gd& g = c->gd;
         if (par.thermo && (c->f.nscalars < 1 || !p.thref))
@@ -471,8 +685,27 @@ namespace
 
         const dim3 grd = grid3(g, g.kstart, g.kend), blk(BX, BY);
         const bool sfc = par.surface_model != 0, th = par.thermo != 0, ms = par.swmason != 0;
-        if (!sfc && th)        { mhh::Timer _t(c, "evisc"); evisc_g<TF, false, true,  false><<<grd, blk, 0, c->stream>>>(a, g); }
-        else if (!sfc && !th)  { mhh::Timer _t(c, "evisc"); evisc_g<TF, false, false, false><<<grd, blk, 0, c->stream>>>(a, g); }
+        if (!sfc)
+        {
+            // plane-ring kernel; with `limits` also the CFL and dnmul maxima
+            const int nti = (g.itot + VTX - 1)/VTX, ntj = (g.jtot + VTY - 1)/VTY, nkc = (g.ktot + VKCHUNK - 1)/VKCHUNK;
+            const int nblocks = (nti*ntj*nkc + 7)/8*8;
+            double* rc = nullptr; double* rd = nullptr;
+            if (lim)
+            {
+                MHH_CHECK(ensure_reduction(c, 2*nblocks + 1));
+                rc = c->red; rd = c->red + nblocks + 1;
+            }
+            const TF tPrfac_i = TF(1)/std::min(TF(1.), TF(par.tPr));
+            const TF dxidxi = TF(1.)/(TF(g.dx)*TF(g.dx)), dyidyi = TF(1.)/(TF(g.dy)*TF(g.dy));
+            const dim3 vb(VTX, VTY);
+            mhh::Timer _t(c, "evisc");
+            if (th && lim)       visc_tile_g<TF, true,  true ><<<nblocks, vb, 0, c->stream>>>(a, g, nti, ntj, nkc, rc, rd, tPrfac_i, dxidxi, dyidyi);
+            else if (th)         visc_tile_g<TF, true,  false><<<nblocks, vb, 0, c->stream>>>(a, g, nti, ntj, nkc, rc, rd, tPrfac_i, dxidxi, dyidyi);
+            else if (lim)        visc_tile_g<TF, false, true ><<<nblocks, vb, 0, c->stream>>>(a, g, nti, ntj, nkc, rc, rd, tPrfac_i, dxidxi, dyidyi);
+            else                 visc_tile_g<TF, false, false><<<nblocks, vb, 0, c->stream>>>(a, g, nti, ntj, nkc, rc, rd, tPrfac_i, dxidxi, dyidyi);
+            if (lim) *nred = nblocks;
+        }
         else if (th && ms)     { mhh::Timer _t(c, "evisc"); evisc_g<TF, true,  true,  true ><<<grd, blk, 0, c->stream>>>(a, g); }
         else if (th && !ms)    { mhh::Timer _t(c, "evisc"); evisc_g<TF, true,  true,  false><<<grd, blk, 0, c->stream>>>(a, g); }
         else if (ms)           { mhh::Timer _t(c, "evisc"); evisc_g<TF, true,  false, true ><<<grd, blk, 0, c->stream>>>(a, g); }
@@ -574,6 +807,20 @@ int mhh_diff_smag2_exec_viscosity(mhh_ctx* c, const mhh_smag_params* par)
     if (!par) return mhh::fail(MHH_EINVAL, "null smag params");
     if (!c->f.evisc) return mhh::fail(MHH_ESTATE, "evisc field is not bound");
     return c->precision == MHH_FP64 ? viscosity<double>(c, c->pd, *par) : viscosity<float>(c, c->pf, *par);
+}
+
+int mhh_diff_smag2_exec_viscosity_limits(mhh_ctx* c, const mhh_smag_params* par, double* cfl_rate, double* dnmul)
+{
+    MHH_REQUIRE_BOUND(c);
+    if (!par || !cfl_rate || !dnmul) return mhh::fail(MHH_EINVAL, "null argument");
+    if (!c->f.evisc) return mhh::fail(MHH_ESTATE, "evisc field is not bound");
+    if (par->surface_model) return mhh::fail(MHH_EINVAL, "the fused time-step limits cover the default boundary only");
+    if (c->gd.igc < 3 || c->gd.jgc < 3) return mhh::fail(MHH_EINVAL, "the 2i5 CFL needs igc = jgc >= 3");
+    int nb = 0;
+    if (c->precision == MHH_FP64) MHH_CHECK(viscosity<double>(c, c->pd, *par, true, &nb));
+    else MHH_CHECK(viscosity<float>(c, c->pf, *par, true, &nb));
+    MHH_CHECK(mhh::reduce_max_partials_at(c, c->red, nb, cfl_rate));
+    return mhh::reduce_max_partials_at(c, c->red + nb + 1, nb, dnmul);
 }
 
 int mhh_diff_smag2_exec(mhh_ctx* c, const mhh_smag_params* par)

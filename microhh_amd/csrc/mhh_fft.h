@@ -56,6 +56,13 @@ namespace mhh
         return true;
     }
 
+    // LDS layout of rows and of the twiddle table: one unused slot after every
+    // 8 complex values, so that the stride-R stores of the early Stockham
+    // passes and the strided twiddle reads spread over the LDS banks; rows
+    // are an odd number of 16-byte slots apart for the row-crossing loops.
+    __host__ __device__ __forceinline__ int lpad(int x) { return x + (x >> 3); }
+    __host__ __device__ __forceinline__ int row_half(int n) { return (lpad(n - 1) + 1) | 1; }
+
     template<typename TF> struct C2;
     template<> struct C2<double> { using type = double2; };
     template<> struct C2<float> { using type = float2; };
@@ -135,13 +142,13 @@ namespace mhh
         using T = cplx<TF>;
         #pragma unroll
         for (int r = 0; r < R; ++r)
-            v[r] = a[j + r*nr];
+            v[r] = a[lpad(j + r*nr)];
         if (k > 0)
         {
             #pragma unroll
             for (int r = 1; r < R; ++r)
             {
-                T w = tw[r*k*tstride];
+                T w = tw[lpad(r*k*tstride)];
                 if (sign > 0) w.y = -w.y;
                 v[r] = cmul(v[r], w);
             }
@@ -155,7 +162,7 @@ namespace mhh
         const int idxD = (j / Ns)*Ns*R + k;
         #pragma unroll
         for (int r = 0; r < R; ++r)
-            a[idxD + r*Ns] = v[r];
+            a[lpad(idxD + r*Ns)] = v[r];
     }
 
     // Fast engine: in place in row a; tw is the LDS twiddle table of length n.
@@ -232,7 +239,7 @@ namespace mhh
                 T v[FFT_MAXRADIX];
                 for (int r = 0; r < R; ++r)
                 {
-                    T x = a[j + r*nr];
+                    T x = a[lpad(j + r*nr)];
                     if (r > 0 && k > 0)
                     {
                         const double2 w = tw[r*k*tstride];
@@ -256,7 +263,7 @@ namespace mhh
                 }
                 const int idxD = (j / Ns)*Ns*R + k;
                 for (int r = 0; r < R; ++r)
-                    b[idxD + r*Ns] = y[r];
+                    b[lpad(idxD + r*Ns)] = y[r];
             }
             __syncthreads();
             T* t = a; a = b; b = t;
@@ -266,7 +273,8 @@ namespace mhh
     }
 
     // Row FFT through whichever engine the plan allows.  Row layout per block:
-    // fast plans use one buffer of n per row, the fallback two (a, a + n).
+    // fast plans use one padded buffer per row, the fallback two
+    // (a, a + row_half(n)).  Element x of a row lives at index lpad(x).
     template<typename TF>
     __device__ __forceinline__ cplx<TF>* fft_row(cplx<TF>* a, const cplx<TF>* tw_lds, const double2* __restrict__ tw_glob,
                                                  const Plan& plan, int sign, int lane)
@@ -276,7 +284,7 @@ namespace mhh
             fft_row_inplace<TF>(a, tw_lds, plan, sign, lane);
             return a;
         }
-        return fft_row_pingpong<TF>(a, a + plan.n, plan, tw_glob, sign, lane);
+        return fft_row_pingpong<TF>(a, a + row_half(plan.n), plan, tw_glob, sign, lane);
     }
 
     // Copy the twiddle table into LDS (all threads of the block participate).
@@ -287,10 +295,11 @@ namespace mhh
         for (int t = threadIdx.x + blockDim.x*threadIdx.y; t < n; t += nthr)
         {
             cplx<TF> w; w.x = TF(tw[t].x); w.y = TF(tw[t].y);
-            tw_lds[t] = w;
+            tw_lds[lpad(t)] = w;
         }
     }
 
-    // Elements per row buffer (complex) for a plan.
-    __host__ __device__ inline int row_elems(const Plan& p) { return p.fast ? p.n : 2*p.n; }
+    // Elements per row buffer (complex) for a plan, and of the LDS twiddle table.
+    __host__ __device__ inline int row_elems(const Plan& p) { return p.fast ? row_half(p.n) : 2*row_half(p.n); }
+    __host__ __device__ inline int tw_elems(const Plan& p) { return lpad(p.n - 1) + 1; }
 }

@@ -117,6 +117,7 @@ namespace mhh
 
     // Maximum over a device array of per-block partial maxima, returned on the host.
     int reduce_max_partials(mhh_ctx* ctx, int npartials, double* out);
+    int reduce_max_partials_at(mhh_ctx* ctx, double* base, int npartials, double* out);
     int ensure_reduction(mhh_ctx* ctx, int nblocks);
 
     // Brackets the launches issued during its lifetime with HIP events on the

@@ -75,7 +75,7 @@ namespace
     {
         if (kx == 0) { cplx<TF> d; d.x = xa.x; d.y = xb.x; S[0] = d; }
         else if (2*kx == n) { cplx<TF> d; d.x = xa.x; d.y = xb.x; S[1] = d; }
-        else { const int b = stage_base(kx, n); S[b] = xa; S[b+1] = xb; }
+        else { const int b = stage_base(kx, n); S[lpad(b)] = xa; S[lpad(b+1)] = xb; }
     }
 
     // Forward x transform: RB complex rows per block, each packing two real
@@ -89,7 +89,7 @@ namespace
         extern __shared__ unsigned char smem_raw[];
         T* tws = reinterpret_cast<T*>(smem_raw);
         const int n = plan.n, nx = n/2 + 1, M = g.jtot, re = row_elems(plan);
-        T* rows = tws + n;
+        T* rows = tws + tw_elems(plan);
         const int row = threadIdx.y, lane = threadIdx.x;
         const int k = blockIdx.y;
         const int j0 = blockIdx.x*2*RB;              // first real row of the block
@@ -106,7 +106,7 @@ namespace
                 T z;
                 z.x = (ja < M) ? TF(double(Pk[x + ja*n]) - m) : TF(0);
                 z.y = (jb < M) ? TF(double(Pk[x + jb*n]) - m) : TF(0);
-                A[x] = z;
+                A[lpad(x)] = z;
             }
         }
         else
@@ -115,7 +115,7 @@ namespace
                 T z;
                 z.x = (ja < M) ? Pk[x + ja*n] : TF(0);
                 z.y = (jb < M) ? Pk[x + jb*n] : TF(0);
-                A[x] = z;
+                A[lpad(x)] = z;
             }
         __syncthreads();
         T* Z = fft_row<TF>(A, tws, tw, plan, -1, lane);
@@ -129,7 +129,7 @@ namespace
             for (int q = 0; q < MAXQ; ++q)
             {
                 const int kx = lane + 64*q;
-                if (kx < nx) unpack_pair<TF>(Z[kx], Z[(n - kx) % n], xs[q][0], xs[q][1]);
+                if (kx < nx) unpack_pair<TF>(Z[lpad(kx)], Z[lpad((n - kx) % n)], xs[q][0], xs[q][1]);
             }
             __syncthreads();
             #pragma unroll
@@ -142,12 +142,12 @@ namespace
         }
         else
         {
-            soff = (Z == A) ? n : 0;
+            soff = (Z == A) ? row_half(n) : 0;
             T* S = rows + row*re + soff;
             for (int kx = lane; kx < nx; kx += 64)
             {
                 T xa, xb;
-                unpack_pair<TF>(Z[kx], Z[(n - kx) % n], xa, xb);
+                unpack_pair<TF>(Z[lpad(kx)], Z[lpad((n - kx) % n)], xa, xb);
                 stage_store<TF>(S, kx, n, xa, xb);
             }
         }
@@ -166,11 +166,11 @@ namespace
             T val;
             if (kx == 0 || 2*kx == n)
             {
-                const T d = Sr[kx == 0 ? 0 : 1];
+                const T d = Sr[kx == 0 ? 0 : 1];   // lpad(0) = 0, lpad(1) = 1
                 val.x = part ? d.y : d.x; val.y = TF(0);
             }
             else
-                val = Sr[stage_base(kx, n) + part];
+                val = Sr[lpad(stage_base(kx, n) + part)];
             Ck[static_cast<size_t>(kx)*M + j] = val;
         }
     }
@@ -187,19 +187,19 @@ namespace
         const int n = plan.n, re = row_elems(plan);
         const int row = threadIdx.y, lane = threadIdx.x;
         const int r = blockIdx.x*RB + row;
-        T* A = tws + n + row*re;
+        T* A = tws + tw_elems(plan) + row*re;
         if (plan.fast) load_twiddles<TF>(tws, tw, n);
         const bool active = r < nrows;
         T* Cr = C + static_cast<size_t>(active ? r : 0)*n;
         if (active)
-            for (int x = lane; x < n; x += 64) A[x] = Cr[x];
+            for (int x = lane; x < n; x += 64) A[lpad(x)] = Cr[x];
         __syncthreads();
         T* Z = fft_row<TF>(A, tws, tw, plan, sign, lane);
         if (!active) return;
         if (sign < 0)
-            for (int x = lane; x < n; x += 64) Cr[x] = Z[x];
+            for (int x = lane; x < n; x += 64) Cr[x] = Z[lpad(x)];
         else
-            for (int x = lane; x < n; x += 64) { T z = Z[x]; z.x /= TF(n); z.y /= TF(n); Cr[x] = z; }
+            for (int x = lane; x < n; x += 64) { T z = Z[lpad(x)]; z.x /= TF(n); z.y /= TF(n); Cr[x] = z; }
     }
 
     // Tridiagonal solve per (kx, ky) mode (Pres_2::solve + tdma,
@@ -281,7 +281,7 @@ namespace
         extern __shared__ unsigned char smem_raw[];
         T* tws = reinterpret_cast<T*>(smem_raw);
         const int n = plan.n, nx = n/2 + 1, M = g.jtot, re = row_elems(plan);
-        T* rows = tws + n;
+        T* rows = tws + tw_elems(plan);
         const int row = threadIdx.y, lane = threadIdx.x;
         const int k = blockIdx.y;
         const int j0 = blockIdx.x*2*RB;
@@ -292,7 +292,7 @@ namespace
         // Stage the block's spectra packed as in the forward kernel; the DC and
         // Nyquist entries of a real row are real.  The staging buffer is the row
         // buffer itself (in-place engine) or the row's second buffer (fallback).
-        const int soff = plan.fast ? 0 : n;
+        const int soff = plan.fast ? 0 : row_half(n);
         for (int idx = tid; idx < nx*JB; idx += nthr)
         {
             const int kx = idx / JB, jl = idx % JB;
@@ -308,7 +308,7 @@ namespace
                 d[part] = val.x;
             }
             else
-                Sr[stage_base(kx, n) + part] = val;
+                Sr[lpad(stage_base(kx, n) + part)] = val;
         }
         __syncthreads();
         T* A = rows + row*re;
@@ -325,7 +325,7 @@ namespace
             else
             {
                 const int b = stage_base(kx, n);
-                xa = B[b]; xb = B[b+1];
+                xa = B[lpad(b)]; xb = B[lpad(b+1)];
                 if (x > n/2) { xa.y = -xa.y; xb.y = -xb.y; }
             }
             T z; z.x = xa.x - xb.y; z.y = xa.y + xb.x;
@@ -346,11 +346,11 @@ namespace
             for (int q = 0; q < MAXX; ++q)
             {
                 const int x = lane + 64*q;
-                if (x < n) A[x] = zs[q];
+                if (x < n) A[lpad(x)] = zs[q];
             }
         }
         else
-            for (int x = lane; x < n; x += 64) A[x] = zval(x);
+            for (int x = lane; x < n; x += 64) A[lpad(x)] = zval(x);
         __syncthreads();
         T* Z = fft_row<TF>(A, tws, tw, plan, +1, lane);
         const int ja = j0 + 2*row, jb = ja + 1;
@@ -370,7 +370,7 @@ namespace
                 // ghosted column x maps to interior column x - igc, wrapped
                 int xi = x - g.igc;
                 xi = (xi < 0) ? xi + n : (xi >= n ? xi - n : xi);
-                const T z = Z[xi];
+                const T z = Z[lpad(xi)];
                 const TF val = (pr ? z.y : z.x) / TF(n);
                 for (int q = 0; q < nrow; ++q)
                 {
@@ -538,13 +538,14 @@ namespace
     {
         const size_t budget = 65536 / elem;          // complex elements
         const size_t per_row = static_cast<size_t>(row_elems(p));
-        const size_t avail = budget > static_cast<size_t>(p.n) ? budget - p.n : 0;
+        const size_t tw = static_cast<size_t>(tw_elems(p));
+        const size_t avail = budget > tw ? budget - tw : 0;
         return std::max(1, std::min(4, static_cast<int>(avail / per_row)));
     }
 
     size_t lds_bytes(const Plan& p, int RB, size_t elem)
     {
-        return (static_cast<size_t>(p.n) + static_cast<size_t>(RB)*row_elems(p))*elem;
+        return (static_cast<size_t>(tw_elems(p)) + static_cast<size_t>(RB)*row_elems(p))*elem;
     }
 
     template<typename TF>

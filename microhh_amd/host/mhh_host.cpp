@@ -387,7 +387,12 @@ namespace mhh_host
     unsigned long Advec<TF>::get_time_limit(unsigned long idt, double dt)
     {
         // src/advec_2i5.cxx:973-992
-        double cfl = get_cfl(dt);
+        return time_limit_from_cfl(idt, get_cfl(dt));
+    }
+
+    template<typename TF>
+    unsigned long Advec<TF>::time_limit_from_cfl(unsigned long idt, double cfl) const
+    {
         cfl = std::max(cflmin, cfl);
         return idt * cflmax / cfl;
     }
@@ -466,6 +471,12 @@ namespace mhh_host
         // src/diff_smag2.cxx:312-333
         double dnmul;
         check(mhh_diff_smag2_dnmul(this->ctx, par.tPr, &dnmul), "Diff_smag2::get_time_limit");
+        return time_limit_from_dnmul(idt, dt, dnmul);
+    }
+
+    template<typename TF>
+    unsigned long Diff_smag2<TF>::time_limit_from_dnmul(unsigned long idt, double dt, double dnmul) const
+    {
         dnmul = std::max(1.e-9, dnmul);
         return idt * dnmax / (dt * dnmul);
     }
@@ -689,13 +700,41 @@ namespace mhh_host
     }
 
     template<typename TF>
+    void Model<TF>::set_time_step_from(double cfl_rate, double dnmul)
+    {
+        // Model::set_time_step with the CFL and dnmul maxima already at hand
+        // (the fused path computes them in the viscosity pass).
+        if (timeloop->in_substep()) return;
+        auto& ds = static_cast<Diff_smag2<TF>&>(*diff);
+        timeloop->set_time_step_limit();
+        timeloop->set_time_step_limit(advec->time_limit_from_cfl(timeloop->get_idt(), cfl_rate*timeloop->get_dt()));
+        timeloop->set_time_step_limit(ds.time_limit_from_dnmul(timeloop->get_idt(), timeloop->get_dt(), dnmul));
+        if (thermo)
+            timeloop->set_time_step_limit(thermo->get_time_limit(timeloop->get_idt(), timeloop->get_dt()));
+        timeloop->set_time_step();
+    }
+
+    template<typename TF>
     void Model<TF>::substep()
     {
         // The in-scope part of Model::exec (src/model.cxx:368-507).
         boundary->set_prognostic_cyclic_bcs();
         boundary->set_ghost_cells();
-        diff->exec_viscosity();
-        set_time_step();
+        if (fused_tendency && !timeloop->in_substep())
+        {
+            // Diff_smag2::exec_viscosity with the two time-step inputs of the
+            // first substep in the same pass
+            const auto& par = static_cast<Diff_smag2<TF>&>(*diff).params();
+            double cfl_rate, dnmul;
+            check(mhh_diff_smag2_exec_viscosity_limits(ctx_, &par, &cfl_rate, &dnmul),
+                  "mhh_diff_smag2_exec_viscosity_limits");
+            set_time_step_from(cfl_rate, dnmul);
+        }
+        else
+        {
+            diff->exec_viscosity();
+            set_time_step();
+        }
         const double subdt = timeloop->get_sub_time_step();
         if (fused_tendency)
         {

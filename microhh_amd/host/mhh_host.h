@@ -132,6 +132,8 @@ namespace mhh_host
             virtual void exec() = 0;
             virtual double get_cfl(double dt) = 0;
             unsigned long get_time_limit(unsigned long idt, double dt);
+            // The same limit from a CFL number computed elsewhere.
+            unsigned long time_limit_from_cfl(unsigned long idt, double cfl) const;
             static std::unique_ptr<Advec<TF>> factory(mhh_ctx* ctx, Grid<TF>& grid, int swadvec, double cflmax);
 
         protected:
@@ -183,6 +185,8 @@ namespace mhh_host
             void exec() override;
             unsigned long get_time_limit(unsigned long idt, double dt) override;
             double get_dn(double dt) override;
+            // The same limit from a dnmul computed elsewhere.
+            unsigned long time_limit_from_dnmul(unsigned long idt, double dt, double dnmul) const;
             const mhh_smag_params& params() const { return par; }
         private:
             mhh_smag_params par{};
@@ -301,6 +305,7 @@ namespace mhh_host
 
         private:
             void set_time_step();
+            void set_time_step_from(double cfl_rate, double dnmul);
             mhh_case cs;
             bool fused_tendency = false;
             mhh_ctx* ctx_ = nullptr;

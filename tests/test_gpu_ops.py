@@ -340,6 +340,27 @@ def test_pres_rk3_matches_pres_then_rk3(setup, substep):
         assert err < 1e-14, f"{n}: {err:.3e}"
 
 
+@pytest.mark.parametrize("thermo", [1, 0])
+def test_viscosity_limits_match_separate_calls(setup, thermo):
+    """mhh_diff_smag2_exec_viscosity_limits against exec_viscosity followed by
+    the 2i5 CFL and dnmul reductions: same evisc, same maxima."""
+    from microhh_amd import abi
+    p, i, d, g, *_ = setup
+    par = abi.SmagParams(0.23, 1.0 / 3.0, 1.5e-5, (C.c_double * 8)(*([2e-5] * 8)), 1, 0, thermo)
+    _load_common(g, i)
+    cfl_rate, dn = C.c_double(), C.c_double()
+    g.call("mhh_diff_smag2_exec_viscosity_limits", C.byref(par), C.byref(cfl_rate), C.byref(dn))
+    ev_fused = g.get("evisc")
+    _load_common(g, i)
+    g.call("mhh_diff_smag2_exec_viscosity", C.byref(par))
+    cfl1, dn1 = C.c_double(), C.c_double()
+    g.call("mhh_advec_2i5_cfl", 1.0, C.byref(cfl1))
+    g.call("mhh_diff_smag2_dnmul", 1.0 / 3.0, C.byref(dn1))
+    assert normwise(ev_fused, g.get("evisc")) < 1e-15
+    assert abs(cfl_rate.value - cfl1.value) <= 1e-15 * cfl1.value
+    assert abs(dn.value - dn1.value) <= 1e-15 * dn1.value
+
+
 def test_fused_tendency_matches_operator_sequence(setup):
     """mhh_tendency_2i5_smag2 against thermo_dry + advec_2i5 + diff_smag2 run
     one after another through the ABI, then against the oracle chain."""
