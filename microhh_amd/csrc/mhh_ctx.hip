@@ -24,7 +24,8 @@ namespace
     template<typename TF>
     void free_prof(mhh::Prof<TF>& p)
     {
-        TF* all[] = {p.z, p.zh, p.dz, p.dzh, p.dzi, p.dzhi, p.rhoref, p.rhorefh, p.thref, p.threfh, p.threfh_rel};
+        TF* all[] = {p.z, p.zh, p.dz, p.dzh, p.dzi, p.dzhi, p.rhoref, p.rhorefh, p.thref, p.threfh, p.threfh_rel,
+                     p.cfac, p.wfac, p.gthi};
         for (TF* a : all) if (a) (void)hipFree(a);
         p = mhh::Prof<TF>();
     }
@@ -40,7 +41,15 @@ namespace
         MHH_CHECK(upload(&p.dzhi, c->h_dzhi));
         MHH_CHECK(upload(&p.rhoref, c->h_rhoref));
         MHH_CHECK(upload(&p.rhorefh, c->h_rhorefh));
-        return 0;
+        // per-level factors that replace divisions in the fused tendency kernel
+        std::vector<double> cf(c->h_dzi.size()), wf(c->h_dzhi.size());
+        for (size_t k = 0; k < cf.size(); ++k)
+        {
+            cf[k] = c->h_rhoref[k] != 0. ? c->h_dzi[k]/c->h_rhoref[k] : 0.;
+            wf[k] = c->h_rhorefh[k] != 0. ? c->h_dzhi[k]/c->h_rhorefh[k] : 0.;
+        }
+        MHH_CHECK(upload(&p.cfac, cf));
+        return upload(&p.wfac, wf);
     }
 
     // Second stage of a max-reduction: one block folds the partials into red[0].
@@ -139,12 +148,14 @@ namespace
     template<typename TF>
     int upload_base_state(mhh_ctx* c, mhh::Prof<TF>& p)
     {
-        if (p.thref) { (void)hipFree(p.thref); (void)hipFree(p.threfh); (void)hipFree(p.threfh_rel); }
-        p.thref = p.threfh = p.threfh_rel = nullptr;
-        std::vector<double> rel(c->h_threfh);
+        if (p.thref) { (void)hipFree(p.thref); (void)hipFree(p.threfh); (void)hipFree(p.threfh_rel); (void)hipFree(p.gthi); }
+        p.thref = p.threfh = p.threfh_rel = p.gthi = nullptr;
+        std::vector<double> rel(c->h_threfh), gi(c->h_threfh.size());
         for (double& x : rel) x -= c->s_offset[0];
+        for (size_t k = 0; k < gi.size(); ++k) gi[k] = c->h_threfh[k] != 0. ? 9.81/c->h_threfh[k] : 0.;
         MHH_CHECK(upload(&p.thref, c->h_thref));
         MHH_CHECK(upload(&p.threfh, c->h_threfh));
+        MHH_CHECK(upload(&p.gthi, gi));
         return upload(&p.threfh_rel, rel);
     }
 

@@ -31,6 +31,9 @@ namespace mhh
         TF* dzi = nullptr; TF* dzhi = nullptr; TF* rhoref = nullptr; TF* rhorefh = nullptr;
         TF* thref = nullptr; TF* threfh = nullptr;
         TF* threfh_rel = nullptr;   // threfh - offset of scalar 0 (the buoyancy difference)
+        TF* cfac = nullptr;         // dzi/rhoref: vertical flux divergence factor of cell-centred fields
+        TF* wfac = nullptr;         // dzhi/rhorefh: the same for w
+        TF* gthi = nullptr;         // grav/threfh
     };
 
     // Pressure-solver tables (Pres_2::set_values, src/pres_2.cxx:124-153) and

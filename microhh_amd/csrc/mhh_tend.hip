@@ -36,6 +36,7 @@ namespace
         const TF* __restrict__ dzi; const TF* __restrict__ dzhi;
         const TF* __restrict__ rho; const TF* __restrict__ rhoh; const TF* __restrict__ threfh;
         const TF* __restrict__ threfh_rel;
+        const TF* __restrict__ cfac; const TF* __restrict__ wfac; const TF* __restrict__ gthi;
         TF dxi, dyi, dxidxi, dyidyi, visc, svisc, tPr_i;
         int buoyancy;
         int nti, ntj, nkc;
@@ -249,7 +250,11 @@ namespace
             {
                 const int ijk = ij + k*kk;
                 const TF rhok = a.rho[k], rhohk = a.rhoh[k], rhohkp = a.rhoh[k+1];
-                const TF dzik = a.dzi[k], dzhik = a.dzhi[k], dzhikp = a.dzhi[k+1];
+                const TF dzhik = a.dzhi[k], dzhikp = a.dzhi[k+1];
+                const TF dzik = a.dzi[k];
+                // dzi/rhoref and dzhi/rhorefh: the divisions of the reference's
+                // "/ rhoref[k] * dzi[k]" taken out of the per-point work
+                const TF cf = a.cfac[k], wf = a.wfac[k];
 
                 // ---------------------------------------------------------- u
                 {
@@ -259,7 +264,7 @@ namespace
                     TF tc, tu;
                     vface_c(k+1, g.kstart, g.kend, rhohkp, i2(Wn[c-1], Wn[c]),
                             qu[1], qu[2], qu[3], qu[4], qu[5], qu[6], tc, tu);
-                    t += - ( tc - bu_c ) / rhok * dzik + ( tu - bu_u ) / rhok * dzik;
+                    t += - ( tc - bu_c ) * cf + ( tu - bu_u ) * cf;
                     bu_c = tc; bu_u = tu;
                     const TF evisce = Ec[ce] + visc;
                     const TF eviscw = Ec[ce-1] + visc;
@@ -272,7 +277,7 @@ namespace
                              + ( eviscn*((Uc[c+PW]-Uc[c   ])*dyi + (Vc[c+PW]-Vc[c-1+PW])*dxi)
                                - eviscs*((Uc[c   ]-Uc[c-PW])*dyi + (Vc[c   ]-Vc[c-1   ])*dxi) ) * dyi;
                     const TF dt = rhohkp * evisct*((qu[4]-qu[3])*dzhikp + (Wn[c]-Wn[c-1])*dxi);
-                    const TF ver = ( dt - du_b ) / rhok * dzik;
+                    const TF ver = ( dt - du_b ) * cf;
                     du_b = dt;
                     t += hor + ver;
                     a.ut[ijk] = t;
@@ -285,7 +290,7 @@ namespace
                     TF tc, tu;
                     vface_c(k+1, g.kstart, g.kend, rhohkp, i2(Wn[c-PW], Wn[c]),
                             qv[1], qv[2], qv[3], qv[4], qv[5], qv[6], tc, tu);
-                    t += - ( tc - bv_c ) / rhok * dzik + ( tu - bv_u ) / rhok * dzik;
+                    t += - ( tc - bv_c ) * cf + ( tu - bv_u ) * cf;
                     bv_c = tc; bv_u = tu;
                     const TF evisce = TF(0.25)*(Ec[ce-EW] + Ec[ce] + Ec[ce+1-EW] + Ec[ce+1]) + visc;
                     const TF eviscw = TF(0.25)*(Ec[ce-1-EW] + Ec[ce-1] + Ec[ce-EW] + Ec[ce]) + visc;
@@ -298,7 +303,7 @@ namespace
                              + ( eviscn*(Vc[c+PW]-Vc[c   ])*dyi
                                - eviscs*(Vc[c   ]-Vc[c-PW])*dyi ) * TF(2.)*dyi;
                     const TF dt = rhohkp * evisct*((qv[4]-qv[3])*dzhikp + (Wn[c]-Wn[c-PW])*dyi);
-                    const TF ver = ( dt - dv_b ) / rhok * dzik;
+                    const TF ver = ( dt - dv_b ) * cf;
                     dv_b = dt;
                     t += hor + ver;
                     a.vt[ijk] = t;
@@ -313,10 +318,10 @@ namespace
                     {
                         TF t = a.wt[ijk];
                         if (a.buoyancy)
-                            t += TF(grav)/a.threfh[k] * (i2(qs[2], qs[3]) - a.threfh_rel[k]);
+                            t += a.gthi[k] * (i2(qs[2], qs[3]) - a.threfh_rel[k]);
                         t += hsum(hflux_2i5(i2(Up[c+1], Uc[c+1]), i2(Up[c], Uc[c]), Wc, c, 1, dxi),
                                   hflux_2i5(i2(Vp[c+PW], Vc[c+PW]), i2(Vp[c], Vc[c]), Wc, c, PW, dyi));
-                        t += - ( tc - bw_c ) / rhohk * dzhik + ( tu - bw_u ) / rhohk * dzhik;
+                        t += - ( tc - bw_c ) * wf + ( tu - bw_u ) * wf;
                         const TF evisce = TF(0.25)*(Ep[ce] + Ec[ce] + Ep[ce+1] + Ec[ce+1]) + visc;
                         const TF eviscw = TF(0.25)*(Ep[ce-1] + Ec[ce-1] + Ep[ce] + Ec[ce]) + visc;
                         const TF eviscn = TF(0.25)*(Ep[ce] + Ec[ce] + Ep[ce+EW] + Ec[ce+EW]) + visc;
@@ -325,7 +330,7 @@ namespace
                                - eviscw*((Wc[c  ]-Wc[c-1])*dxi + (Uc[c  ]-Up[c  ])*dzhik) ) * dxi
                              + ( eviscn*((Wc[c+PW]-Wc[c   ])*dyi + (Vc[c+PW]-Vp[c+PW])*dzhik)
                                - eviscs*((Wc[c   ]-Wc[c-PW])*dyi + (Vc[c   ]-Vp[c   ])*dzhik) ) * dyi
-                             + ( dt - dw_b ) / rhohk * TF(2.)*dzhik;
+                             + ( dt - dw_b ) * TF(2.)*wf;
                         a.wt[ijk] = t;
                     }
                     bw_c = tc; bw_u = tu; dw_b = dt;
@@ -338,7 +343,7 @@ namespace
                     TF tc, tu;
                     vface_c(k+1, g.kstart, g.kend, rhohkp, Wn[c],
                             qs[1], qs[2], qs[3], qs[4], qs[5], qs[6], tc, tu);
-                    t += - ( tc - bs_c ) / rhok * dzik + ( tu - bs_u ) / rhok * dzik;
+                    t += - ( tc - bs_c ) * cf + ( tu - bs_u ) * cf;
                     bs_c = tc; bs_u = tu;
                     const TF ti = a.tPr_i, sv = a.svisc;
                     const TF evisce = TF(0.5)*(Ec[ce   ]+Ec[ce+1 ]) * ti + sv;
@@ -352,7 +357,7 @@ namespace
                              + ( eviscn*(Sc[c+PW]-Sc[c   ])
                                - eviscs*(Sc[c   ]-Sc[c-PW]) ) * a.dyidyi;
                     const TF dt = rhohkp * evisct*(qs[4]-qs[3])*dzhikp;
-                    const TF ver = ( dt - ds_b ) / rhok * dzik;
+                    const TF ver = ( dt - ds_b ) * cf;
                     ds_b = dt;
                     t += hor + ver;
                     a.st[ijk] = t;
@@ -407,6 +412,7 @@ namespace
         a.w = static_cast<const TF*>(c->f.w); a.s = static_cast<const TF*>(c->f.s[0]);
         a.evisc = static_cast<const TF*>(c->f.evisc);
         a.dzi = p.dzi; a.dzhi = p.dzhi; a.rho = p.rhoref; a.rhoh = p.rhorefh; a.threfh = p.threfh; a.threfh_rel = p.threfh_rel;
+        a.cfac = p.cfac; a.wfac = p.wfac; a.gthi = p.gthi;
         a.dxi = TF(1.)/TF(g.dx); a.dyi = TF(1.)/TF(g.dy);
         a.dxidxi = TF(1./(g.dx*g.dx)); a.dyidyi = TF(1./(g.dy*g.dy));
         a.visc = TF(par.visc); a.svisc = TF(par.svisc[0]); a.tPr_i = TF(1)/TF(par.tPr);
