@@ -199,7 +199,10 @@ namespace
         if (sign < 0)
             for (int x = lane; x < n; x += 64) Cr[x] = Z[lpad(x)];
         else
-            for (int x = lane; x < n; x += 64) { T z = Z[lpad(x)]; z.x /= TF(n); z.y /= TF(n); Cr[x] = z; }
+        {
+            const TF ni = TF(1)/TF(n);   // exact for powers of two
+            for (int x = lane; x < n; x += 64) { T z = Z[lpad(x)]; z.x *= ni; z.y *= ni; Cr[x] = z; }
+        }
     }
 
     // Tridiagonal solve per (kx, ky) mode (Pres_2::solve + tdma,
@@ -238,23 +241,23 @@ namespace
         };
 
         // forward elimination (src/pres_2.cxx:200-252)
-        TC w2 = bcoef(0);
+        // one reciprocal of the pivot per level instead of three divisions
+        TC wi = TC(1)/bcoef(0);
         TC px, py;
         {
             const cplx<TF> q = C[mode];
-            px = dz2[0]*TC(q.x); py = dz2[0]*TC(q.y);
-            px /= w2; py /= w2;
+            px = dz2[0]*TC(q.x)*wi; py = dz2[0]*TC(q.y)*wi;
             C[mode] = cplx<TF>{TF(px), TF(py)};
         }
         for (int k = 1; k < K; ++k)
         {
             const cplx<TF> q = C[k*ks + mode];
-            const TC gk = c[k-1] / w2;
+            const TC gk = c[k-1]*wi;
             gam[k*ks + mode] = gk;
-            w2 = bcoef(k) - a[k]*gk;
+            wi = TC(1)/(bcoef(k) - a[k]*gk);
             TC qx = dz2[k]*TC(q.x), qy = dz2[k]*TC(q.y);
             qx -= a[k]*px; qy -= a[k]*py;
-            qx /= w2; qy /= w2;
+            qx *= wi; qy *= wi;
             C[k*ks + mode] = cplx<TF>{TF(qx), TF(qy)};
             px = qx; py = qy;
         }
@@ -355,6 +358,7 @@ namespace
         T* Z = fft_row<TF>(A, tws, tw, plan, +1, lane);
         const int ja = j0 + 2*row, jb = ja + 1;
         const int jj = g.icells, kk = g.ijcells;
+        const TF ni = TF(1)/TF(n);   // the /itot of src/fft.cxx:446, exact for powers of two
         const int kc = k + g.kgc;
         for (int pr = 0; pr < 2; ++pr)
         {
@@ -371,7 +375,7 @@ namespace
                 int xi = x - g.igc;
                 xi = (xi < 0) ? xi + n : (xi >= n ? xi - n : xi);
                 const T z = Z[lpad(xi)];
-                const TF val = (pr ? z.y : z.x) / TF(n);
+                const TF val = (pr ? z.y : z.x) * ni;
                 for (int q = 0; q < nrow; ++q)
                 {
                     const int ij = x + rows_[q]*jj;
