@@ -41,6 +41,8 @@ HBM_PEAK_GBS = 8000.0
 #   fft x/y (fwd, bwd): spectrum in and out = 2.  tdma: spectrum in and out = 2.
 #   pres_out: p, ut, vt, wt; ut, vt, wt = 7.  rk3 (one field): a, at; a, at = 4.
 #   buoyancy: th, wt; wt = 3.
+#   pres_in_fft_x (right-hand side evaluated inside the forward x transform):
+#       u, v, w, ut, vt, wt in; spectrum out = 7.
 #   pres_out_rk3 (gradient + RK3 of u, v, w): p, u, v, w, ut, vt, wt in;
 #       u, v, w, ut, vt, wt out = 13.
 #   tend_2i5_smag2 (fused buoyancy + advection + diffusion of u, v, w, th):
@@ -49,7 +51,7 @@ PASSES = {
     "tend_2i5_smag2": 13,
     "advec_2i5_u": 5, "advec_2i5_v": 5, "advec_2i5_w": 5, "advec_2i5_s": 6,
     "evisc": 5, "diff_u": 6, "diff_v": 6, "diff_w": 6, "diff_c": 4,
-    "pres_in": 7, "fft_x_fwd": 2, "fft_y_fwd": 2, "tdma": 2, "fft_y_bwd": 2, "fft_x_bwd": 2,
+    "pres_in": 7, "pres_in_fft_x": 7, "fft_x_fwd": 2, "fft_y_fwd": 2, "tdma": 2, "fft_y_bwd": 2, "fft_x_bwd": 2,
     "pres_out": 7, "pres_out_rk3": 13, "rk3": 4, "buoyancy": 3,
 }
 KERNELS = list(PASSES) + ["cyclic", "ghost_cells", "mirror_walls", "rk3_zero_ghosts", "cfl", "dnmul"]
@@ -127,6 +129,57 @@ def copy_calibration(nbytes=1 << 30, reps=5):
                       "ms_per_copy": round(el * 1e3, 4), "gbs_read_plus_write": round(2 * nbytes / el / 1e9, 1)}))
 
 
+def mixed_precision(mm, abi, args, fp64_model, fp64_ms_per_step):
+    """The mixed-precision half of the metric: the same workload with FP32
+    fields and transforms (FP64 tridiagonal, mean pressure and divergence),
+    timed over the same number of steps, and its error against FP64 for one
+    substep started from the FP64 model's current state.  The error of a
+    20-step run is not reported here: this flow amplifies a 1e-6 K theta
+    perturbation to ~2e-2 in u within 20 FP64 steps, so such a figure measures
+    the flow's sensitivity, not the arithmetic."""
+    itot, jtot, ktot = args.itot, args.jtot, args.ktot
+    kw = dict(itot=itot, jtot=jtot, ktot=ktot, xsize=50.0 * itot, ysize=50.0 * jtot, zsize=12.5 * ktot)
+    L = abi.lib()
+    log("bench: mixed-precision model")
+    m = mm.Model(mm.dry_cbl_case(precision=mm.MHH_MIXED, **kw))
+    m.steps(args.warmup)
+    abi.check(L.mhh_device_synchronize(), "sync")
+    t0 = time.perf_counter()
+    m.steps(args.steps)
+    abi.check(L.mhh_device_synchronize(), "sync")
+    el = time.perf_counter() - t0
+    m.close()
+    npts = itot * jtot * ktot
+    ms = el / args.steps * 1e3
+    out = {"value": round(npts * 3 * args.steps / el / 1e6, 2), "unit": UNIT, "ms_per_step": round(ms, 3),
+           "speedup_vs_fp64": round(fp64_ms_per_step / ms, 3), "dtype": "f32 (f64 tridiagonal, mean pressure, divergence)"}
+    # one substep from the FP64 state, with a fixed dt so both take the same step
+    log("bench: mixed-precision one-substep error against FP64")
+    t, dt, substep, _ = fp64_model.time()
+    if substep == 0:
+        names = ("u", "v", "w", "th", "ut", "vt", "wt", "tht")
+        fixed = mm.dry_cbl_case(precision=mm.MHH_MIXED, adaptive=False, dt=dt, **kw)
+        b = mm.Model(fixed)
+        a = mm.Model(mm.dry_cbl_case(precision=mm.MHH_FP64, adaptive=False, dt=dt, **kw))
+        for n in names:
+            f = fp64_model.field(n)
+            a.set_field(n, f)
+            b.set_field(n, f)
+            del f
+        a.substep(); b.substep()
+        sl = (slice(1, -1), slice(3, -3), slice(3, -3))
+        err = {}
+        for n in ("u", "w", "th"):
+            fa = a.field(n)[sl]
+            fb = b.field(n)[sl]
+            err[n] = float(abs(fb - fa).max() / abs(fa).max())
+            del fa, fb
+        out["one_substep_rel_err"] = {k: float(f"{v:.3e}") for k, v in err.items()}
+        out["div_max"] = float(f"{b.div():.3e}")
+        a.close(); b.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +190,7 @@ def main():
     ap.add_argument("--jtot", type=int, default=512)
     ap.add_argument("--ktot", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel events in the timed steps")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the known-size copy kernel (HBM counter calibration)")
@@ -211,6 +265,9 @@ def main():
     substep_gbs = B_ALG_PASSES * esize * npts / (sub_ms * 1e-3) / 1e9
     div = m.div()
     t, dt, _, it = m.time()
+    mixed = None
+    if prec == mm.MHH_FP64 and not args.no_mixed:
+        mixed = mixed_precision(mm, abi, args, m, ms_per_step)
     m.close()
 
     cpu = None
@@ -235,6 +292,7 @@ def main():
                              "achieved": round(substep_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": round(substep_gbs / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
+        "mixed": mixed,
         "kernels": kernels,
         "state": {"model_time_s": round(t, 3), "dt_s": round(dt, 4), "iteration": it, "div_max": div},
     }

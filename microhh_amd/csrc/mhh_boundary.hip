@@ -58,6 +58,73 @@ namespace
         }
     }
 
+    // The same two copies for several fields in one launch each (blockIdx.z
+    // selects the field): Boundary::set_prognostic_cyclic_bcs in two launches.
+    template<typename TF>
+    struct Field_list { TF* f[3 + MHH_MAX_SCALARS]; };
+
+    template<typename TF>
+    __global__ void cyclic_ew_multi_g(Field_list<TF> fl, Gd g)
+    {
+        TF* __restrict__ a = fl.f[blockIdx.z];
+        const int t = blockIdx.x*blockDim.x + threadIdx.x;
+        const int k = blockIdx.y;
+        const int per = 2*g.igc;
+        if (t >= per*g.jcells) return;
+        const int j = t / per, c = t % per;
+        const int base = j*g.icells + k*g.ijcells;
+        if (c < g.igc)
+            a[base + c] = a[base + g.iend - g.igc + c];
+        else
+        {
+            const int i = c - g.igc;
+            a[base + g.iend + i] = a[base + g.istart + i];
+        }
+    }
+
+    template<typename TF>
+    __global__ void cyclic_ns_multi_g(Field_list<TF> fl, Gd g)
+    {
+        TF* __restrict__ a = fl.f[blockIdx.z];
+        const int t = blockIdx.x*blockDim.x + threadIdx.x;
+        const int k = blockIdx.y;
+        if (t >= 2*g.jgc*g.icells) return;
+        const int r = t / g.icells, i = t % g.icells;
+        const int kk = k*g.ijcells, jj = g.icells;
+        if (g.jtot > 1)
+        {
+            if (r < g.jgc)
+                a[i + r*jj + kk] = a[i + (g.jend - g.jgc + r)*jj + kk];
+            else
+            {
+                const int jr = r - g.jgc;
+                a[i + (g.jend + jr)*jj + kk] = a[i + (g.jstart + jr)*jj + kk];
+            }
+        }
+        else if (k >= g.kstart && k < g.kend)
+        {
+            const TF ref = a[i + g.jstart*jj + kk];
+            if (r < g.jgc) a[i + r*jj + kk] = ref;
+            else a[i + (g.jend + r - g.jgc)*jj + kk] = ref;
+        }
+    }
+
+    template<typename TF>
+    int cyclic_prognostic(mhh_ctx* c)
+    {
+        const Gd& g = c->gd;
+        Field_list<TF> fl{};
+        int nf = 0;
+        fl.f[nf++] = static_cast<TF*>(c->f.u);
+        fl.f[nf++] = static_cast<TF*>(c->f.v);
+        fl.f[nf++] = static_cast<TF*>(c->f.w);
+        for (int n = 0; n < c->f.nscalars; ++n) fl.f[nf++] = static_cast<TF*>(c->f.s[n]);
+        const int new_ = 2*g.igc*g.jcells, nns = 2*g.jgc*g.icells;
+        { mhh::Timer _t(c, "cyclic"); cyclic_ew_multi_g<TF><<<dim3((new_ + 255)/256, g.kcells, nf), 256, 0, c->stream>>>(fl, g); }
+        { mhh::Timer _t(c, "cyclic"); cyclic_ns_multi_g<TF><<<dim3((nns + 255)/256, g.kcells, nf), 256, 0, c->stream>>>(fl, g); }
+        return check_launch("set_prognostic_cyclic_bcs");
+    }
+
     // calc_ghost_cells_bot_2nd / top_2nd (src/boundary.cxx:700-775) over the full plane.
     template<typename TF>
     __global__ void ghost_2nd_g(TF* __restrict__ a, const TF* __restrict__ abot, const TF* __restrict__ agradbot,
@@ -271,12 +338,7 @@ int mhh_boundary_cyclic(mhh_ctx* c, void* field, int edge)
 int mhh_boundary_set_prognostic_cyclic_bcs(mhh_ctx* c)
 {
     MHH_REQUIRE_BOUND(c);
-    MHH_CHECK(mhh_boundary_cyclic(c, c->f.u, MHH_EDGE_BOTH));
-    MHH_CHECK(mhh_boundary_cyclic(c, c->f.v, MHH_EDGE_BOTH));
-    MHH_CHECK(mhh_boundary_cyclic(c, c->f.w, MHH_EDGE_BOTH));
-    for (int n = 0; n < c->f.nscalars; ++n)
-        MHH_CHECK(mhh_boundary_cyclic(c, c->f.s[n], MHH_EDGE_BOTH));
-    return 0;
+    return c->precision == MHH_FP64 ? cyclic_prognostic<double>(c) : cyclic_prognostic<float>(c);
 }
 
 int mhh_boundary_set_ghost_cells(mhh_ctx* c, const mhh_bc_desc* bc)

@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 
@@ -200,6 +201,8 @@ int mhh_ctx_create(const mhh_grid_desc* g, int precision, void* hip_stream, mhh_
 
     mhh_ctx* c = new mhh_ctx();
     c->mixed = precision == MHH_MIXED;
+    // levels per block of the fused tendency kernel (a tuning choice, same results)
+    if (const char* e = std::getenv("MHH_TEND_KCHUNK")) c->tend_kchunk = std::max(0, std::atoi(e));
     c->precision = c->mixed ? MHH_FP32 : precision;
     c->stream = static_cast<hipStream_t>(hip_stream);
     (void)hipGetDevice(&c->device);

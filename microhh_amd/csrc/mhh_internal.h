@@ -75,6 +75,8 @@ struct mhh_ctx
     bool have_base_state = false;
     mhh_field_set f{};
     bool bound = false;
+    // Levels per block of the fused tendency kernel (0: the built-in default).
+    int tend_kchunk = 0;
     // Scalars advected with the Koren flux limiter ([advec] fluxlimit_list).
     int fluxlimit[MHH_MAX_SCALARS] = {};
     // Reduction scratch: per-block partial maxima and one result slot.

@@ -54,6 +54,32 @@ namespace
                   - rhoh[kc  ] * (wt[ijk   ] + w[ijk   ] * dti) ) * dzi[kc];
     }
 
+    // Inputs of Pres_2::input (src/pres_2.cxx:155-196) for the forward x
+    // transform that evaluates the right-hand side while loading its rows.
+    template<typename TF>
+    struct Pin_args
+    {
+        const TF* __restrict__ u; const TF* __restrict__ v; const TF* __restrict__ w;
+        const TF* __restrict__ ut; const TF* __restrict__ vt; const TF* __restrict__ wt;
+        const TF* __restrict__ dzi; const TF* __restrict__ rho; const TF* __restrict__ rhoh;
+        TF dxi, dyi, dti;
+    };
+
+    // P at interior point (i, j, k) (0-based), the expression of pres_in_g.
+    template<typename TF>
+    __device__ __forceinline__ TF pres_rhs(const Pin_args<TF>& a, int i, int j, int k, const Gd& g)
+    {
+        const int jj = g.icells, kk = g.ijcells;
+        const int ijk = i+g.igc + (j+g.jgc)*jj + (k+g.kgc)*kk;
+        const int ie = (i+1 == g.itot) ? ijk + 1 - g.itot : ijk + 1;
+        const int jn = (j+1 == g.jtot) ? ijk + jj - g.jtot*jj : ijk + jj;
+        const int kc = k + g.kgc;
+        return a.rho[kc] * ( (a.ut[ie] + a.u[ijk+1] * a.dti) - (a.ut[ijk] + a.u[ijk] * a.dti) ) * a.dxi
+             + a.rho[kc] * ( (a.vt[jn] + a.v[ijk+jj] * a.dti) - (a.vt[ijk] + a.v[ijk] * a.dti) ) * a.dyi
+             + ( a.rhoh[kc+1] * (a.wt[ijk+kk] + a.w[ijk+kk] * a.dti)
+               - a.rhoh[kc  ] * (a.wt[ijk   ] + a.w[ijk   ] * a.dti) ) * a.dzi[kc];
+    }
+
     // Packed staging of the two half-spectra of a pair of real rows in n complex
     // slots: the DC (and, for even n, Nyquist) entries are real and share one
     // slot each as (row a, row b); every other kx holds (Xa, Xb) in two slots.
@@ -83,7 +109,7 @@ namespace
     template<typename TF>
     __global__ void __launch_bounds__(256) fft_x_fwd_g(const TF* __restrict__ P, cplx<TF>* __restrict__ C,
                                                        Plan plan, const double2* __restrict__ tw, int RB, Gd g,
-                                                       const double* __restrict__ pmean)
+                                                       const double* __restrict__ pmean, Pin_args<TF> pin, int fuse_in)
     {
         using T = cplx<TF>;
         extern __shared__ unsigned char smem_raw[];
@@ -97,7 +123,18 @@ namespace
         T* A = rows + row*re;
         if (plan.fast) load_twiddles<TF>(tws, tw, n);
         const TF* Pk = P + static_cast<size_t>(k)*n*M;
-        if (pmean)
+        if (fuse_in)
+        {
+            // the right-hand side evaluated here instead of in pres_in_g
+            for (int x = lane; x < n; x += 64)
+            {
+                T z;
+                z.x = (ja < M) ? pres_rhs(pin, x, ja, k, g) : TF(0);
+                z.y = (jb < M) ? pres_rhs(pin, x, jb, k, g) : TF(0);
+                A[lpad(x)] = z;
+            }
+        }
+        else if (pmean)
         {
             // mixed mode: the plane mean is solved separately in FP64
             const double m = pmean[k];
@@ -638,7 +675,13 @@ namespace
         TF* gam = static_cast<TF*>(c->f.tmp2);
         const TF dxi = TF(1.)/TF(g.dx), dyi = TF(1.)/TF(g.dy), dti = TF(1.)/TF(dt);
 
-        // 1. divergence into the dense array P (stored in p)
+        // 1. divergence into the dense array P (stored in p); without the mixed
+        // mode's plane means the forward x transform evaluates it itself
+        Pin_args<TF> pin{static_cast<const TF*>(c->f.u), static_cast<const TF*>(c->f.v), static_cast<const TF*>(c->f.w),
+                         static_cast<const TF*>(c->f.ut), static_cast<const TF*>(c->f.vt), static_cast<const TF*>(c->f.wt),
+                         p.dzi, p.rhoref, p.rhorefh, dxi, dyi, dti};
+        const int fuse_in = c->mixed ? 0 : 1;
+        if (!fuse_in)
         {
             const dim3 grd((N + BX - 1)/BX, (M + BY - 1)/BY, K);
             { mhh::Timer _t(c, "pres_in"); pres_in_g<TF><<<grd, dim3(BX, BY), 0, c->stream>>>(P,
@@ -661,8 +704,8 @@ namespace
             const int RB = rows_per_block(px, sizeof(T));
             const dim3 grd((M + 2*RB - 1)/(2*RB), K);
             const size_t lds = lds_bytes(px, RB, sizeof(T));
-            { mhh::Timer _t(c, "fft_x_fwd"); fft_x_fwd_g<TF><<<grd, dim3(64, RB), lds, c->stream>>>(P, C, px, reinterpret_cast<const double2*>(t.twx), RB, g,
-                    c->mixed ? c->pm_mean : nullptr); }
+            { mhh::Timer _t(c, fuse_in ? "pres_in_fft_x" : "fft_x_fwd"); fft_x_fwd_g<TF><<<grd, dim3(64, RB), lds, c->stream>>>(P, C, px, reinterpret_cast<const double2*>(t.twx), RB, g,
+                    c->mixed ? c->pm_mean : nullptr, pin, fuse_in); }
             MHH_CHECK(check_launch("fft_x_fwd"));
         }
         // 3. forward y transform

@@ -39,7 +39,7 @@ namespace
         const TF* __restrict__ cfac; const TF* __restrict__ wfac; const TF* __restrict__ gthi;
         TF dxi, dyi, dxidxi, dyidyi, visc, svisc, tPr_i;
         int buoyancy;
-        int nti, ntj, nkc;
+        int nti, ntj, nkc, kchunk;
     };
 
     template<typename TF>
@@ -162,8 +162,8 @@ namespace
         const int i0 = g.istart + ti*TX, j0 = g.jstart + tj*TY;
         const int i = i0 + tx, j = j0 + ty;
         const bool active = i < g.iend && j < g.jend;
-        const int kb = g.kstart + kc*KCHUNK;
-        const int ke = min(kb + KCHUNK, g.kend);
+        const int kb = g.kstart + kc*a.kchunk;
+        const int ke = min(kb + a.kchunk, g.kend);
         const int jj = g.icells, kk = g.ijcells;
 
         TF* Up = lds;        TF* Uc = Up + PN;
@@ -419,7 +419,8 @@ namespace
         a.buoyancy = buoyancy;
         a.nti = (g.itot + TX - 1)/TX;
         a.ntj = (g.jtot + TY - 1)/TY;
-        a.nkc = (g.ktot + KCHUNK - 1)/KCHUNK;
+        a.kchunk = c->tend_kchunk > 0 ? c->tend_kchunk : KCHUNK;
+        a.nkc = (g.ktot + a.kchunk - 1)/a.kchunk;
         const int ntiles = a.nti*a.ntj*a.nkc;
         const int nblocks = (ntiles + 7)/8*8;
         { mhh::Timer _t(c, "tend_2i5_smag2"); tend_2i5_smag2_g<TF><<<nblocks, dim3(TX, TY), 0, c->stream>>>(a, g); }

@@ -231,6 +231,18 @@ def test_cyclic(setup, edge):
     assert np.array_equal(g.get("u"), ref)
 
 
+def test_prognostic_cyclic_bcs(setup):
+    """Boundary::set_prognostic_cyclic_bcs: periodic halos of u, v, w and the
+    scalar in one call, each equal to the oracle's copy."""
+    p, i, d, g, *_ = setup
+    _load_common(g, i)
+    g.call("mhh_boundary_set_prognostic_cyclic_bcs")
+    for name, key in (("u", "u"), ("v", "v"), ("w", "w"), ("s0", "s")):
+        ref = _c(i[key])
+        po.lib().mo_boundary_cyclic(d, po.ptr(ref), 2)
+        assert np.array_equal(g.get(name), ref), name
+
+
 @pytest.mark.parametrize("bot,top", [(0, 1), (1, 0), (2, 4), (0, 0)])
 def test_ghost_cells(setup, bot, top):
     from microhh_amd import abi
