@@ -25,10 +25,37 @@ namespace
     template<typename TF>
     void free_prof(mhh::Prof<TF>& p)
     {
-        TF* all[] = {p.z, p.zh, p.dz, p.dzh, p.dzi, p.dzhi, p.rhoref, p.rhorefh, p.thref, p.threfh, p.threfh_rel,
-                     p.cfac, p.wfac, p.gthi};
+        TF* all[] = {p.z, p.zh, p.dz, p.dzh, p.dzi, p.dzhi, p.rhoref, p.rhorefh, p.thref, p.threfh, p.threfh_rel};
         for (TF* a : all) if (a) (void)hipFree(a);
+        if (p.lev) (void)hipFree(p.lev);
         p = mhh::Prof<TF>();
+    }
+
+    // The per-level records of the fused tendency kernel, from the host
+    // profiles and (when set) the base state; rebuilt whenever either changes.
+    template<typename TF>
+    int upload_levels(mhh_ctx* c, mhh::Prof<TF>& p)
+    {
+        const size_t n = c->h_dzi.size();
+        if (n == 0) return 0;
+        auto at = [](const std::vector<double>& v, size_t k) { return k < v.size() ? v[k] : 0.; };
+        std::vector<mhh::Level<TF>> lv(n);
+        for (size_t k = 0; k < n; ++k)
+        {
+            const double rho = at(c->h_rhoref, k), rhoh = at(c->h_rhorefh, k);
+            const double threfh = at(c->h_threfh, k);
+            mhh::Level<TF>& l = lv[k];
+            l.rho = TF(rho); l.rhoh = TF(rhoh); l.rhohp = TF(at(c->h_rhorefh, k+1));
+            l.dzi = TF(at(c->h_dzi, k)); l.dzhi = TF(at(c->h_dzhi, k)); l.dzhip = TF(at(c->h_dzhi, k+1));
+            l.cfac = TF(rho != 0. ? at(c->h_dzi, k)/rho : 0.);
+            l.wfac = TF(rhoh != 0. ? at(c->h_dzhi, k)/rhoh : 0.);
+            l.gthi = TF(threfh != 0. ? 9.81/threfh : 0.);
+            l.threfh_rel = TF(c->h_threfh.empty() ? 0. : threfh - c->s_offset[0]);
+        }
+        if (!p.lev)
+            MHH_CHECK(mhh::check_hip(hipMalloc(&p.lev, sizeof(mhh::Level<TF>)*n), "hipMalloc(levels)"));
+        return mhh::check_hip(hipMemcpy(p.lev, lv.data(), sizeof(mhh::Level<TF>)*n, hipMemcpyHostToDevice),
+                              "hipMemcpy(levels)");
     }
 
     template<typename TF>
@@ -42,15 +69,7 @@ namespace
         MHH_CHECK(upload(&p.dzhi, c->h_dzhi));
         MHH_CHECK(upload(&p.rhoref, c->h_rhoref));
         MHH_CHECK(upload(&p.rhorefh, c->h_rhorefh));
-        // per-level factors that replace divisions in the fused tendency kernel
-        std::vector<double> cf(c->h_dzi.size()), wf(c->h_dzhi.size());
-        for (size_t k = 0; k < cf.size(); ++k)
-        {
-            cf[k] = c->h_rhoref[k] != 0. ? c->h_dzi[k]/c->h_rhoref[k] : 0.;
-            wf[k] = c->h_rhorefh[k] != 0. ? c->h_dzhi[k]/c->h_rhorefh[k] : 0.;
-        }
-        MHH_CHECK(upload(&p.cfac, cf));
-        return upload(&p.wfac, wf);
+        return upload_levels(c, p);
     }
 
     // Second stage of a max-reduction: one block folds the partials into red[0].
@@ -149,15 +168,14 @@ namespace
     template<typename TF>
     int upload_base_state(mhh_ctx* c, mhh::Prof<TF>& p)
     {
-        if (p.thref) { (void)hipFree(p.thref); (void)hipFree(p.threfh); (void)hipFree(p.threfh_rel); (void)hipFree(p.gthi); }
-        p.thref = p.threfh = p.threfh_rel = p.gthi = nullptr;
-        std::vector<double> rel(c->h_threfh), gi(c->h_threfh.size());
+        if (p.thref) { (void)hipFree(p.thref); (void)hipFree(p.threfh); (void)hipFree(p.threfh_rel); }
+        p.thref = p.threfh = p.threfh_rel = nullptr;
+        std::vector<double> rel(c->h_threfh);
         for (double& x : rel) x -= c->s_offset[0];
-        for (size_t k = 0; k < gi.size(); ++k) gi[k] = c->h_threfh[k] != 0. ? 9.81/c->h_threfh[k] : 0.;
         MHH_CHECK(upload(&p.thref, c->h_thref));
         MHH_CHECK(upload(&p.threfh, c->h_threfh));
-        MHH_CHECK(upload(&p.gthi, gi));
-        return upload(&p.threfh_rel, rel);
+        MHH_CHECK(upload(&p.threfh_rel, rel));
+        return upload_levels(c, p);
     }
 
     int refresh_base_state(mhh_ctx* c)

@@ -23,6 +23,19 @@ namespace mhh
         double dx, dy, dxi, dyi;
     };
 
+    // The per-level factors of the fused tendency kernel in one record, so
+    // that one scalar load brings all of them for a level.
+    template<typename TF>
+    struct Level
+    {
+        TF rho, rhoh, rhohp;      // rhoref[k], rhorefh[k], rhorefh[k+1]
+        TF dzi, dzhi, dzhip;      // dzi[k], dzhi[k], dzhi[k+1]
+        TF cfac;                  // dzi/rhoref: vertical flux divergence factor of cell-centred fields
+        TF wfac;                  // dzhi/rhorefh: the same for w
+        TF gthi;                  // grav/threfh (0 without a base state)
+        TF threfh_rel;            // threfh - offset of scalar 0
+    };
+
     // Device copies of the 1-D grid and base-state profiles in the element type.
     template<typename TF>
     struct Prof
@@ -31,9 +44,7 @@ namespace mhh
         TF* dzi = nullptr; TF* dzhi = nullptr; TF* rhoref = nullptr; TF* rhorefh = nullptr;
         TF* thref = nullptr; TF* threfh = nullptr;
         TF* threfh_rel = nullptr;   // threfh - offset of scalar 0 (the buoyancy difference)
-        TF* cfac = nullptr;         // dzi/rhoref: vertical flux divergence factor of cell-centred fields
-        TF* wfac = nullptr;         // dzhi/rhorefh: the same for w
-        TF* gthi = nullptr;         // grav/threfh
+        Level<TF>* lev = nullptr;   // kcells records
     };
 
     // Pressure-solver tables (Pres_2::set_values, src/pres_2.cxx:124-153) and

@@ -27,16 +27,13 @@ namespace
     constexpr int KCHUNK = 64;
     constexpr double grav = 9.81;        // include/constants.h:29
 
+    // Scalars of the fused tendency; the field and profile pointers are kernel
+    // parameters of their own, declared __restrict__ there, so that the
+    // compiler knows the stores into the tendencies leave every other array
+    // unchanged (the per-level records then come in through scalar loads).
     template<typename TF>
     struct Tend_args
     {
-        TF* __restrict__ ut; TF* __restrict__ vt; TF* __restrict__ wt; TF* __restrict__ st;
-        const TF* __restrict__ u; const TF* __restrict__ v; const TF* __restrict__ w;
-        const TF* __restrict__ s; const TF* __restrict__ evisc;
-        const TF* __restrict__ dzi; const TF* __restrict__ dzhi;
-        const TF* __restrict__ rho; const TF* __restrict__ rhoh; const TF* __restrict__ threfh;
-        const TF* __restrict__ threfh_rel;
-        const TF* __restrict__ cfac; const TF* __restrict__ wfac; const TF* __restrict__ gthi;
         TF dxi, dyi, dxidxi, dyidyi, visc, svisc, tPr_i;
         int buoyancy;
         int nti, ntj, nkc, kchunk;
@@ -60,11 +57,28 @@ namespace
     constexpr int NQ3 = (PN + TX*TY - 1)/(TX*TY);
     constexpr int NQ1 = (EN + TX*TY - 1)/(TX*TY);
 
-    // Offsets (relative to the start of a plane) of the thread's share of a
-    // halo plane, -1 where the element lies outside the field; the same for
-    // every level, so they are computed once per chunk.
-    template<int NQ>
-    __device__ __forceinline__ void plane_offsets(int (&o)[NQ], int i0, int j0, int halo, int w, int n,
+    // Element at a byte offset from a (block-uniform) base.  The 32-bit
+    // unsigned offset lets every load use the scalar-base addressing form: no
+    // per-lane 64-bit address arithmetic.
+    template<typename TF>
+    __device__ __forceinline__ TF ldo(const TF* __restrict__ base, uint32_t off)
+    {
+        return *reinterpret_cast<const TF*>(reinterpret_cast<const char*>(base) + off);
+    }
+
+    template<typename TF>
+    __device__ __forceinline__ void sto(TF* __restrict__ base, uint32_t off, TF x)
+    {
+        *reinterpret_cast<TF*>(reinterpret_cast<char*>(base) + off) = x;
+    }
+
+    // Byte offsets (relative to the start of a plane) of the thread's share of
+    // a halo plane; the same for every level, so they are computed once per
+    // chunk.  Elements past the plane or outside the field get offset 0: they
+    // are loaded (in bounds) but never stored (past the plane) or only read by
+    // threads whose point lies outside the domain (outside the field).
+    template<typename TF, int NQ>
+    __device__ __forceinline__ void plane_offsets(uint32_t (&o)[NQ], int i0, int j0, int halo, int w, int n,
                                                   const Gd& g, int tid)
     {
         #pragma unroll
@@ -73,30 +87,29 @@ namespace
             const int idx = tid + q*TX*TY;
             const int pj = idx / w, pi = idx - pj*w;
             const int gi = i0 - halo + pi, gj = j0 - halo + pj;
-            o[q] = (idx < n && gi < g.icells && gj < g.jcells) ? gi + gj*g.icells : -1;
+            o[q] = (idx < n && gi < g.icells && gj < g.jcells) ? uint32_t(gi + gj*g.icells)*sizeof(TF) : 0u;
         }
     }
 
+    // Unconditional loads of the thread's share of plane k (k < kcells).
     template<typename TF, int NQ>
     __device__ __forceinline__ void fetch_plane(TF (&r)[NQ], const TF* __restrict__ src, int k,
-                                                const int (&o)[NQ], const Gd& g)
+                                                const uint32_t (&o)[NQ], const Gd& g)
     {
         const TF* __restrict__ sk = src + static_cast<size_t>(k)*g.ijcells;
-        const bool kin = k < g.kcells;
         #pragma unroll
         for (int q = 0; q < NQ; ++q)
-            r[q] = (kin && o[q] >= 0) ? sk[o[q]] : TF(0);
+            r[q] = ldo(sk, o[q]);
     }
 
     template<typename TF, int NQ>
     __device__ __forceinline__ void store_plane(TF* __restrict__ dst, const TF (&r)[NQ], int n, int tid)
     {
         #pragma unroll
-        for (int q = 0; q < NQ; ++q)
-        {
-            const int idx = tid + q*TX*TY;
-            if (idx < n) dst[idx] = r[q];
-        }
+        for (int q = 0; q < NQ-1; ++q)
+            dst[tid + q*TX*TY] = r[q];
+        const int idx = tid + (NQ-1)*TX*TY;
+        if (idx < n) dst[idx] = r[NQ-1];
     }
 
     // Vertical 2i5 face products of a cell-centred field at w level f (the face
@@ -144,7 +157,11 @@ namespace
     }
 
     template<typename TF>
-    __global__ void __launch_bounds__(TX*TY, 2) tend_2i5_smag2_g(Tend_args<TF> a, Gd g)
+    __global__ void __launch_bounds__(TX*TY, 2) tend_2i5_smag2_g(
+            TF* __restrict__ ut, TF* __restrict__ vt, TF* __restrict__ wt, TF* __restrict__ st,
+            const TF* __restrict__ u, const TF* __restrict__ v, const TF* __restrict__ w,
+            const TF* __restrict__ s, const TF* __restrict__ evisc,
+            const Level<TF>* __restrict__ lev, Tend_args<TF> a, Gd g)
     {
         __shared__ TF lds[7*PN + 3*EN];
 
@@ -164,7 +181,10 @@ namespace
         const bool active = i < g.iend && j < g.jend;
         const int kb = g.kstart + kc*a.kchunk;
         const int ke = min(kb + a.kchunk, g.kend);
-        const int jj = g.icells, kk = g.ijcells;
+        const int kk = g.ijcells;
+        // Threads outside the domain compute on the clamped column (in bounds)
+        // and store nothing.
+        const int ij = min(i, g.iend-1) + min(j, g.jend-1)*g.icells;
 
         TF* Up = lds;        TF* Uc = Up + PN;
         TF* Vp = Uc + PN;    TF* Vc = Vp + PN;
@@ -172,24 +192,23 @@ namespace
         TF* Sc = Wn + PN;
         TF* Ep = Sc + PN;    TF* Ec = Ep + EN;   TF* En = Ec + EN;
 
-        load_plane(Up, a.u, kb-1, i0, j0, H, PW, PN, g, tid);
-        load_plane(Vp, a.v, kb-1, i0, j0, H, PW, PN, g, tid);
-        load_plane(Wc, a.w, kb, i0, j0, H, PW, PN, g, tid);
-        load_plane(Ep, a.evisc, kb-1, i0, j0, 1, EW, EN, g, tid);
-        load_plane(Ec, a.evisc, kb, i0, j0, 1, EW, EN, g, tid);
+        load_plane(Up, u, kb-1, i0, j0, H, PW, PN, g, tid);
+        load_plane(Vp, v, kb-1, i0, j0, H, PW, PN, g, tid);
+        load_plane(Wc, w, kb, i0, j0, H, PW, PN, g, tid);
+        load_plane(Ep, evisc, kb-1, i0, j0, 1, EW, EN, g, tid);
+        load_plane(Ec, evisc, kb, i0, j0, 1, EW, EN, g, tid);
 
-        // Column queues: q[m] holds level k-3+m of the thread's own column.
+        // Column queues: q[m] holds level k-3+m of the thread's own column
+        // (levels past the field repeat its top level: the stencils of the
+        // faces near the top wall never reach them).
+        const uint32_t oij = uint32_t(ij)*sizeof(TF);    // column offset within a level
         TF qu[7], qv[7], qw[7], qs[7];
-        const int ij = i + j*jj;
         #pragma unroll
         for (int m = 0; m < 7; ++m)
         {
-            const int kq = kb - 3 + m;
-            const bool ok = active && kq >= 0 && kq < g.kcells;
-            qu[m] = ok ? a.u[ij + kq*kk] : TF(0);
-            qv[m] = ok ? a.v[ij + kq*kk] : TF(0);
-            qw[m] = ok ? a.w[ij + kq*kk] : TF(0);
-            qs[m] = ok ? a.s[ij + kq*kk] : TF(0);
+            const size_t kq = size_t(min(max(kb - 3 + m, 0), g.kcells - 1))*kk;
+            qu[m] = ldo(u + kq, oij); qv[m] = ldo(v + kq, oij);
+            qw[m] = ldo(w + kq, oij); qs[m] = ldo(s + kq, oij);
         }
 
         const int c = (ty + H)*PW + (tx + H);     // centre in the halo-3 planes
@@ -199,169 +218,175 @@ namespace
         // Planes of the first level of the chunk go straight to LDS; the planes
         // of every next level are fetched into registers while the current one
         // is computed (software pipelining over k).
-        load_plane(Uc, a.u, kb, i0, j0, H, PW, PN, g, tid);
-        load_plane(Vc, a.v, kb, i0, j0, H, PW, PN, g, tid);
-        load_plane(Wn, a.w, kb+1, i0, j0, H, PW, PN, g, tid);
-        load_plane(Sc, a.s, kb, i0, j0, H, PW, PN, g, tid);
-        load_plane(En, a.evisc, kb+1, i0, j0, 1, EW, EN, g, tid);
+        load_plane(Uc, u, kb, i0, j0, H, PW, PN, g, tid);
+        load_plane(Vc, v, kb, i0, j0, H, PW, PN, g, tid);
+        load_plane(Wn, w, kb+1, i0, j0, H, PW, PN, g, tid);
+        load_plane(Sc, s, kb, i0, j0, H, PW, PN, g, tid);
+        load_plane(En, evisc, kb+1, i0, j0, 1, EW, EN, g, tid);
         TF ru[NQ3], rv[NQ3], rw[NQ3], rs[NQ3], re[NQ1];
-        int o3[NQ3], o1[NQ1];
-        plane_offsets(o3, i0, j0, H, PW, PN, g, tid);
-        plane_offsets(o1, i0, j0, 1, EW, EN, g, tid);
+        uint32_t o3[NQ3], o1[NQ1];
+        plane_offsets<TF>(o3, i0, j0, H, PW, PN, g, tid);
+        plane_offsets<TF>(o1, i0, j0, 1, EW, EN, g, tid);
 
         // Bottom-face terms of the first level of the chunk; every later level
         // takes them from the top face of the level below.
         TF bu_c, bu_u, bv_c, bv_u, bw_c, bw_u, bs_c, bs_u;   // advective
         TF du_b, dv_b, dw_b, ds_b;                          // diffusive
         __syncthreads();
-        if (active)
         {
-            vface_c(kb, g.kstart, g.kend, a.rhoh[kb], i2(Wc[c-1], Wc[c]),
+            const Level<TF> L = lev[kb], Lm = lev[kb-1];
+            vface_c(kb, g.kstart, g.kend, L.rhoh, i2(Wc[c-1], Wc[c]),
                     qu[0], qu[1], qu[2], qu[3], qu[4], qu[5], bu_c, bu_u);
-            vface_c(kb, g.kstart, g.kend, a.rhoh[kb], i2(Wc[c-PW], Wc[c]),
+            vface_c(kb, g.kstart, g.kend, L.rhoh, i2(Wc[c-PW], Wc[c]),
                     qv[0], qv[1], qv[2], qv[3], qv[4], qv[5], bv_c, bv_u);
-            vface_c(kb, g.kstart, g.kend, a.rhoh[kb], Wc[c],
+            vface_c(kb, g.kstart, g.kend, L.rhoh, Wc[c],
                     qs[0], qs[1], qs[2], qs[3], qs[4], qs[5], bs_c, bs_u);
-            vface_w(kb-1, g.kstart, g.kend, a.rho[kb-1],
+            vface_w(kb-1, g.kstart, g.kend, Lm.rho,
                     qw[0], qw[1], qw[2], qw[3], qw[4], qw[5], bw_c, bw_u);
             const TF eu = TF(0.25)*(Ep[ce-1] + Ep[ce] + Ec[ce-1] + Ec[ce]) + visc;
-            du_b = a.rhoh[kb] * eu*((qu[3]-qu[2])*a.dzhi[kb] + (Wc[c]-Wc[c-1])*dxi);
+            du_b = L.rhoh * eu*((qu[3]-qu[2])*L.dzhi + (Wc[c]-Wc[c-1])*dxi);
             const TF ev = TF(0.25)*(Ep[ce-EW] + Ep[ce] + Ec[ce-EW] + Ec[ce]) + visc;
-            dv_b = a.rhoh[kb] * ev*((qv[3]-qv[2])*a.dzhi[kb] + (Wc[c]-Wc[c-PW])*dyi);
-            dw_b = a.rho[kb-1] * (Ep[ce] + visc)*(qw[3]-qw[2])*a.dzi[kb-1];
+            dv_b = L.rhoh * ev*((qv[3]-qv[2])*L.dzhi + (Wc[c]-Wc[c-PW])*dyi);
+            dw_b = Lm.rho * (Ep[ce] + visc)*(qw[3]-qw[2])*Lm.dzi;
             const TF es = TF(0.5)*(Ep[ce]+Ec[ce]) * a.tPr_i + a.svisc;
-            ds_b = a.rhoh[kb] * es*(qs[3]-qs[2])*a.dzhi[kb];
+            ds_b = L.rhoh * es*(qs[3]-qs[2])*L.dzhi;
         }
+        // Every load of the preamble has landed: the waits inside the loop then
+        // only count the loads issued in the loop.
+        __builtin_amdgcn_s_waitcnt(0);
 
         for (int k = kb; k < ke; ++k)
         {
-            const bool more = k + 1 < ke;
-            if (more)
+            const size_t kofs = size_t(k)*kk;
+            TF* __restrict__ utk = ut + kofs; TF* __restrict__ vtk = vt + kofs;
+            TF* __restrict__ wtk = wt + kofs; TF* __restrict__ stk = st + kofs;
+            // Issue order sets what a wait covers (loads complete in order):
+            // the tendencies of this level first, then the column values of
+            // level k+4 (used by the next level), then the planes of the next
+            // level, which stay in flight until the end of this one.  Every
+            // level issues the same loads (the last one of the chunk fetches
+            // clamped, unused planes), so the waits are the same on every pass.
+            TF tu = ldo(utk, oij), tv = ldo(vtk, oij), tw = ldo(wtk, oij), ts = ldo(stk, oij);
+            __builtin_amdgcn_sched_barrier(0);
+            const size_t kq = size_t(min(k + 4, g.kcells - 1))*kk;
+            const TF nu = ldo(u + kq, oij), nv = ldo(v + kq, oij), nw = ldo(w + kq, oij), ns = ldo(s + kq, oij);
+            __builtin_amdgcn_sched_barrier(0);
             {
-                fetch_plane(ru, a.u, k+1, o3, g);
-                fetch_plane(rv, a.v, k+1, o3, g);
-                fetch_plane(rw, a.w, k+2, o3, g);
-                fetch_plane(rs, a.s, k+1, o3, g);
-                fetch_plane(re, a.evisc, k+2, o1, g);
+                const int k1 = min(k + 1, g.kcells - 1), k2 = min(k + 2, g.kcells - 1);
+                fetch_plane(ru, u, k1, o3, g);
+                fetch_plane(rv, v, k1, o3, g);
+                fetch_plane(rw, w, k2, o3, g);
+                fetch_plane(rs, s, k1, o3, g);
+                fetch_plane(re, evisc, k2, o1, g);
             }
             __syncthreads();
 
-            if (active)
-            {
-                const int ijk = ij + k*kk;
-                const TF rhok = a.rho[k], rhohk = a.rhoh[k], rhohkp = a.rhoh[k+1];
-                const TF dzhik = a.dzhi[k], dzhikp = a.dzhi[k+1];
-                const TF dzik = a.dzi[k];
-                // dzi/rhoref and dzhi/rhorefh: the divisions of the reference's
-                // "/ rhoref[k] * dzi[k]" taken out of the per-point work
-                const TF cf = a.cfac[k], wf = a.wfac[k];
+            const Level<TF> L = lev[k];
+            // L.cfac, L.wfac: dzi/rhoref and dzhi/rhorefh, the divisions of the
+            // reference's "/ rhoref[k] * dzi[k]" taken out of the per-point work
+            const TF cf = L.cfac, wf = L.wfac;
 
-                // ---------------------------------------------------------- u
+            // -------------------------------------------------------------- u
+            {
+                tu += hsum(hflux_2i5(i2(Uc[c], Uc[c+1]), i2(Uc[c-1], Uc[c]), Uc, c, 1, dxi),
+                           hflux_2i5(i2(Vc[c-1+PW], Vc[c+PW]), i2(Vc[c-1], Vc[c]), Uc, c, PW, dyi));
+                TF tc, tq;
+                vface_c(k+1, g.kstart, g.kend, L.rhohp, i2(Wn[c-1], Wn[c]),
+                        qu[1], qu[2], qu[3], qu[4], qu[5], qu[6], tc, tq);
+                tu += - ( tc - bu_c ) * cf + ( tq - bu_u ) * cf;
+                bu_c = tc; bu_u = tq;
+                const TF evisce = Ec[ce] + visc;
+                const TF eviscw = Ec[ce-1] + visc;
+                const TF eviscn = TF(0.25)*(Ec[ce-1] + Ec[ce] + Ec[ce-1+EW] + Ec[ce+EW]) + visc;
+                const TF eviscs = TF(0.25)*(Ec[ce-1-EW] + Ec[ce-EW] + Ec[ce-1] + Ec[ce]) + visc;
+                const TF evisct = TF(0.25)*(Ec[ce-1] + Ec[ce] + En[ce-1] + En[ce]) + visc;
+                const TF hor =
+                         + ( evisce*(Uc[c+1]-Uc[c  ])*dxi
+                           - eviscw*(Uc[c  ]-Uc[c-1])*dxi ) * TF(2.)*dxi
+                         + ( eviscn*((Uc[c+PW]-Uc[c   ])*dyi + (Vc[c+PW]-Vc[c-1+PW])*dxi)
+                           - eviscs*((Uc[c   ]-Uc[c-PW])*dyi + (Vc[c   ]-Vc[c-1   ])*dxi) ) * dyi;
+                const TF dt = L.rhohp * evisct*((qu[4]-qu[3])*L.dzhip + (Wn[c]-Wn[c-1])*dxi);
+                const TF ver = ( dt - du_b ) * cf;
+                du_b = dt;
+                tu += hor + ver;
+                if (active) sto(utk, oij, tu);
+            }
+            // -------------------------------------------------------------- v
+            {
+                tv += hsum(hflux_2i5(i2(Uc[c+1-PW], Uc[c+1]), i2(Uc[c-PW], Uc[c]), Vc, c, 1, dxi),
+                           hflux_2i5(i2(Vc[c], Vc[c+PW]), i2(Vc[c-PW], Vc[c]), Vc, c, PW, dyi));
+                TF tc, tq;
+                vface_c(k+1, g.kstart, g.kend, L.rhohp, i2(Wn[c-PW], Wn[c]),
+                        qv[1], qv[2], qv[3], qv[4], qv[5], qv[6], tc, tq);
+                tv += - ( tc - bv_c ) * cf + ( tq - bv_u ) * cf;
+                bv_c = tc; bv_u = tq;
+                const TF evisce = TF(0.25)*(Ec[ce-EW] + Ec[ce] + Ec[ce+1-EW] + Ec[ce+1]) + visc;
+                const TF eviscw = TF(0.25)*(Ec[ce-1-EW] + Ec[ce-1] + Ec[ce-EW] + Ec[ce]) + visc;
+                const TF eviscn = Ec[ce] + visc;
+                const TF eviscs = Ec[ce-EW] + visc;
+                const TF evisct = TF(0.25)*(Ec[ce-EW] + Ec[ce] + En[ce-EW] + En[ce]) + visc;
+                const TF hor =
+                         + ( evisce*((Vc[c+1]-Vc[c  ])*dxi + (Uc[c+1]-Uc[c+1-PW])*dyi)
+                           - eviscw*((Vc[c  ]-Vc[c-1])*dxi + (Uc[c  ]-Uc[c  -PW])*dyi) ) * dxi
+                         + ( eviscn*(Vc[c+PW]-Vc[c   ])*dyi
+                           - eviscs*(Vc[c   ]-Vc[c-PW])*dyi ) * TF(2.)*dyi;
+                const TF dt = L.rhohp * evisct*((qv[4]-qv[3])*L.dzhip + (Wn[c]-Wn[c-PW])*dyi);
+                const TF ver = ( dt - dv_b ) * cf;
+                dv_b = dt;
+                tv += hor + ver;
+                if (active) sto(vtk, oij, tv);
+            }
+            // -------------------------------------------------------------- w
+            {
+                // faces at the cell centres k (top) and k-1 (bottom, carried)
+                TF tc, tq;
+                vface_w(k, g.kstart, g.kend, L.rho, qw[1], qw[2], qw[3], qw[4], qw[5], qw[6], tc, tq);
+                const TF dt = L.rho * (Ec[ce] + visc)*(qw[4]-qw[3])*L.dzi;
+                if (k > g.kstart)
                 {
-                    TF t = a.ut[ijk];
-                    t += hsum(hflux_2i5(i2(Uc[c], Uc[c+1]), i2(Uc[c-1], Uc[c]), Uc, c, 1, dxi),
-                              hflux_2i5(i2(Vc[c-1+PW], Vc[c+PW]), i2(Vc[c-1], Vc[c]), Uc, c, PW, dyi));
-                    TF tc, tu;
-                    vface_c(k+1, g.kstart, g.kend, rhohkp, i2(Wn[c-1], Wn[c]),
-                            qu[1], qu[2], qu[3], qu[4], qu[5], qu[6], tc, tu);
-                    t += - ( tc - bu_c ) * cf + ( tu - bu_u ) * cf;
-                    bu_c = tc; bu_u = tu;
-                    const TF evisce = Ec[ce] + visc;
-                    const TF eviscw = Ec[ce-1] + visc;
-                    const TF eviscn = TF(0.25)*(Ec[ce-1] + Ec[ce] + Ec[ce-1+EW] + Ec[ce+EW]) + visc;
-                    const TF eviscs = TF(0.25)*(Ec[ce-1-EW] + Ec[ce-EW] + Ec[ce-1] + Ec[ce]) + visc;
-                    const TF evisct = TF(0.25)*(Ec[ce-1] + Ec[ce] + En[ce-1] + En[ce]) + visc;
-                    const TF hor =
-                             + ( evisce*(Uc[c+1]-Uc[c  ])*dxi
-                               - eviscw*(Uc[c  ]-Uc[c-1])*dxi ) * TF(2.)*dxi
-                             + ( eviscn*((Uc[c+PW]-Uc[c   ])*dyi + (Vc[c+PW]-Vc[c-1+PW])*dxi)
-                               - eviscs*((Uc[c   ]-Uc[c-PW])*dyi + (Vc[c   ]-Vc[c-1   ])*dxi) ) * dyi;
-                    const TF dt = rhohkp * evisct*((qu[4]-qu[3])*dzhikp + (Wn[c]-Wn[c-1])*dxi);
-                    const TF ver = ( dt - du_b ) * cf;
-                    du_b = dt;
-                    t += hor + ver;
-                    a.ut[ijk] = t;
+                    if (a.buoyancy)
+                        tw += L.gthi * (i2(qs[2], qs[3]) - L.threfh_rel);
+                    tw += hsum(hflux_2i5(i2(Up[c+1], Uc[c+1]), i2(Up[c], Uc[c]), Wc, c, 1, dxi),
+                               hflux_2i5(i2(Vp[c+PW], Vc[c+PW]), i2(Vp[c], Vc[c]), Wc, c, PW, dyi));
+                    tw += - ( tc - bw_c ) * wf + ( tq - bw_u ) * wf;
+                    const TF evisce = TF(0.25)*(Ep[ce] + Ec[ce] + Ep[ce+1] + Ec[ce+1]) + visc;
+                    const TF eviscw = TF(0.25)*(Ep[ce-1] + Ec[ce-1] + Ep[ce] + Ec[ce]) + visc;
+                    const TF eviscn = TF(0.25)*(Ep[ce] + Ec[ce] + Ep[ce+EW] + Ec[ce+EW]) + visc;
+                    const TF eviscs = TF(0.25)*(Ep[ce-EW] + Ec[ce-EW] + Ep[ce] + Ec[ce]) + visc;
+                    tw += + ( evisce*((Wc[c+1]-Wc[c  ])*dxi + (Uc[c+1]-Up[c+1])*L.dzhi)
+                            - eviscw*((Wc[c  ]-Wc[c-1])*dxi + (Uc[c  ]-Up[c  ])*L.dzhi) ) * dxi
+                          + ( eviscn*((Wc[c+PW]-Wc[c   ])*dyi + (Vc[c+PW]-Vp[c+PW])*L.dzhi)
+                            - eviscs*((Wc[c   ]-Wc[c-PW])*dyi + (Vc[c   ]-Vp[c   ])*L.dzhi) ) * dyi
+                          + ( dt - dw_b ) * TF(2.)*wf;
+                    if (active) sto(wtk, oij, tw);
                 }
-                // ---------------------------------------------------------- v
-                {
-                    TF t = a.vt[ijk];
-                    t += hsum(hflux_2i5(i2(Uc[c+1-PW], Uc[c+1]), i2(Uc[c-PW], Uc[c]), Vc, c, 1, dxi),
-                              hflux_2i5(i2(Vc[c], Vc[c+PW]), i2(Vc[c-PW], Vc[c]), Vc, c, PW, dyi));
-                    TF tc, tu;
-                    vface_c(k+1, g.kstart, g.kend, rhohkp, i2(Wn[c-PW], Wn[c]),
-                            qv[1], qv[2], qv[3], qv[4], qv[5], qv[6], tc, tu);
-                    t += - ( tc - bv_c ) * cf + ( tu - bv_u ) * cf;
-                    bv_c = tc; bv_u = tu;
-                    const TF evisce = TF(0.25)*(Ec[ce-EW] + Ec[ce] + Ec[ce+1-EW] + Ec[ce+1]) + visc;
-                    const TF eviscw = TF(0.25)*(Ec[ce-1-EW] + Ec[ce-1] + Ec[ce-EW] + Ec[ce]) + visc;
-                    const TF eviscn = Ec[ce] + visc;
-                    const TF eviscs = Ec[ce-EW] + visc;
-                    const TF evisct = TF(0.25)*(Ec[ce-EW] + Ec[ce] + En[ce-EW] + En[ce]) + visc;
-                    const TF hor =
-                             + ( evisce*((Vc[c+1]-Vc[c  ])*dxi + (Uc[c+1]-Uc[c+1-PW])*dyi)
-                               - eviscw*((Vc[c  ]-Vc[c-1])*dxi + (Uc[c  ]-Uc[c  -PW])*dyi) ) * dxi
-                             + ( eviscn*(Vc[c+PW]-Vc[c   ])*dyi
-                               - eviscs*(Vc[c   ]-Vc[c-PW])*dyi ) * TF(2.)*dyi;
-                    const TF dt = rhohkp * evisct*((qv[4]-qv[3])*dzhikp + (Wn[c]-Wn[c-PW])*dyi);
-                    const TF ver = ( dt - dv_b ) * cf;
-                    dv_b = dt;
-                    t += hor + ver;
-                    a.vt[ijk] = t;
-                }
-                // ---------------------------------------------------------- w
-                {
-                    // faces at the cell centres k (top) and k-1 (bottom, carried)
-                    TF tc, tu;
-                    vface_w(k, g.kstart, g.kend, rhok, qw[1], qw[2], qw[3], qw[4], qw[5], qw[6], tc, tu);
-                    const TF dt = rhok * (Ec[ce] + visc)*(qw[4]-qw[3])*dzik;
-                    if (k > g.kstart)
-                    {
-                        TF t = a.wt[ijk];
-                        if (a.buoyancy)
-                            t += a.gthi[k] * (i2(qs[2], qs[3]) - a.threfh_rel[k]);
-                        t += hsum(hflux_2i5(i2(Up[c+1], Uc[c+1]), i2(Up[c], Uc[c]), Wc, c, 1, dxi),
-                                  hflux_2i5(i2(Vp[c+PW], Vc[c+PW]), i2(Vp[c], Vc[c]), Wc, c, PW, dyi));
-                        t += - ( tc - bw_c ) * wf + ( tu - bw_u ) * wf;
-                        const TF evisce = TF(0.25)*(Ep[ce] + Ec[ce] + Ep[ce+1] + Ec[ce+1]) + visc;
-                        const TF eviscw = TF(0.25)*(Ep[ce-1] + Ec[ce-1] + Ep[ce] + Ec[ce]) + visc;
-                        const TF eviscn = TF(0.25)*(Ep[ce] + Ec[ce] + Ep[ce+EW] + Ec[ce+EW]) + visc;
-                        const TF eviscs = TF(0.25)*(Ep[ce-EW] + Ec[ce-EW] + Ep[ce] + Ec[ce]) + visc;
-                        t += + ( evisce*((Wc[c+1]-Wc[c  ])*dxi + (Uc[c+1]-Up[c+1])*dzhik)
-                               - eviscw*((Wc[c  ]-Wc[c-1])*dxi + (Uc[c  ]-Up[c  ])*dzhik) ) * dxi
-                             + ( eviscn*((Wc[c+PW]-Wc[c   ])*dyi + (Vc[c+PW]-Vp[c+PW])*dzhik)
-                               - eviscs*((Wc[c   ]-Wc[c-PW])*dyi + (Vc[c   ]-Vp[c   ])*dzhik) ) * dyi
-                             + ( dt - dw_b ) * TF(2.)*wf;
-                        a.wt[ijk] = t;
-                    }
-                    bw_c = tc; bw_u = tu; dw_b = dt;
-                }
-                // ---------------------------------------------------------- s
-                {
-                    TF t = a.st[ijk];
-                    t += hsum(hflux_2i5(Uc[c+1], Uc[c], Sc, c, 1, dxi),
-                              hflux_2i5(Vc[c+PW], Vc[c], Sc, c, PW, dyi));
-                    TF tc, tu;
-                    vface_c(k+1, g.kstart, g.kend, rhohkp, Wn[c],
-                            qs[1], qs[2], qs[3], qs[4], qs[5], qs[6], tc, tu);
-                    t += - ( tc - bs_c ) * cf + ( tu - bs_u ) * cf;
-                    bs_c = tc; bs_u = tu;
-                    const TF ti = a.tPr_i, sv = a.svisc;
-                    const TF evisce = TF(0.5)*(Ec[ce   ]+Ec[ce+1 ]) * ti + sv;
-                    const TF eviscw = TF(0.5)*(Ec[ce-1 ]+Ec[ce   ]) * ti + sv;
-                    const TF eviscn = TF(0.5)*(Ec[ce   ]+Ec[ce+EW]) * ti + sv;
-                    const TF eviscs = TF(0.5)*(Ec[ce-EW]+Ec[ce   ]) * ti + sv;
-                    const TF evisct = TF(0.5)*(Ec[ce   ]+En[ce   ]) * ti + sv;
-                    const TF hor =
-                             + ( evisce*(Sc[c+1]-Sc[c  ])
-                               - eviscw*(Sc[c  ]-Sc[c-1]) ) * a.dxidxi
-                             + ( eviscn*(Sc[c+PW]-Sc[c   ])
-                               - eviscs*(Sc[c   ]-Sc[c-PW]) ) * a.dyidyi;
-                    const TF dt = rhohkp * evisct*(qs[4]-qs[3])*dzhikp;
-                    const TF ver = ( dt - ds_b ) * cf;
-                    ds_b = dt;
-                    t += hor + ver;
-                    a.st[ijk] = t;
-                }
+                bw_c = tc; bw_u = tq; dw_b = dt;
+            }
+            // -------------------------------------------------------------- s
+            {
+                ts += hsum(hflux_2i5(Uc[c+1], Uc[c], Sc, c, 1, dxi),
+                           hflux_2i5(Vc[c+PW], Vc[c], Sc, c, PW, dyi));
+                TF tc, tq;
+                vface_c(k+1, g.kstart, g.kend, L.rhohp, Wn[c],
+                        qs[1], qs[2], qs[3], qs[4], qs[5], qs[6], tc, tq);
+                ts += - ( tc - bs_c ) * cf + ( tq - bs_u ) * cf;
+                bs_c = tc; bs_u = tq;
+                const TF tPr_i = a.tPr_i, sv = a.svisc;
+                const TF evisce = TF(0.5)*(Ec[ce   ]+Ec[ce+1 ]) * tPr_i + sv;
+                const TF eviscw = TF(0.5)*(Ec[ce-1 ]+Ec[ce   ]) * tPr_i + sv;
+                const TF eviscn = TF(0.5)*(Ec[ce   ]+Ec[ce+EW]) * tPr_i + sv;
+                const TF eviscs = TF(0.5)*(Ec[ce-EW]+Ec[ce   ]) * tPr_i + sv;
+                const TF evisct = TF(0.5)*(Ec[ce   ]+En[ce   ]) * tPr_i + sv;
+                const TF hor =
+                         + ( evisce*(Sc[c+1]-Sc[c  ])
+                           - eviscw*(Sc[c  ]-Sc[c-1]) ) * a.dxidxi
+                         + ( eviscn*(Sc[c+PW]-Sc[c   ])
+                           - eviscs*(Sc[c   ]-Sc[c-PW]) ) * a.dyidyi;
+                const TF dt = L.rhohp * evisct*(qs[4]-qs[3])*L.dzhip;
+                const TF ver = ( dt - ds_b ) * cf;
+                ds_b = dt;
+                ts += hor + ver;
+                if (active) sto(stk, oij, ts);
             }
             __syncthreads();
 
@@ -371,25 +396,17 @@ namespace
             x = Vp; Vp = Vc; Vc = x;
             x = Wc; Wc = Wn; Wn = x;
             x = Ep; Ep = Ec; Ec = En; En = x;
-            if (more)
-            {
-                store_plane(Uc, ru, PN, tid);
-                store_plane(Vc, rv, PN, tid);
-                store_plane(Wn, rw, PN, tid);
-                store_plane(Sc, rs, PN, tid);
-                store_plane(En, re, EN, tid);
-            }
+            store_plane(Uc, ru, PN, tid);
+            store_plane(Vc, rv, PN, tid);
+            store_plane(Wn, rw, PN, tid);
+            store_plane(Sc, rs, PN, tid);
+            store_plane(En, re, EN, tid);
             #pragma unroll
             for (int m = 0; m < 6; ++m)
             {
                 qu[m] = qu[m+1]; qv[m] = qv[m+1]; qw[m] = qw[m+1]; qs[m] = qs[m+1];
             }
-            const int kq = k + 4;
-            const bool ok = active && kq < g.kcells;
-            qu[6] = ok ? a.u[ij + kq*kk] : TF(0);
-            qv[6] = ok ? a.v[ij + kq*kk] : TF(0);
-            qw[6] = ok ? a.w[ij + kq*kk] : TF(0);
-            qs[6] = ok ? a.s[ij + kq*kk] : TF(0);
+            qu[6] = nu; qv[6] = nv; qw[6] = nw; qs[6] = ns;
         }
     }
 
@@ -405,14 +422,9 @@ namespace
             return fail(MHH_EINVAL, "the fused tendency covers the default boundary (no surface model)");
         if (buoyancy && !p.threfh)
             return fail(MHH_ESTATE, "buoyancy needs mhh_set_base_state()");
+        if (!p.lev)
+            return fail(MHH_ESTATE, "the fused tendency needs the grid profiles");
         Tend_args<TF> a{};
-        a.ut = static_cast<TF*>(c->f.ut); a.vt = static_cast<TF*>(c->f.vt); a.wt = static_cast<TF*>(c->f.wt);
-        a.st = static_cast<TF*>(c->f.st[0]);
-        a.u = static_cast<const TF*>(c->f.u); a.v = static_cast<const TF*>(c->f.v);
-        a.w = static_cast<const TF*>(c->f.w); a.s = static_cast<const TF*>(c->f.s[0]);
-        a.evisc = static_cast<const TF*>(c->f.evisc);
-        a.dzi = p.dzi; a.dzhi = p.dzhi; a.rho = p.rhoref; a.rhoh = p.rhorefh; a.threfh = p.threfh; a.threfh_rel = p.threfh_rel;
-        a.cfac = p.cfac; a.wfac = p.wfac; a.gthi = p.gthi;
         a.dxi = TF(1.)/TF(g.dx); a.dyi = TF(1.)/TF(g.dy);
         a.dxidxi = TF(1./(g.dx*g.dx)); a.dyidyi = TF(1./(g.dy*g.dy));
         a.visc = TF(par.visc); a.svisc = TF(par.svisc[0]); a.tPr_i = TF(1)/TF(par.tPr);
@@ -423,7 +435,12 @@ namespace
         a.nkc = (g.ktot + a.kchunk - 1)/a.kchunk;
         const int ntiles = a.nti*a.ntj*a.nkc;
         const int nblocks = (ntiles + 7)/8*8;
-        { mhh::Timer _t(c, "tend_2i5_smag2"); tend_2i5_smag2_g<TF><<<nblocks, dim3(TX, TY), 0, c->stream>>>(a, g); }
+        { mhh::Timer _t(c, "tend_2i5_smag2");
+          tend_2i5_smag2_g<TF><<<nblocks, dim3(TX, TY), 0, c->stream>>>(
+                static_cast<TF*>(c->f.ut), static_cast<TF*>(c->f.vt), static_cast<TF*>(c->f.wt),
+                static_cast<TF*>(c->f.st[0]),
+                static_cast<const TF*>(c->f.u), static_cast<const TF*>(c->f.v), static_cast<const TF*>(c->f.w),
+                static_cast<const TF*>(c->f.s[0]), static_cast<const TF*>(c->f.evisc), p.lev, a, g); }
         return check_launch("tend_2i5_smag2");
     }
 }
