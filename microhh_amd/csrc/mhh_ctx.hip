@@ -51,6 +51,8 @@ namespace
             l.wfac = TF(rhoh != 0. ? at(c->h_dzhi, k)/rhoh : 0.);
             l.gthi = TF(threfh != 0. ? 9.81/threfh : 0.);
             l.threfh_rel = TF(c->h_threfh.empty() ? 0. : threfh - c->s_offset[0]);
+            const double thref = at(c->h_thref, k);
+            l.n2fac = TF(thref != 0. ? 9.81/thref*0.5*at(c->h_dzi, k) : 0.);
         }
         if (!p.lev)
             MHH_CHECK(mhh::check_hip(hipMalloc(&p.lev, sizeof(mhh::Level<TF>)*n), "hipMalloc(levels)"));
@@ -221,6 +223,7 @@ int mhh_ctx_create(const mhh_grid_desc* g, int precision, void* hip_stream, mhh_
     c->mixed = precision == MHH_MIXED;
     // levels per block of the fused tendency kernel (a tuning choice, same results)
     if (const char* e = std::getenv("MHH_TEND_KCHUNK")) c->tend_kchunk = std::max(0, std::atoi(e));
+    if (const char* e = std::getenv("MHH_FFT_GENERIC")) c->fft_generic = e[0] == '1';
     c->precision = c->mixed ? MHH_FP32 : precision;
     c->stream = static_cast<hipStream_t>(hip_stream);
     (void)hipGetDevice(&c->device);

@@ -226,29 +226,49 @@ namespace
     }
 
     template<typename TF>
+    __device__ __forceinline__ TF vldo(const TF* __restrict__ base, uint32_t off)
+    {
+        return *reinterpret_cast<const TF*>(reinterpret_cast<const char*>(base) + off);
+    }
+
+    // Unconditional loads of the thread's share of plane k (k < kcells) at the
+    // byte offsets ob (0 for elements past the plane or outside the field).
+    template<typename TF>
     __device__ __forceinline__ void vplane_fetch(TF (&r)[VNQ], const TF* __restrict__ src, int k,
-                                                 const int (&o)[VNQ], const Gd& g)
+                                                 const uint32_t (&ob)[VNQ], const Gd& g)
     {
         const TF* __restrict__ sk = src + static_cast<size_t>(k)*g.ijcells;
         #pragma unroll
-        for (int q = 0; q < VNQ; ++q) r[q] = o[q] >= 0 ? sk[o[q]] : TF(0);
+        for (int q = 0; q < VNQ; ++q) r[q] = vldo(sk, ob[q]);
     }
 
     template<typename TF>
     __device__ __forceinline__ void vplane_store(TF* __restrict__ dst, const TF (&r)[VNQ], int tid)
     {
         #pragma unroll
-        for (int q = 0; q < VNQ; ++q)
-        {
-            const int idx = tid + q*VTX*VTY;
-            if (idx < VPN) dst[idx] = r[q];
-        }
+        for (int q = 0; q < VNQ-1; ++q)
+            dst[tid + q*VTX*VTY] = r[q];
+        const int idx = tid + (VNQ-1)*VTX*VTY;
+        if (idx < VPN) dst[idx] = r[VNQ-1];
     }
 
+    template<typename TF>
+    struct Visc_scalars
+    {
+        TF dxi, dyi, tPr_i, visc, zsize, tPrfac_i, dxidxi, dyidyi;
+    };
+
+    // Viscosity of the default boundary (no surface model) over a 64 x 4 tile
+    // marching up a chunk of levels: u, v at k-1, k, k+1 and w at k, k+1 in an
+    // LDS plane ring (halo 1).  Pointers are __restrict__ kernel parameters,
+    // so the per-level factors come in through scalar loads.  With `limits`
+    // also the block maxima of the 2i5 CFL rate and of dnmul.
     template<typename TF, bool thermo, bool limits>
-    __global__ void __launch_bounds__(VTX*VTY) visc_tile_g(Visc_args<TF> a, Gd g, int nti, int ntj, int nkc,
-                                                           double* __restrict__ red_cfl, double* __restrict__ red_dn,
-                                                           TF tPrfac_i, TF dxidxi, TF dyidyi)
+    __global__ void __launch_bounds__(VTX*VTY) visc_tile_g(
+            TF* __restrict__ evisc, const TF* __restrict__ u, const TF* __restrict__ v,
+            const TF* __restrict__ w, const TF* __restrict__ th, const Level<TF>* __restrict__ lev,
+            const TF* __restrict__ mlen0, const TF* __restrict__ zc, Visc_scalars<TF> a, Gd g,
+            int nti, int ntj, int nkc, double* __restrict__ red_cfl, double* __restrict__ red_dn)
     {
         __shared__ TF lds[8*VPN];
         __shared__ double smc[limits ? VTX*VTY : 1], smd[limits ? VTX*VTY : 1];
@@ -266,9 +286,13 @@ namespace
         const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
         const bool active = i < g.iend && j < g.jend;
         const int kb = g.kstart + kc*VKCHUNK, ke = min(kb + VKCHUNK, g.kend);
-        const int ij = i + j*g.icells, kk = g.ijcells;
+        const int kk = g.ijcells;
+        // threads outside the domain compute on the clamped column and store nothing
+        const int ij = min(i, g.iend-1) + min(j, g.jend-1)*g.icells;
+        const uint32_t oij = uint32_t(ij)*sizeof(TF);
 
         int o[VNQ];
+        uint32_t ob[VNQ];
         #pragma unroll
         for (int q = 0; q < VNQ; ++q)
         {
@@ -276,97 +300,114 @@ namespace
             const int pj = idx / VPW, pi = idx - pj*VPW;
             const int gi = i0 - 1 + pi, gj = j0 - 1 + pj;
             o[q] = (idx < VPN && gi < g.icells && gj < g.jcells) ? gi + gj*g.icells : -1;
+            ob[q] = o[q] >= 0 ? uint32_t(o[q])*sizeof(TF) : 0u;
         }
         TF* Um = lds;        TF* Uc = Um + VPN;   TF* Up = Uc + VPN;
         TF* Vm = Up + VPN;   TF* Vc = Vm + VPN;   TF* Vp = Vc + VPN;
         TF* Wc = Vp + VPN;   TF* Wp = Wc + VPN;
-        vplane_load(Um, a.u, kb-1, o, tid, g); vplane_load(Uc, a.u, kb, o, tid, g); vplane_load(Up, a.u, kb+1, o, tid, g);
-        vplane_load(Vm, a.v, kb-1, o, tid, g); vplane_load(Vc, a.v, kb, o, tid, g); vplane_load(Vp, a.v, kb+1, o, tid, g);
-        vplane_load(Wc, a.w, kb, o, tid, g);   vplane_load(Wp, a.w, kb+1, o, tid, g);
+        vplane_load(Um, u, kb-1, o, tid, g); vplane_load(Uc, u, kb, o, tid, g); vplane_load(Up, u, kb+1, o, tid, g);
+        vplane_load(Vm, v, kb-1, o, tid, g); vplane_load(Vc, v, kb, o, tid, g); vplane_load(Vp, v, kb+1, o, tid, g);
+        vplane_load(Wc, w, kb, o, tid, g);   vplane_load(Wp, w, kb+1, o, tid, g);
 
         // theta at k-1, k+1 of the own column (N2, src/thermo_dry.cxx:66-79)
         TF t0 = TF(0), t1 = TF(0), t2 = TF(0);
-        if (thermo && active) { t0 = a.th[ij + (kb-1)*kk]; t1 = a.th[ij + kb*kk]; t2 = a.th[ij + (kb+1)*kk]; }
+        if (thermo)
+        {
+            t0 = vldo(th + size_t(kb-1)*kk, oij); t1 = vldo(th + size_t(kb)*kk, oij);
+            t2 = vldo(th + size_t(kb+1)*kk, oij);
+        }
         // van Driest wall friction velocities of the column (neutral case)
         TF u_tau_bot = TF(0), u_tau_top = TF(0);
-        if (!thermo && active)
+        if (!thermo)
         {
-            const TF* __restrict__ u = a.u; const TF* __restrict__ v = a.v;
             const int kbot = ij + g.kstart*kk, ktop = ij + g.kend*kk;
-            u_tau_bot = pow(sq(a.visc*(u[kbot] - u[kbot-kk])*a.dzhi[g.kstart])
-                          + sq(a.visc*(v[kbot] - v[kbot-kk])*a.dzhi[g.kstart]), TF(0.25));
-            u_tau_top = pow(sq(a.visc*(u[ktop] - u[ktop-kk])*a.dzhi[g.kend])
-                          + sq(a.visc*(v[ktop] - v[ktop-kk])*a.dzhi[g.kend]), TF(0.25));
+            u_tau_bot = pow(sq(a.visc*(u[kbot] - u[kbot-kk])*lev[g.kstart].dzhi)
+                          + sq(a.visc*(v[kbot] - v[kbot-kk])*lev[g.kstart].dzhi), TF(0.25));
+            u_tau_top = pow(sq(a.visc*(u[ktop] - u[ktop-kk])*lev[g.kend].dzhi)
+                          + sq(a.visc*(v[ktop] - v[ktop-kk])*lev[g.kend].dzhi), TF(0.25));
         }
         const int c = (threadIdx.y + 1)*VPW + (threadIdx.x + 1);
         double cm = 0., dm = 0.;
         TF ru[VNQ], rv[VNQ], rw[VNQ];
-        TF tn = TF(0);
+        __builtin_amdgcn_s_waitcnt(0);
 
         for (int k = kb; k < ke; ++k)
         {
-            const bool more = k + 1 < ke;
-            if (more)
+            // Issue order sets what a wait covers (loads complete in order): the
+            // CFL stencil of this level first, then theta of level k+2, then the
+            // planes of level k+2, which stay in flight until the end of the
+            // level.  Every level issues the same loads (clamped levels).
+            TF cu[6], cv[6], cw[6];
+            if (limits)
             {
-                vplane_fetch(ru, a.u, k+2, o, g);
-                vplane_fetch(rv, a.v, k+2, o, g);
-                vplane_fetch(rw, a.w, k+2, o, g);
-                if (thermo && active) tn = a.th[ij + (k+2)*kk];
+                const TF* __restrict__ uk = u + size_t(k)*kk;
+                const TF* __restrict__ vk = v + size_t(k)*kk;
+                #pragma unroll
+                for (int m = 0; m < 6; ++m)
+                {
+                    cu[m] = vldo(uk, oij + uint32_t((m-2)*int(sizeof(TF))));
+                    cv[m] = vldo(vk, oij + uint32_t((m-2)*g.icells*int(sizeof(TF))));
+                    cw[m] = vldo(w + size_t(min(max(k-2+m, 0), g.kcells-1))*kk, oij);
+                }
+                __builtin_amdgcn_sched_barrier(0);
             }
+            const int k2 = min(k + 2, g.kcells - 1);
+            TF tn = TF(0);
+            if (thermo) tn = vldo(th + size_t(k2)*kk, oij);
+            __builtin_amdgcn_sched_barrier(0);
+            vplane_fetch(ru, u, k2, ob, g);
+            vplane_fetch(rv, v, k2, ob, g);
+            vplane_fetch(rw, w, k2, ob, g);
             __syncthreads();
-            if (active)
+
+            const Level<TF> L = lev[k];
+            const TF s2 = strain2_planes(Um, Uc, Up, Vm, Vc, Vp, Wc, Wp, c, a.dxi, a.dyi, L.dzi, L.dzhi, L.dzhip);
+            TF ev;
+            if (thermo)
             {
-                const int ijk = ij + k*kk;
-                const TF s2 = strain2_planes(Um, Uc, Up, Vm, Vc, Vp, Wc, Wp, c, a.dxi, a.dyi,
-                                             a.dzi[k], a.dzhi[k], a.dzhi[k+1]);
-                TF ev;
-                if (thermo)
-                {
-                    const TF num = TF(grav)/a.thref[k]*TF(0.5)*(t2 - t0)*a.dzi[k];
-                    TF RitPrratio = num / s2 / a.tPr;
-                    RitPrratio = fmin(RitPrratio, TF(1.-dsmall));
-                    const TF mlen = a.mlen0[k];
-                    ev = mlen*mlen * sqrt(s2) * sqrt(TF(1.)-RitPrratio);
-                }
+                // mlen^2 sqrt(S2) sqrt(1 - min(N2/(S2 tPr), 1 - dsmall)) of
+                // calc_evisc (src/diff_smag2.cxx:148-269), evaluated as
+                // mlen^2 sqrt(max(S2 - N2/tPr, dsmall S2)): the same value
+                // for S2 > 0 (S2 >= dsmall here), without a division and
+                // with one square root.
+                const TF n2_tPr = L.n2fac*(t2 - t0)*a.tPr_i;
+                const TF mlen = mlen0[k];
+                ev = mlen*mlen * sqrt(fmax(s2 - n2_tPr, TF(dsmall)*s2));
+            }
+            else
+            {
+                const TF A_vandriest = TF(26.);
+                const TF zk = zc[k];
+                const TF fac_bot = TF(1.) - exp(-(        zk *u_tau_bot) / (A_vandriest*a.visc));
+                const TF fac_top = TF(1.) - exp(-((a.zsize-zk)*u_tau_top) / (A_vandriest*a.visc));
+                const TF fac = fmin(fac_bot, fac_top);
+                ev = sq(fac * mlen0[k]) * sqrt(s2);
+            }
+            if (active) *reinterpret_cast<TF*>(reinterpret_cast<char*>(evisc + size_t(k)*kk) + oij) = ev;
+            if (limits && active)
+            {
+                dm = fmax(dm, static_cast<double>(fabs(ev*a.tPrfac_i*(a.dxidxi + a.dyidyi + L.dzi*L.dzi))));
+                // the 2i5 CFL rate as in cfl_g
+                TF wpart;
+                if (k == g.kstart || k == g.kend-1)
+                    wpart = fabs(i2(cw[2], cw[3]))*L.dzi;
+                else if (k == g.kstart+1 || k == g.kend-2)
+                    wpart = fabs(i4ws(cw[1], cw[2], cw[3], cw[4]))*L.dzi;
                 else
-                {
-                    const TF A_vandriest = TF(26.);
-                    const TF fac_bot = TF(1.) - exp(-(        a.z[k] *u_tau_bot) / (A_vandriest*a.visc));
-                    const TF fac_top = TF(1.) - exp(-((a.zsize-a.z[k])*u_tau_top) / (A_vandriest*a.visc));
-                    const TF fac = fmin(fac_bot, fac_top);
-                    ev = sq(fac * a.mlen0[k]) * sqrt(s2);
-                }
-                a.evisc[ijk] = ev;
-                if (limits)
-                {
-                    dm = fmax(dm, static_cast<double>(fabs(ev*tPrfac_i*(dxidxi + dyidyi + a.dzi[k]*a.dzi[k]))));
-                    // the 2i5 CFL rate as in cfl_g (the horizontal stencils from L2)
-                    const TF* __restrict__ u = a.u; const TF* __restrict__ v = a.v; const TF* __restrict__ w = a.w;
-                    const int ii = 1, jj = g.icells;
-                    TF wpart;
-                    if (k == g.kstart || k == g.kend-1)
-                        wpart = fabs(i2(w[ijk], w[ijk+kk]))*a.dzi[k];
-                    else if (k == g.kstart+1 || k == g.kend-2)
-                        wpart = fabs(i4ws(w[ijk-kk], w[ijk], w[ijk+kk], w[ijk+2*kk]))*a.dzi[k];
-                    else
-                        wpart = fabs(i6ws(w[ijk-2*kk], w[ijk-kk], w[ijk], w[ijk+kk], w[ijk+2*kk], w[ijk+3*kk]))*a.dzi[k];
-                    const TF val = fabs(i6ws(u[ijk-2*ii], u[ijk-ii], u[ijk], u[ijk+ii], u[ijk+2*ii], u[ijk+3*ii]))*a.dxi
-                                 + fabs(i6ws(v[ijk-2*jj], v[ijk-jj], v[ijk], v[ijk+jj], v[ijk+2*jj], v[ijk+3*jj]))*a.dyi
-                                 + wpart;
-                    cm = fmax(cm, static_cast<double>(val));
-                }
+                    wpart = fabs(i6ws(cw[0], cw[1], cw[2], cw[3], cw[4], cw[5]))*L.dzi;
+                const TF val = fabs(i6ws(cu[0], cu[1], cu[2], cu[3], cu[4], cu[5]))*a.dxi
+                             + fabs(i6ws(cv[0], cv[1], cv[2], cv[3], cv[4], cv[5]))*a.dyi
+                             + wpart;
+                cm = fmax(cm, static_cast<double>(val));
             }
             __syncthreads();
             TF* x;
             x = Um; Um = Uc; Uc = Up; Up = x;
             x = Vm; Vm = Vc; Vc = Vp; Vp = x;
             x = Wc; Wc = Wp; Wp = x;
-            if (more)
-            {
-                vplane_store(Up, ru, tid);
-                vplane_store(Vp, rv, tid);
-                vplane_store(Wp, rw, tid);
-            }
+            vplane_store(Up, ru, tid);
+            vplane_store(Vp, rv, tid);
+            vplane_store(Wp, rw, tid);
             t0 = t1; t1 = t2; t2 = tn;
         }
         if (limits)
@@ -696,14 +737,22 @@ namespace
                 MHH_CHECK(ensure_reduction(c, 2*nblocks + 1));
                 rc = c->red; rd = c->red + nblocks + 1;
             }
-            const TF tPrfac_i = TF(1)/std::min(TF(1.), TF(par.tPr));
-            const TF dxidxi = TF(1.)/(TF(g.dx)*TF(g.dx)), dyidyi = TF(1.)/(TF(g.dy)*TF(g.dy));
+            if (!p.lev)
+                return fail(MHH_ESTATE, "the viscosity needs the grid profiles");
+            Visc_scalars<TF> vs;
+            vs.dxi = a.dxi; vs.dyi = a.dyi; vs.tPr_i = TF(1)/TF(par.tPr); vs.visc = a.visc; vs.zsize = a.zsize;
+            vs.tPrfac_i = TF(1)/std::min(TF(1.), TF(par.tPr));
+            vs.dxidxi = TF(1.)/(TF(g.dx)*TF(g.dx)); vs.dyidyi = TF(1.)/(TF(g.dy)*TF(g.dy));
             const dim3 vb(VTX, VTY);
+            const TF* ml = mlen_ptr<TF>(c);
             mhh::Timer _t(c, "evisc");
-            if (th && lim)       visc_tile_g<TF, true,  true ><<<nblocks, vb, 0, c->stream>>>(a, g, nti, ntj, nkc, rc, rd, tPrfac_i, dxidxi, dyidyi);
-            else if (th)         visc_tile_g<TF, true,  false><<<nblocks, vb, 0, c->stream>>>(a, g, nti, ntj, nkc, rc, rd, tPrfac_i, dxidxi, dyidyi);
-            else if (lim)        visc_tile_g<TF, false, true ><<<nblocks, vb, 0, c->stream>>>(a, g, nti, ntj, nkc, rc, rd, tPrfac_i, dxidxi, dyidyi);
-            else                 visc_tile_g<TF, false, false><<<nblocks, vb, 0, c->stream>>>(a, g, nti, ntj, nkc, rc, rd, tPrfac_i, dxidxi, dyidyi);
+#define MHH_VISC_LAUNCH(T, L) visc_tile_g<TF, T, L><<<nblocks, vb, 0, c->stream>>>( \
+                a.evisc, a.u, a.v, a.w, a.th, p.lev, ml, p.z, vs, g, nti, ntj, nkc, rc, rd)
+            if (th && lim)       MHH_VISC_LAUNCH(true, true);
+            else if (th)         MHH_VISC_LAUNCH(true, false);
+            else if (lim)        MHH_VISC_LAUNCH(false, true);
+            else                 MHH_VISC_LAUNCH(false, false);
+#undef MHH_VISC_LAUNCH
             if (lim) *nred = nblocks;
         }
         else if (th && ms)     { mhh::Timer _t(c, "evisc"); evisc_g<TF, true,  true,  true ><<<grd, blk, 0, c->stream>>>(a, g); }

@@ -34,6 +34,7 @@ namespace mhh
         TF wfac;                  // dzhi/rhorefh: the same for w
         TF gthi;                  // grav/threfh (0 without a base state)
         TF threfh_rel;            // threfh - offset of scalar 0
+        TF n2fac;                 // grav/thref * 0.5 * dzi: N2 per theta difference over k-1..k+1
     };
 
     // Device copies of the 1-D grid and base-state profiles in the element type.
@@ -88,6 +89,9 @@ struct mhh_ctx
     bool bound = false;
     // Levels per block of the fused tendency kernel (0: the built-in default).
     int tend_kchunk = 0;
+    // Spectral transforms through the general LDS engine only (no kernels
+    // specialised for n = 128, 256, 512): a check of the specialised ones.
+    bool fft_generic = false;
     // Scalars advected with the Koren flux limiter ([advec] fluxlimit_list).
     int fluxlimit[MHH_MAX_SCALARS] = {};
     // Reduction scratch: per-block partial maxima and one result slot.

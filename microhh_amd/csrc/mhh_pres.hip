@@ -242,6 +242,114 @@ namespace
         }
     }
 
+    // LDS ordering between the lanes of one wave: LDS operations of a wave
+    // execute in issue order, so a wave exchanging data through its own LDS
+    // rows needs no block barrier, only a point the compiler may not move
+    // memory operations across.
+    __device__ __forceinline__ void wave_lds_sync()
+    {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+
+    // Number of radix-R passes of a row of n = 64 R (R = 2, 4, 8: n = 128,
+    // 256, 512) and the padded row length in LDS.
+    template<int R> constexpr int wave_passes() { return R == 8 ? 3 : R == 4 ? 4 : 7; }
+    template<int R> constexpr int wave_row_elems() { return ((64*R - 1) + ((64*R - 1) >> 3) + 1) | 1; }
+
+    // y transform of contiguous rows of length n = 64 R.  With a uniform radix
+    // R every Stockham pass has exactly 64 butterflies, one per lane, so one
+    // wave transforms a row on its own: the first pass works on the row as
+    // loaded (lane + 64 q), the last pass leaves the row in the same order for
+    // the store, and only the passes between go through the wave's LDS row.
+    // Twiddles of passes 1 .. P-1 depend on the lane only and stay in
+    // registers.  Waves walk the rows with a stride and load the next row
+    // while transforming the current one.  S = -1 forward, +1 inverse (which
+    // divides by n, as src/fft.cxx:420 does).
+    template<typename TF, int R, int S>
+    __global__ void __launch_bounds__(256) fft_y_wave_g(cplx<TF>* __restrict__ C, const double2* __restrict__ tw,
+                                                        int nrows)
+    {
+        using T = cplx<TF>;
+        constexpr int n = 64*R, P = wave_passes<R>();
+        __shared__ T rows[4][wave_row_elems<R>()];
+        const int lane = threadIdx.x, wv = threadIdx.y;
+        T* A = rows[wv];
+        const int nw = gridDim.x*4;
+        int r = blockIdx.x*4 + wv;
+        if (r >= nrows) return;    // per wave; the kernel has no block barrier
+
+        T wt[P-1][R-1];
+        {
+            int Ns = R;
+            #pragma unroll
+            for (int p = 1; p < P; ++p)
+            {
+                const int k = lane & (Ns - 1), ts = n/(Ns*R);
+                #pragma unroll
+                for (int q = 1; q < R; ++q)
+                {
+                    const double2 x = tw[q*k*ts];
+                    wt[p-1][q-1].x = TF(x.x); wt[p-1][q-1].y = TF(S < 0 ? x.y : -x.y);
+                }
+                Ns *= R;
+            }
+        }
+
+        T cur[R];
+        #pragma unroll
+        for (int q = 0; q < R; ++q) cur[q] = C[static_cast<size_t>(r)*n + lane + 64*q];
+        for (; r < nrows; r += nw)
+        {
+            T nxt[R];
+            const size_t rn = static_cast<size_t>(min(r + nw, nrows - 1));
+            #pragma unroll
+            for (int q = 0; q < R; ++q) nxt[q] = C[rn*n + lane + 64*q];
+
+            // pass 0 (Ns = 1, no twiddles): butterfly lane, outputs lane*R + q
+            dft<TF, R>(cur, S);
+            #pragma unroll
+            for (int q = 0; q < R; ++q) A[lpad(lane*R + q)] = cur[q];
+            wave_lds_sync();
+            int Ns = R;
+            #pragma unroll
+            for (int p = 1; p < P-1; ++p)
+            {
+                T v[R];
+                #pragma unroll
+                for (int q = 0; q < R; ++q) v[q] = A[lpad(lane + 64*q)];
+                #pragma unroll
+                for (int q = 1; q < R; ++q) v[q] = cmul(v[q], wt[p-1][q-1]);
+                dft<TF, R>(v, S);
+                wave_lds_sync();
+                const int d = (lane/Ns)*Ns*R + (lane & (Ns - 1));
+                #pragma unroll
+                for (int q = 0; q < R; ++q) A[lpad(d + q*Ns)] = v[q];
+                wave_lds_sync();
+                Ns *= R;
+            }
+            // last pass (Ns = 64): outputs lane + 64 q
+            T v[R];
+            #pragma unroll
+            for (int q = 0; q < R; ++q) v[q] = A[lpad(lane + 64*q)];
+            wave_lds_sync();
+            #pragma unroll
+            for (int q = 1; q < R; ++q) v[q] = cmul(v[q], wt[P-2][q-1]);
+            dft<TF, R>(v, S);
+            T* __restrict__ Cr = C + static_cast<size_t>(r)*n;
+            #pragma unroll
+            for (int q = 0; q < R; ++q)
+            {
+                T z = v[q];
+                if (S > 0) { z.x *= TF(1)/TF(n); z.y *= TF(1)/TF(n); }
+                Cr[lane + 64*q] = z;
+            }
+            #pragma unroll
+            for (int q = 0; q < R; ++q) cur[q] = nxt[q];
+        }
+    }
+
     // Tridiagonal solve per (kx, ky) mode (Pres_2::solve + tdma,
     // src/pres_2.cxx:266-324, 200-263), one thread per mode marching in k.
     // TF is the element type of the spectrum, TC the arithmetic type of the
@@ -656,6 +764,37 @@ namespace
         t = Pres_tables<TF>();
     }
 
+    // y transform of nrows contiguous rows: the wave-per-row kernel for n = 128,
+    // 256 and 512, the general engine otherwise.
+    template<typename TF, int R>
+    void launch_fft_y_wave(mhh_ctx* c, cplx<TF>* C, const double* tw, int sign, int nrows)
+    {
+        // enough waves to fill the chip a few times over; each walks its rows
+        const int nblocks = std::max(1, std::min((nrows + 3)/4, 4*256));
+        const double2* t2 = reinterpret_cast<const double2*>(tw);
+        if (sign < 0) fft_y_wave_g<TF, R, -1><<<nblocks, dim3(64, 4), 0, c->stream>>>(C, t2, nrows);
+        else          fft_y_wave_g<TF, R, +1><<<nblocks, dim3(64, 4), 0, c->stream>>>(C, t2, nrows);
+    }
+
+    template<typename TF>
+    int fft_y(mhh_ctx* c, cplx<TF>* C, const Plan& py, const double* tw, int sign, int nrows, const char* name)
+    {
+        {
+            mhh::Timer _t(c, name);
+            if (!c->fft_generic && py.n == 512)      launch_fft_y_wave<TF, 8>(c, C, tw, sign, nrows);
+            else if (!c->fft_generic && py.n == 256) launch_fft_y_wave<TF, 4>(c, C, tw, sign, nrows);
+            else if (!c->fft_generic && py.n == 128) launch_fft_y_wave<TF, 2>(c, C, tw, sign, nrows);
+            else
+            {
+                const int RBy = rows_per_block(py, sizeof(cplx<TF>));
+                const size_t ldsy = lds_bytes(py, RBy, sizeof(cplx<TF>));
+                fft_y_g<TF><<<(nrows + RBy - 1)/RBy, dim3(64, RBy), ldsy, c->stream>>>(C, py,
+                        reinterpret_cast<const double2*>(tw), sign, nrows, RBy);
+            }
+        }
+        return check_launch(name);
+    }
+
     // rk3 >= 0: apply the gradient together with the RK3 update of substep
     // rk3 (time step rkdt) instead of Pres_2::output alone.
     template<typename TF>
@@ -710,11 +849,7 @@ namespace
         }
         // 3. forward y transform
         const int nrows = nx*K;
-        const int RBy = rows_per_block(py, sizeof(T));
-        const size_t ldsy = lds_bytes(py, RBy, sizeof(T));
-        { mhh::Timer _t(c, "fft_y_fwd"); fft_y_g<TF><<<(nrows + RBy - 1)/RBy, dim3(64, RBy), ldsy, c->stream>>>(C, py,
-                reinterpret_cast<const double2*>(t.twy), -1, nrows, RBy); }
-        MHH_CHECK(check_launch("fft_y_fwd"));
+        MHH_CHECK(fft_y<TF>(c, C, py, t.twy, -1, nrows, "fft_y_fwd"));
         // 4. tridiagonal per mode
         {
             const int nmodes = nx*M;
@@ -734,9 +869,7 @@ namespace
             MHH_CHECK(check_launch("tdma"));
         }
         // 5. inverse y transform (divides by jtot)
-        { mhh::Timer _t(c, "fft_y_bwd"); fft_y_g<TF><<<(nrows + RBy - 1)/RBy, dim3(64, RBy), ldsy, c->stream>>>(C, py,
-                reinterpret_cast<const double2*>(t.twy), +1, nrows, RBy); }
-        MHH_CHECK(check_launch("fft_y_bwd"));
+        MHH_CHECK(fft_y<TF>(c, C, py, t.twy, +1, nrows, "fft_y_bwd"));
         // 6. inverse x transform (divides by itot) into the ghosted p with halos
         {
             const int RB = rows_per_block(px, sizeof(T));

@@ -1,0 +1,93 @@
+"""Pressure projection at transform lengths that have specialised kernels
+(128, 256 and 512 points along x and y) against the oracle, and the
+specialised kernels against the general LDS engine (MHH_FFT_GENERIC=1) on the
+same inputs.  FP64; tolerance 1e-12 normwise, as for every pressure test.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests._golden import interior, normwise
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64).copy()
+
+
+def _inputs(n, seed):
+    from tests.golden.make_golden import make_inputs
+    return make_inputs(dict(itot=n, jtot=n, ktot=6, stretch=True, anelastic=True, dx=50.0, dy=40.0, seed=seed))
+
+
+def _gpu_pres(p, i, dt, generic):
+    from tests._gpu import GpuCase
+    old = os.environ.get("MHH_FFT_GENERIC")
+    os.environ["MHH_FFT_GENERIC"] = "1" if generic else "0"
+    try:
+        g = GpuCase(p["itot"], p["jtot"], p["ktot"], p["igc"], p["jgc"], p["kgc"], p["dx"], p["dy"], i,
+                    nscalars=1)
+    finally:
+        if old is None:
+            del os.environ["MHH_FFT_GENERIC"]
+        else:
+            os.environ["MHH_FFT_GENERIC"] = old
+    try:
+        for n in ("u", "v", "w", "ut", "vt", "wt"):
+            g.put(n, i[n])
+        for n in ("u", "v", "w"):
+            g.call("mhh_boundary_cyclic", g.ptr(n), 2)
+        u, v, w = g.get("u"), g.get("v"), g.get("w")
+        g.call("mhh_pres_2_exec", dt)
+        return {n: g.get(n) for n in ("ut", "vt", "wt", "p")}, (u, v, w)
+    finally:
+        g.close()
+
+
+def _oracle_pres(p, i, uvw, dt):
+    d = po.dims(p["itot"], p["jtot"], p["ktot"], p["igc"], p["jgc"], p["kgc"],
+                p["dx"] * p["itot"], p["dy"] * p["jtot"], float(i["zh"][p["kgc"] + p["ktot"]]))
+    L = po.lib()
+    u, v, w = (_c(a) for a in uvw)
+    ut, vt, wt = _c(i["ut"]), _c(i["vt"]), _c(i["wt"])
+    pf, tmp1, tmp2 = np.zeros_like(ut), np.zeros_like(ut), np.zeros_like(ut)
+    bmati, bmatj = np.zeros(d.itot), np.zeros(d.jtot)
+    a, c = np.zeros(d.kmax), np.zeros(d.kmax)
+    L.mo_pres_2_set_values(d, po.ptr(bmati), po.ptr(bmatj), po.ptr(a), po.ptr(c), po.ptr(_c(i["dz"])),
+                           po.ptr(_c(i["dzhi"])), po.ptr(_c(i["rhorefh"])))
+    L.mo_pres_2_input(d, po.ptr(pf), po.ptr(u), po.ptr(v), po.ptr(w), po.ptr(ut), po.ptr(vt), po.ptr(wt),
+                      po.ptr(_c(i["dzi"])), po.ptr(_c(i["rhoref"])), po.ptr(_c(i["rhorefh"])), dt)
+    L.mo_pres_2_solve(d, po.ptr(pf), po.ptr(tmp1), po.ptr(tmp2), po.ptr(_c(i["dz"])), po.ptr(_c(i["rhoref"])),
+                      po.ptr(bmati), po.ptr(bmatj), po.ptr(a), po.ptr(c))
+    L.mo_pres_2_output(d, po.ptr(ut), po.ptr(vt), po.ptr(wt), po.ptr(pf), po.ptr(_c(i["dzhi"])))
+    return d, {"ut": ut, "vt": vt, "wt": wt, "p": pf}
+
+
+@pytest.mark.parametrize("n", [128, 256, 512])
+def test_pres_specialised_lengths(n):
+    p, i = _inputs(n, seed=31 + n)
+    dt = 0.41
+    out, uvw = _gpu_pres(p, i, dt, generic=False)
+    d, ref = _oracle_pres(p, i, uvw, dt)
+    m = interior(d)
+    for name in ("ut", "vt", "wt"):
+        err = normwise(out[name], ref[name], m)
+        assert err < TOL, f"n={n} {name}: {err:.3e}"
+    ks, ke = d.kstart, d.kend
+    err = normwise(out["p"][ks - 1:ke], ref["p"][ks - 1:ke])
+    assert err < TOL, f"n={n} p: {err:.3e}"
+
+
+@pytest.mark.parametrize("n", [128, 512])
+def test_specialised_matches_general_engine(n):
+    p, i = _inputs(n, seed=7 + n)
+    dt = 0.29
+    fast, _ = _gpu_pres(p, i, dt, generic=False)
+    gen, _ = _gpu_pres(p, i, dt, generic=True)
+    for name in ("ut", "vt", "wt", "p"):
+        err = normwise(fast[name], gen[name])
+        assert err < 1e-13, f"n={n} {name}: {err:.3e}"
