@@ -258,15 +258,73 @@ namespace
     template<int R> constexpr int wave_passes() { return R == 8 ? 3 : R == 4 ? 4 : 7; }
     template<int R> constexpr int wave_row_elems() { return ((64*R - 1) + ((64*R - 1) >> 3) + 1) | 1; }
 
-    // y transform of contiguous rows of length n = 64 R.  With a uniform radix
-    // R every Stockham pass has exactly 64 butterflies, one per lane, so one
-    // wave transforms a row on its own: the first pass works on the row as
-    // loaded (lane + 64 q), the last pass leaves the row in the same order for
-    // the store, and only the passes between go through the wave's LDS row.
-    // Twiddles of passes 1 .. P-1 depend on the lane only and stay in
-    // registers.  Waves walk the rows with a stride and load the next row
-    // while transforming the current one.  S = -1 forward, +1 inverse (which
-    // divides by n, as src/fft.cxx:420 does).
+    // Per-lane twiddles of passes 1 .. P-1 of the wave FFT of length n = 64 R:
+    // pass p (Ns = R^p) multiplies input q of butterfly `lane` by
+    // tw[q (lane mod Ns) n/(Ns R)], conjugated for the inverse (S = +1).
+    template<typename TF, int R, int S>
+    __device__ __forceinline__ void wave_twiddles(cplx<TF> (&wt)[wave_passes<R>()-1][R-1],
+                                                  const double2* __restrict__ tw, int lane)
+    {
+        constexpr int n = 64*R, P = wave_passes<R>();
+        int Ns = R;
+        #pragma unroll
+        for (int p = 1; p < P; ++p)
+        {
+            const int k = lane & (Ns - 1), ts = n/(Ns*R);
+            #pragma unroll
+            for (int q = 1; q < R; ++q)
+            {
+                const double2 x = tw[q*k*ts];
+                wt[p-1][q-1].x = TF(x.x); wt[p-1][q-1].y = TF(S < 0 ? x.y : -x.y);
+            }
+            Ns *= R;
+        }
+    }
+
+    // Unnormalised FFT of one row of n = 64 R by one wave.  With a uniform
+    // radix R every Stockham pass has exactly 64 butterflies, one per lane:
+    // the first pass works on the row as the lanes hold it (v[q] = x[lane +
+    // 64 q]), the last pass leaves the transform in the same order, and only
+    // the passes between go through the wave's LDS row A (padded indices).
+    template<typename TF, int R, int S>
+    __device__ __forceinline__ void wave_fft(cplx<TF> (&v)[R], cplx<TF>* A,
+                                             const cplx<TF> (&wt)[wave_passes<R>()-1][R-1], int lane)
+    {
+        constexpr int P = wave_passes<R>();
+        // pass 0 (Ns = 1, no twiddles): butterfly lane, outputs lane*R + q
+        dft<TF, R>(v, S);
+        #pragma unroll
+        for (int q = 0; q < R; ++q) A[lpad(lane*R + q)] = v[q];
+        wave_lds_sync();
+        int Ns = R;
+        #pragma unroll
+        for (int p = 1; p < P-1; ++p)
+        {
+            #pragma unroll
+            for (int q = 0; q < R; ++q) v[q] = A[lpad(lane + 64*q)];
+            #pragma unroll
+            for (int q = 1; q < R; ++q) v[q] = cmul(v[q], wt[p-1][q-1]);
+            dft<TF, R>(v, S);
+            wave_lds_sync();
+            const int d = (lane/Ns)*Ns*R + (lane & (Ns - 1));
+            #pragma unroll
+            for (int q = 0; q < R; ++q) A[lpad(d + q*Ns)] = v[q];
+            wave_lds_sync();
+            Ns *= R;
+        }
+        // last pass (Ns = 64): outputs lane + 64 q
+        #pragma unroll
+        for (int q = 0; q < R; ++q) v[q] = A[lpad(lane + 64*q)];
+        wave_lds_sync();
+        #pragma unroll
+        for (int q = 1; q < R; ++q) v[q] = cmul(v[q], wt[P-2][q-1]);
+        dft<TF, R>(v, S);
+    }
+
+    // y transform of contiguous rows of length n = 64 R, one wave per row
+    // (no block barriers).  Waves walk the rows with a stride and load the next
+    // row while transforming the current one.  S = -1 forward, +1 inverse
+    // (which divides by n, as src/fft.cxx:420 does).
     template<typename TF, int R, int S>
     __global__ void __launch_bounds__(256) fft_y_wave_g(cplx<TF>* __restrict__ C, const double2* __restrict__ tw,
                                                         int nrows)
@@ -281,22 +339,7 @@ namespace
         if (r >= nrows) return;    // per wave; the kernel has no block barrier
 
         T wt[P-1][R-1];
-        {
-            int Ns = R;
-            #pragma unroll
-            for (int p = 1; p < P; ++p)
-            {
-                const int k = lane & (Ns - 1), ts = n/(Ns*R);
-                #pragma unroll
-                for (int q = 1; q < R; ++q)
-                {
-                    const double2 x = tw[q*k*ts];
-                    wt[p-1][q-1].x = TF(x.x); wt[p-1][q-1].y = TF(S < 0 ? x.y : -x.y);
-                }
-                Ns *= R;
-            }
-        }
-
+        wave_twiddles<TF, R, S>(wt, tw, lane);
         T cur[R];
         #pragma unroll
         for (int q = 0; q < R; ++q) cur[q] = C[static_cast<size_t>(r)*n + lane + 64*q];
@@ -306,47 +349,223 @@ namespace
             const size_t rn = static_cast<size_t>(min(r + nw, nrows - 1));
             #pragma unroll
             for (int q = 0; q < R; ++q) nxt[q] = C[rn*n + lane + 64*q];
-
-            // pass 0 (Ns = 1, no twiddles): butterfly lane, outputs lane*R + q
-            dft<TF, R>(cur, S);
-            #pragma unroll
-            for (int q = 0; q < R; ++q) A[lpad(lane*R + q)] = cur[q];
-            wave_lds_sync();
-            int Ns = R;
-            #pragma unroll
-            for (int p = 1; p < P-1; ++p)
-            {
-                T v[R];
-                #pragma unroll
-                for (int q = 0; q < R; ++q) v[q] = A[lpad(lane + 64*q)];
-                #pragma unroll
-                for (int q = 1; q < R; ++q) v[q] = cmul(v[q], wt[p-1][q-1]);
-                dft<TF, R>(v, S);
-                wave_lds_sync();
-                const int d = (lane/Ns)*Ns*R + (lane & (Ns - 1));
-                #pragma unroll
-                for (int q = 0; q < R; ++q) A[lpad(d + q*Ns)] = v[q];
-                wave_lds_sync();
-                Ns *= R;
-            }
-            // last pass (Ns = 64): outputs lane + 64 q
-            T v[R];
-            #pragma unroll
-            for (int q = 0; q < R; ++q) v[q] = A[lpad(lane + 64*q)];
-            wave_lds_sync();
-            #pragma unroll
-            for (int q = 1; q < R; ++q) v[q] = cmul(v[q], wt[P-2][q-1]);
-            dft<TF, R>(v, S);
+            wave_fft<TF, R, S>(cur, A, wt, lane);
             T* __restrict__ Cr = C + static_cast<size_t>(r)*n;
             #pragma unroll
             for (int q = 0; q < R; ++q)
             {
-                T z = v[q];
+                T z = cur[q];
                 if (S > 0) { z.x *= TF(1)/TF(n); z.y *= TF(1)/TF(n); }
                 Cr[lane + 64*q] = z;
             }
             #pragma unroll
             for (int q = 0; q < R; ++q) cur[q] = nxt[q];
+        }
+    }
+
+    // Forward x transform of n = 64 R points: block (blockIdx.x, k) covers the
+    // 8 real rows j0 .. j0+7 of level k (jtot a multiple of 8), one wave per
+    // pair of rows packed as a + i b.  MODE 0: P as stored; 1: P minus the
+    // plane mean pmean[k] (mixed mode); 2: the right-hand side of pres_in_g
+    // evaluated from u, v, w and their tendencies.  Writes C[k][kx][j].
+    template<typename TF, int R, int MODE>
+    __global__ void __launch_bounds__(256) fft_x_fwd_wave_g(
+            const TF* __restrict__ P, cplx<TF>* __restrict__ C, const double2* __restrict__ tw, Gd g,
+            const double* __restrict__ pmean,
+            const TF* __restrict__ u, const TF* __restrict__ v, const TF* __restrict__ w,
+            const TF* __restrict__ ut, const TF* __restrict__ vt, const TF* __restrict__ wt,
+            const TF* __restrict__ dzi, const TF* __restrict__ rho, const TF* __restrict__ rhoh,
+            TF dxi, TF dyi, TF dti)
+    {
+        using T = cplx<TF>;
+        constexpr int n = 64*R, nx = n/2 + 1, JB = 8, P_ = wave_passes<R>();
+        constexpr int NQ = (nx + 63)/64;
+        __shared__ T rows[4][wave_row_elems<R>()];
+        const int lane = threadIdx.x, wv = threadIdx.y, tid = lane + 64*wv;
+        const int M = g.jtot, k = blockIdx.y;
+        const int j0 = blockIdx.x*JB, ja = j0 + 2*wv, jb = ja + 1;
+        T* A = rows[wv];
+
+        T z[R];
+        if (MODE == 2)
+        {
+            // pres_in_g's expression; the east and north tendency neighbours
+            // wrap periodically, u, v, w have periodic halos already
+            const int jj = g.icells, kk = g.ijcells, kc = k + g.kgc;
+            const TF rk = rho[kc], rhk = rhoh[kc], rhkp = rhoh[kc+1], dzk = dzi[kc];
+            #pragma unroll
+            for (int h = 0; h < 2; ++h)
+            {
+                const int j = h ? jb : ja;
+                const int row = g.igc + (j + g.jgc)*jj + kc*kk;
+                const int dn = (j + 1 == g.jtot) ? jj - g.jtot*jj : jj;
+                #pragma unroll
+                for (int q = 0; q < R; ++q)
+                {
+                    const int x = lane + 64*q;
+                    const int ijk = row + x;
+                    const int de = (x + 1 == n) ? 1 - n : 1;
+                    const TF val = rk * ( (ut[ijk+de] + u[ijk+1] * dti) - (ut[ijk] + u[ijk] * dti) ) * dxi
+                                 + rk * ( (vt[ijk+dn] + v[ijk+jj] * dti) - (vt[ijk] + v[ijk] * dti) ) * dyi
+                                 + ( rhkp * (wt[ijk+kk] + w[ijk+kk] * dti)
+                                   - rhk  * (wt[ijk   ] + w[ijk   ] * dti) ) * dzk;
+                    if (h) z[q].y = val; else z[q].x = val;
+                }
+            }
+        }
+        else
+        {
+            const TF* Pk = P + static_cast<size_t>(k)*n*M;
+            const double m = MODE == 1 ? pmean[k] : 0.;
+            #pragma unroll
+            for (int q = 0; q < R; ++q)
+            {
+                const int x = lane + 64*q;
+                if (MODE == 1) { z[q].x = TF(double(Pk[x + ja*n]) - m); z[q].y = TF(double(Pk[x + jb*n]) - m); }
+                else           { z[q].x = Pk[x + ja*n]; z[q].y = Pk[x + jb*n]; }
+            }
+        }
+        T wtw[P_-1][R-1];
+        wave_twiddles<TF, R, -1>(wtw, tw, lane);
+        wave_fft<TF, R, -1>(z, A, wtw, lane);
+
+        // the two half-spectra: Z[kx] and Z[n-kx] come from other lanes
+        #pragma unroll
+        for (int q = 0; q < R; ++q) A[lpad(lane + 64*q)] = z[q];
+        wave_lds_sync();
+        T xs[NQ][2];
+        #pragma unroll
+        for (int q = 0; q < NQ; ++q)
+        {
+            const int kx = lane + 64*q;
+            if (kx < nx) unpack_pair<TF>(A[lpad(kx)], A[lpad((n - kx) % n)], xs[q][0], xs[q][1]);
+        }
+        wave_lds_sync();
+        #pragma unroll
+        for (int q = 0; q < NQ; ++q)
+        {
+            const int kx = lane + 64*q;
+            if (kx < nx) stage_store<TF>(A, kx, n, xs[q][0], xs[q][1]);
+        }
+        __syncthreads();
+        // C[k][kx][j0 .. j0+8): consecutive threads on consecutive j
+        T* Ck = C + static_cast<size_t>(k)*nx*M;
+        for (int idx = tid; idx < nx*JB; idx += 256)
+        {
+            const int kx = idx / JB, jl = idx % JB;
+            const int r = jl >> 1, part = jl & 1;
+            const T* Sr = rows[r];
+            T val;
+            if (kx == 0 || 2*kx == n)
+            {
+                const T d = Sr[kx == 0 ? 0 : 1];
+                val.x = part ? d.y : d.x; val.y = TF(0);
+            }
+            else
+                val = Sr[lpad(stage_base(kx, n) + part)];
+            Ck[static_cast<size_t>(kx)*M + j0 + jl] = val;
+        }
+    }
+
+    // Inverse x transform of n = 64 R points (divides by n, src/fft.cxx:446)
+    // into the ghosted p: interior rows, their periodic images in the y halo,
+    // the x halo columns and, at the lowest level, the ghost level below.
+    template<typename TF, int R>
+    __global__ void __launch_bounds__(256) fft_x_bwd_wave_g(const cplx<TF>* __restrict__ C, TF* __restrict__ p,
+                                                            const double2* __restrict__ tw, Gd g)
+    {
+        using T = cplx<TF>;
+        constexpr int n = 64*R, nx = n/2 + 1, JB = 8, P_ = wave_passes<R>();
+        __shared__ T rows[4][wave_row_elems<R>()];
+        const int lane = threadIdx.x, wv = threadIdx.y, tid = lane + 64*wv;
+        const int M = g.jtot, k = blockIdx.y;
+        const int j0 = blockIdx.x*JB;
+        T* A = rows[wv];
+        T wtw[P_-1][R-1];
+        wave_twiddles<TF, R, +1>(wtw, tw, lane);
+
+        // stage the block's spectra packed as in the forward kernel
+        const T* Ck = C + static_cast<size_t>(k)*nx*M;
+        for (int idx = tid; idx < nx*JB; idx += 256)
+        {
+            const int kx = idx / JB, jl = idx % JB;
+            const int r = jl >> 1, part = jl & 1;
+            const T val = Ck[static_cast<size_t>(kx)*M + j0 + jl];
+            T* Sr = rows[r];
+            if (kx == 0 || 2*kx == n)
+            {
+                TF* d = reinterpret_cast<TF*>(&Sr[kx == 0 ? 0 : 1]);
+                d[part] = val.x;
+            }
+            else
+                Sr[lpad(stage_base(kx, n) + part)] = val;
+        }
+        __syncthreads();
+        // Hermitian extension of both spectra and packing Z = Xa + i Xb
+        T z[R];
+        #pragma unroll
+        for (int q = 0; q < R; ++q)
+        {
+            const int x = lane + 64*q;
+            const int kx = (x <= n/2) ? x : n - x;
+            T xa, xb;
+            if (kx == 0 || 2*kx == n)
+            {
+                const T d = A[kx == 0 ? 0 : 1];
+                xa.x = d.x; xa.y = TF(0); xb.x = d.y; xb.y = TF(0);
+            }
+            else
+            {
+                const int b = stage_base(kx, n);
+                xa = A[lpad(b)]; xb = A[lpad(b+1)];
+                if (x > n/2) { xa.y = -xa.y; xb.y = -xb.y; }
+            }
+            z[q].x = xa.x - xb.y; z[q].y = xa.y + xb.x;
+        }
+        wave_lds_sync();
+        wave_fft<TF, R, +1>(z, A, wtw, lane);
+
+        const TF ni = TF(1)/TF(n);   // exact for powers of two
+        const int jj = g.icells, kk = g.ijcells, kc = k + g.kgc;
+        const int ja = j0 + 2*wv;
+        // x halo columns: ghosted x < igc and x >= iend wrap to n - igc + x and x - iend
+        #pragma unroll
+        for (int q = 0; q < R; ++q) A[lpad(lane + 64*q)] = z[q];
+        wave_lds_sync();
+        T zh; zh.x = TF(0); zh.y = TF(0);
+        int xh = -1;
+        if (lane < 2*g.igc)
+        {
+            xh = lane < g.igc ? lane : g.iend + (lane - g.igc);
+            const int xi = lane < g.igc ? n - g.igc + lane : lane - g.igc;
+            zh = A[lpad(xi)];
+        }
+        #pragma unroll
+        for (int h = 0; h < 2; ++h)
+        {
+            const int j = ja + h;
+            // the interior row and, within jgc of an edge, its periodic image
+            int dst[2]; int nd = 0;
+            dst[nd++] = j + g.jgc;
+            if (j >= M - g.jgc) dst[nd++] = j - (M - g.jgc);
+            else if (j < g.jgc) dst[nd++] = g.jend + j;
+            for (int e = 0; e < nd; ++e)
+            {
+                TF* __restrict__ pr = p + static_cast<size_t>(kc)*kk + static_cast<size_t>(dst[e])*jj;
+                #pragma unroll
+                for (int q = 0; q < R; ++q)
+                {
+                    const TF val = (h ? z[q].y : z[q].x)*ni;
+                    pr[g.igc + lane + 64*q] = val;
+                    if (k == 0) pr[g.igc + lane + 64*q - kk] = val;
+                }
+                if (xh >= 0)
+                {
+                    const TF val = (h ? zh.y : zh.x)*ni;
+                    pr[xh] = val;
+                    if (k == 0) pr[xh - kk] = val;
+                }
+            }
         }
     }
 
@@ -795,6 +1014,60 @@ namespace
         return check_launch(name);
     }
 
+    // Forward and inverse x transforms through the wave kernels (itot = 64 R,
+    // R = 2, 4, 8; jtot a multiple of 8).
+    template<typename TF, int R>
+    void launch_fft_x_fwd_wave(mhh_ctx* c, const TF* P, cplx<TF>* C, const double2* tw, int mode,
+                               const Prof<TF>& p, TF dxi, TF dyi, TF dti)
+    {
+        const Gd& g = c->gd;
+        const dim3 grd(g.jtot/8, g.ktot), blk(64, 4);
+        const TF* u = static_cast<const TF*>(c->f.u); const TF* v = static_cast<const TF*>(c->f.v);
+        const TF* w = static_cast<const TF*>(c->f.w); const TF* ut = static_cast<const TF*>(c->f.ut);
+        const TF* vt = static_cast<const TF*>(c->f.vt); const TF* wt = static_cast<const TF*>(c->f.wt);
+#define MHH_XF(MODE) fft_x_fwd_wave_g<TF, R, MODE><<<grd, blk, 0, c->stream>>>(P, C, tw, g, c->pm_mean, \
+                u, v, w, ut, vt, wt, p.dzi, p.rhoref, p.rhorefh, dxi, dyi, dti)
+        if (mode == 2) MHH_XF(2);
+        else if (mode == 1) MHH_XF(1);
+        else MHH_XF(0);
+#undef MHH_XF
+    }
+
+    template<typename TF>
+    int fft_x_fwd_wave(mhh_ctx* c, const TF* P, cplx<TF>* C, const double* tw, int mode,
+                       const Prof<TF>& p, TF dxi, TF dyi, TF dti, const char* name)
+    {
+        const double2* t2 = reinterpret_cast<const double2*>(tw);
+        {
+            mhh::Timer _t(c, name);
+            switch (c->gd.itot)
+            {
+                case 512: launch_fft_x_fwd_wave<TF, 8>(c, P, C, t2, mode, p, dxi, dyi, dti); break;
+                case 256: launch_fft_x_fwd_wave<TF, 4>(c, P, C, t2, mode, p, dxi, dyi, dti); break;
+                default:  launch_fft_x_fwd_wave<TF, 2>(c, P, C, t2, mode, p, dxi, dyi, dti); break;
+            }
+        }
+        return check_launch(name);
+    }
+
+    template<typename TF>
+    int fft_x_bwd_wave(mhh_ctx* c, const cplx<TF>* C, TF* P, const double* tw)
+    {
+        const Gd& g = c->gd;
+        const double2* t2 = reinterpret_cast<const double2*>(tw);
+        const dim3 grd(g.jtot/8, g.ktot), blk(64, 4);
+        {
+            mhh::Timer _t(c, "fft_x_bwd");
+            switch (g.itot)
+            {
+                case 512: fft_x_bwd_wave_g<TF, 8><<<grd, blk, 0, c->stream>>>(C, P, t2, g); break;
+                case 256: fft_x_bwd_wave_g<TF, 4><<<grd, blk, 0, c->stream>>>(C, P, t2, g); break;
+                default:  fft_x_bwd_wave_g<TF, 2><<<grd, blk, 0, c->stream>>>(C, P, t2, g); break;
+            }
+        }
+        return check_launch("fft_x_bwd");
+    }
+
     // rk3 >= 0: apply the gradient together with the RK3 update of substep
     // rk3 (time step rkdt) instead of Pres_2::output alone.
     template<typename TF>
@@ -839,6 +1112,13 @@ namespace
             }
         }
         // 2. forward x transform
+        const bool xwave = !c->fft_generic && (N == 128 || N == 256 || N == 512) && M % 8 == 0 && g.igc <= 32;
+        if (xwave)
+        {
+            const int mode = fuse_in ? 2 : (c->mixed ? 1 : 0);
+            MHH_CHECK(fft_x_fwd_wave<TF>(c, P, C, t.twx, mode, p, dxi, dyi, dti, fuse_in ? "pres_in_fft_x" : "fft_x_fwd"));
+        }
+        else
         {
             const int RB = rows_per_block(px, sizeof(T));
             const dim3 grd((M + 2*RB - 1)/(2*RB), K);
@@ -871,6 +1151,9 @@ namespace
         // 5. inverse y transform (divides by jtot)
         MHH_CHECK(fft_y<TF>(c, C, py, t.twy, +1, nrows, "fft_y_bwd"));
         // 6. inverse x transform (divides by itot) into the ghosted p with halos
+        if (xwave)
+            MHH_CHECK(fft_x_bwd_wave<TF>(c, C, P, t.twx));
+        else
         {
             const int RB = rows_per_block(px, sizeof(T));
             const dim3 grd((M + 2*RB - 1)/(2*RB), K);

@@ -24,13 +24,14 @@ def _inputs(n, seed):
     return make_inputs(dict(itot=n, jtot=n, ktot=6, stretch=True, anelastic=True, dx=50.0, dy=40.0, seed=seed))
 
 
-def _gpu_pres(p, i, dt, generic):
+def _gpu_pres(p, i, dt, generic, precision=None):
+    from microhh_amd import abi
     from tests._gpu import GpuCase
     old = os.environ.get("MHH_FFT_GENERIC")
     os.environ["MHH_FFT_GENERIC"] = "1" if generic else "0"
     try:
         g = GpuCase(p["itot"], p["jtot"], p["ktot"], p["igc"], p["jgc"], p["kgc"], p["dx"], p["dy"], i,
-                    nscalars=1)
+                    nscalars=1, precision=abi.MHH_FP64 if precision is None else precision)
     finally:
         if old is None:
             del os.environ["MHH_FFT_GENERIC"]
@@ -91,3 +92,17 @@ def test_specialised_matches_general_engine(n):
     for name in ("ut", "vt", "wt", "p"):
         err = normwise(fast[name], gen[name])
         assert err < 1e-13, f"n={n} {name}: {err:.3e}"
+
+
+def test_mixed_specialised_matches_general_engine():
+    """The mixed mode (FP32 fields and transforms, FP64 plane means and
+    tridiagonal) through both transform engines: the same projection to FP32
+    rounding."""
+    from microhh_amd import abi
+    p, i = _inputs(256, seed=99)
+    dt = 0.33
+    fast, _ = _gpu_pres(p, i, dt, generic=False, precision=abi.MHH_MIXED)
+    gen, _ = _gpu_pres(p, i, dt, generic=True, precision=abi.MHH_MIXED)
+    for name in ("ut", "vt", "wt"):
+        err = normwise(fast[name], gen[name])
+        assert err < 1e-5, f"{name}: {err:.3e}"
