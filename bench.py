@@ -52,7 +52,9 @@ PASSES = {
     "advec_2i5_u": 5, "advec_2i5_v": 5, "advec_2i5_w": 5, "advec_2i5_s": 6,
     "evisc": 5, "diff_u": 6, "diff_v": 6, "diff_w": 6, "diff_c": 4,
     "pres_in": 7, "pres_in_fft_x": 7, "fft_x_fwd": 2, "fft_y_fwd": 2, "tdma": 2, "fft_y_bwd": 2, "fft_x_bwd": 2,
-    "pres_out": 7, "pres_out_rk3": 13, "rk3": 4, "buoyancy": 3,
+    # gradient + RK3 of u, v, w and scalar 0: reads p, u, v, w, s and the four
+    # tendencies, writes the eight prognostic arrays
+    "pres_out": 7, "pres_out_rk3": 17, "rk3": 4, "buoyancy": 3,
 }
 KERNELS = list(PASSES) + ["cyclic", "ghost_cells", "mirror_walls", "rk3_zero_ghosts", "cfl", "dnmul"]
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE

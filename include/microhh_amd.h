@@ -170,12 +170,15 @@ int  mhh_thermo_dry_exec(mhh_ctx* ctx);
 /* Pres_2::exec (src/pres_2.cxx:64-94): input, FFT, tridiagonal, inverse FFT,
  * output.  dt is the RK sub-time step.  Uses p, tmp1 and tmp2 as work. */
 int  mhh_pres_2_exec(mhh_ctx* ctx, double dt);
-/* Pres_2::exec followed by Timeloop::exec for u, v, w (src/pres_2.cxx:64-94,
+/* Pres_2::exec followed by Timeloop::exec (src/pres_2.cxx:64-94,
  * src/timeloop.cxx:251-286): the pressure gradient and the RK3 update of the
  * momentum fields in one pass; dt is the sub time step of the solve, substep
- * and rkdt those of the RK3 update.  The scalars are integrated with
- * mhh_timeloop_rk3_scalars. */
-int  mhh_pres_2_exec_rk3(mhh_ctx* ctx, double dt, int substep, double rkdt);
+ * and rkdt those of the RK3 update.  scalars != 0: every bound scalar is
+ * integrated as well (scalar 0 in the same pass), so the call is the whole
+ * Pres_2::exec + Timeloop::exec; scalars == 0 leaves them to
+ * mhh_timeloop_rk3_scalars.  No single counterpart in the reference, which
+ * calls the two in sequence (src/model.cxx:436-504). */
+int  mhh_pres_2_exec_rk3(mhh_ctx* ctx, double dt, int substep, double rkdt, int scalars);
 /* MHH_MIXED only: the pressure field then holds p minus its horizontal mean,
  * and the mean (solved in FP64) is kept per level; out receives kcells values
  * (p = field + out[k]).  For FP64 and FP32 contexts out is zero.  No

@@ -82,7 +82,7 @@ SIGNATURES = {
     "mhh_tendency_2i5_smag2": (C.c_int, [VP, C.POINTER(SmagParams), C.c_int]),
     "mhh_timeloop_rk3": (C.c_int, [VP, C.c_int, C.c_double]),
     "mhh_timeloop_rk3_scalars": (C.c_int, [VP, C.c_int, C.c_double]),
-    "mhh_pres_2_exec_rk3": (C.c_int, [VP, C.c_double, C.c_int, C.c_double]),
+    "mhh_pres_2_exec_rk3": (C.c_int, [VP, C.c_double, C.c_int, C.c_double, C.c_int]),
     "mhh_device_alloc": (C.c_int, [C.c_size_t, C.POINTER(VP)]),
     "mhh_device_free": (C.c_int, [VP]),
     "mhh_copy_to_device": (C.c_int, [VP, VP, C.c_size_t]),

@@ -232,6 +232,18 @@ namespace mhh
     }
     template int zero_tendency_ghosts<double>(mhh_ctx*, double*);
     template int zero_tendency_ghosts<float>(mhh_ctx*, float*);
+
+    template<typename TF>
+    int rk3_field(mhh_ctx* c, TF* a, TF* at, TF cBdt, TF cAn, int zero)
+    {
+        const Gd& g = c->gd;
+        { mhh::Timer _t(c, "rk3"); rk3_g<TF><<<grid3(g, g.kstart, g.kend), dim3(BX, BY), 0, c->stream>>>(
+                a, at, cBdt, cAn, zero, g); }
+        MHH_CHECK(check_launch("rk3"));
+        return zero ? zero_tendency_ghosts<TF>(c, at) : 0;
+    }
+    template int rk3_field<double>(mhh_ctx*, double*, double*, double, double, int);
+    template int rk3_field<float>(mhh_ctx*, float*, float*, float, float, int);
 }
 
 namespace

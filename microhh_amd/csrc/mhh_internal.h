@@ -154,6 +154,9 @@ namespace mhh
 
     // Zero the ghost cells of a tendency field (mhh_boundary.hip).
     template<typename TF> int zero_tendency_ghosts(mhh_ctx* c, TF* at);
+    // RK3 update of one field and its tendency (a += cBdt*at; at = zero ? 0 :
+    // cAn*at), the tendency's ghost cells cleared when zero.
+    template<typename TF> int rk3_field(mhh_ctx* c, TF* a, TF* at, TF cBdt, TF cAn, int zero);
 
     // Partial sums per level of the mixed mode's plane mean.
     constexpr int PM_CHUNKS = 16;

@@ -842,12 +842,17 @@ namespace
 
     // Pres_2::output followed by the RK3 update of u, v, w (src/pres_2.cxx:364-387,
     // src/timeloop.cxx:251-286): the same operations in one pass.
-    template<typename TF>
+    // Pres_2::output (src/pres_2.cxx:364-387) followed by the RK3 update of
+    // u, v, w (src/timeloop.cxx:251-286) and, with NS = 1, of scalar s (whose
+    // update needs no pressure), one point per thread on the plane-ordered
+    // grid: a pure streaming pass.
+    template<typename TF, int NS>
     __global__ void __launch_bounds__(BX*BY) pres_out_rk3_g(
             TF* __restrict__ u, TF* __restrict__ v, TF* __restrict__ w,
             TF* __restrict__ ut, TF* __restrict__ vt, TF* __restrict__ wt,
-            const TF* __restrict__ p, const TF* __restrict__ dzhi, TF dxi, TF dyi,
-            TF cBdt, TF cAn, int zero, Gd g, const TF* __restrict__ dpbar)
+            TF* __restrict__ s, TF* __restrict__ st,
+            const TF* __restrict__ p, const TF* __restrict__ dzhi, const TF* __restrict__ dpbar,
+            TF dxi, TF dyi, TF cBdt, TF cAn, int zero, Gd g)
     {
         const int i = blockIdx.x*BX + threadIdx.x + g.istart;
         const int j = blockIdx.y*BY + threadIdx.y + g.jstart;
@@ -863,6 +868,11 @@ namespace
         u[ijk] += cBdt*tu; ut[ijk] = zero ? TF(0) : tu*cAn;
         v[ijk] += cBdt*tv; vt[ijk] = zero ? TF(0) : tv*cAn;
         w[ijk] += cBdt*tw; wt[ijk] = zero ? TF(0) : tw*cAn;
+        if (NS)
+        {
+            const TF ts = st[ijk];
+            s[ijk] += cBdt*ts; st[ijk] = zero ? TF(0) : ts*cAn;
+        }
     }
 
     // Pres_2::calc_divergence (src/pres_2.cxx:389-422), per-block maxima.  TC
@@ -1071,7 +1081,8 @@ namespace
     // rk3 >= 0: apply the gradient together with the RK3 update of substep
     // rk3 (time step rkdt) instead of Pres_2::output alone.
     template<typename TF>
-    int exec(mhh_ctx* c, const Prof<TF>& p, const Pres_tables<TF>& t, double dt, int rk3 = -1, double rkdt = 0.)
+    int exec(mhh_ctx* c, const Prof<TF>& p, const Pres_tables<TF>& t, double dt, int rk3 = -1, double rkdt = 0.,
+             bool with_scalars = false)
     {
         using T = cplx<TF>;
         const Gd& g = c->gd;
@@ -1175,11 +1186,27 @@ namespace
         const TF cA[] = {TF(0.), TF(-5./9.), TF(-153./128.)};
         const TF cB[] = {TF(1./3.), TF(15./16.), TF(8./15.)};
         const int substepn = (rk3 + 1) % 3;
-        { mhh::Timer _t(c, "pres_out_rk3"); pres_out_rk3_g<TF><<<grid3(g, g.kstart, g.kend), dim3(BX, BY), 0, c->stream>>>(
-                static_cast<TF*>(c->f.u), static_cast<TF*>(c->f.v), static_cast<TF*>(c->f.w),
-                static_cast<TF*>(c->f.ut), static_cast<TF*>(c->f.vt), static_cast<TF*>(c->f.wt),
-                P, p.dzhi, dxi, dyi, cB[rk3]*TF(rkdt), cA[substepn], substepn == 0, g, dpb); }
+        const TF cBdt = cB[rk3]*TF(rkdt);
+        // scalar 0 rides along with the momentum update; further scalars get
+        // their own pass
+        const int ns = with_scalars ? c->f.nscalars : 0;
+        {
+            TF* s0 = ns > 0 ? static_cast<TF*>(c->f.s[0]) : nullptr;
+            TF* st0 = ns > 0 ? static_cast<TF*>(c->f.st[0]) : nullptr;
+            mhh::Timer _t(c, "pres_out_rk3");
+#define MHH_PO(NS) pres_out_rk3_g<TF, NS><<<grid3(g, g.kstart, g.kend), dim3(BX, BY), 0, c->stream>>>( \
+                static_cast<TF*>(c->f.u), static_cast<TF*>(c->f.v), static_cast<TF*>(c->f.w), \
+                static_cast<TF*>(c->f.ut), static_cast<TF*>(c->f.vt), static_cast<TF*>(c->f.wt), s0, st0, \
+                P, p.dzhi, dpb, dxi, dyi, cBdt, cA[substepn], substepn == 0, g)
+            if (ns > 0) MHH_PO(1); else MHH_PO(0);
+#undef MHH_PO
+        }
         MHH_CHECK(check_launch("pres_out_rk3"));
+        for (int n = 1; n < ns; ++n)
+            MHH_CHECK(rk3_field<TF>(c, static_cast<TF*>(c->f.s[n]), static_cast<TF*>(c->f.st[n]),
+                                    cBdt, cA[substepn], substepn == 0));
+        if (substepn == 0 && ns > 0)
+            MHH_CHECK(zero_tendency_ghosts<TF>(c, static_cast<TF*>(c->f.st[0])));
         if (substepn == 0)
         {
             MHH_CHECK(zero_tendency_ghosts<TF>(c, static_cast<TF*>(c->f.ut)));
@@ -1260,13 +1287,13 @@ int mhh_pres_2_exec(mhh_ctx* c, double dt)
     return c->precision == MHH_FP64 ? exec<double>(c, c->pd, c->td, dt) : exec<float>(c, c->pf, c->tf, dt);
 }
 
-int mhh_pres_2_exec_rk3(mhh_ctx* c, double dt, int substep, double rkdt)
+int mhh_pres_2_exec_rk3(mhh_ctx* c, double dt, int substep, double rkdt, int scalars)
 {
     MHH_REQUIRE_BOUND(c);
     if (!(dt > 0)) return mhh::fail(MHH_EINVAL, "pres_2: dt must be positive");
     if (substep < 0 || substep > 2) return mhh::fail(MHH_EINVAL, "rk3 substep must be 0, 1 or 2");
-    return c->precision == MHH_FP64 ? exec<double>(c, c->pd, c->td, dt, substep, rkdt)
-                                    : exec<float>(c, c->pf, c->tf, dt, substep, rkdt);
+    return c->precision == MHH_FP64 ? exec<double>(c, c->pd, c->td, dt, substep, rkdt, scalars != 0)
+                                    : exec<float>(c, c->pf, c->tf, dt, substep, rkdt, scalars != 0);
 }
 
 int mhh_pres_2_mean_profile(mhh_ctx* c, double* out)

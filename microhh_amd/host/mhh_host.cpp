@@ -756,12 +756,10 @@ namespace mhh_host
         }
         if (fused_tendency)
         {
-            // Pres_2::exec with the RK3 update of u, v, w in its last kernel,
-            // then the RK3 update of the scalars (Timeloop::exec).
-            check(mhh_pres_2_exec_rk3(ctx_, subdt, timeloop->get_substep(), timeloop->get_dt()),
+            // Pres_2::exec with the RK3 update (Timeloop::exec) of u, v, w
+            // and the scalars in its last kernel.
+            check(mhh_pres_2_exec_rk3(ctx_, subdt, timeloop->get_substep(), timeloop->get_dt(), 1),
                   "mhh_pres_2_exec_rk3");
-            check(mhh_timeloop_rk3_scalars(ctx_, timeloop->get_substep(), timeloop->get_dt()),
-                  "mhh_timeloop_rk3_scalars");
             timeloop->advance_substep();
         }
         else

@@ -328,9 +328,11 @@ def test_rk3(setup, substep):
 
 
 @pytest.mark.parametrize("substep", [0, 1, 2])
-def test_pres_rk3_matches_pres_then_rk3(setup, substep):
-    """mhh_pres_2_exec_rk3 + mhh_timeloop_rk3_scalars against mhh_pres_2_exec
-    followed by mhh_timeloop_rk3: whole fields, ghost cells included."""
+@pytest.mark.parametrize("scalars", [1, 0])
+def test_pres_rk3_matches_pres_then_rk3(setup, substep, scalars):
+    """mhh_pres_2_exec_rk3 (with the scalars, or followed by
+    mhh_timeloop_rk3_scalars) against mhh_pres_2_exec followed by
+    mhh_timeloop_rk3: whole fields, ghost cells included."""
     p, i, d, g, *_ = setup
     dt, rkdt = 0.37, 0.61
 
@@ -340,8 +342,9 @@ def test_pres_rk3_matches_pres_then_rk3(setup, substep):
             g.call("mhh_boundary_cyclic", g.ptr(n), 2)
 
     load()
-    g.call("mhh_pres_2_exec_rk3", dt, substep, rkdt)
-    g.call("mhh_timeloop_rk3_scalars", substep, rkdt)
+    g.call("mhh_pres_2_exec_rk3", dt, substep, rkdt, scalars)
+    if not scalars:
+        g.call("mhh_timeloop_rk3_scalars", substep, rkdt)
     names = ("u", "v", "w", "ut", "vt", "wt", "s0", "st0")
     fused = {n: g.get(n) for n in names}
     load()
