@@ -178,6 +178,15 @@ def mixed_precision(mm, abi, args, fp64_model, fp64_ms_per_step):
             del fa, fb
         out["one_substep_rel_err"] = {k: float(f"{v:.3e}") for k, v in err.items()}
         out["div_max"] = float(f"{b.div():.3e}")
+        # the floor FP32 storage sets: the FP64 substep's projected u, v, w
+        # rounded to FP32, their divergence evaluated in FP64
+        c = mm.Model(fixed)
+        for n in ("u", "v", "w"):
+            f = a.field(n)
+            c.set_field(n, f)
+            del f
+        out["div_fp32_rounding_floor"] = float(f"{c.div():.3e}")
+        c.close()
         a.close(); b.close()
     return out
 

@@ -349,7 +349,23 @@ namespace
             const size_t rn = static_cast<size_t>(min(r + nw, nrows - 1));
             #pragma unroll
             for (int q = 0; q < R; ++q) nxt[q] = C[rn*n + lane + 64*q];
+            // FP32 forward: the row minus its FP64 mean, bin 0 set to the FP64
+            // sum afterwards (only bin 0 changes; the rest rounds against the
+            // fluctuations, not the mean)
+            constexpr bool demean = S < 0 && sizeof(TF) == 4;
+            double sx = 0., sy = 0.;
+            if (demean)
+            {
+                #pragma unroll
+                for (int q = 0; q < R; ++q) { sx += double(cur[q].x); sy += double(cur[q].y); }
+                #pragma unroll
+                for (int o = 32; o > 0; o >>= 1) { sx += __shfl_xor(sx, o); sy += __shfl_xor(sy, o); }
+                const TF mx = TF(sx/n), my = TF(sy/n);
+                #pragma unroll
+                for (int q = 0; q < R; ++q) { cur[q].x -= mx; cur[q].y -= my; }
+            }
             wave_fft<TF, R, S>(cur, A, wt, lane);
+            if (demean && lane == 0) { cur[0].x = TF(sx); cur[0].y = TF(sy); }
             T* __restrict__ Cr = C + static_cast<size_t>(r)*n;
             #pragma unroll
             for (int q = 0; q < R; ++q)
@@ -367,11 +383,14 @@ namespace
     // 8 real rows j0 .. j0+7 of level k (jtot a multiple of 8), one wave per
     // pair of rows packed as a + i b.  MODE 0: P as stored; 1: P minus the
     // plane mean pmean[k] (mixed mode); 2: the right-hand side of pres_in_g
-    // evaluated from u, v, w and their tendencies.  Writes C[k][kx][j].
+    // evaluated from u, v, w and their tendencies; 3: as 2, and the block's
+    // FP64 sum of the right-hand side into part[k*gridDim.x + blockIdx.x]
+    // (mixed mode: the plane means, whose mode the FP64 path solves).
+    // Writes C[k][kx][j].
     template<typename TF, int R, int MODE>
     __global__ void __launch_bounds__(256) fft_x_fwd_wave_g(
             const TF* __restrict__ P, cplx<TF>* __restrict__ C, const double2* __restrict__ tw, Gd g,
-            const double* __restrict__ pmean,
+            const double* __restrict__ pmean, double* __restrict__ part,
             const TF* __restrict__ u, const TF* __restrict__ v, const TF* __restrict__ w,
             const TF* __restrict__ ut, const TF* __restrict__ vt, const TF* __restrict__ wt,
             const TF* __restrict__ dzi, const TF* __restrict__ rho, const TF* __restrict__ rhoh,
@@ -381,13 +400,14 @@ namespace
         constexpr int n = 64*R, nx = n/2 + 1, JB = 8, P_ = wave_passes<R>();
         constexpr int NQ = (nx + 63)/64;
         __shared__ T rows[4][wave_row_elems<R>()];
+        __shared__ double wsum[4];
         const int lane = threadIdx.x, wv = threadIdx.y, tid = lane + 64*wv;
         const int M = g.jtot, k = blockIdx.y;
         const int j0 = blockIdx.x*JB, ja = j0 + 2*wv, jb = ja + 1;
         T* A = rows[wv];
 
         T z[R];
-        if (MODE == 2)
+        if (MODE >= 2)
         {
             // pres_in_g's expression; the east and north tendency neighbours
             // wrap periodically, u, v, w have periodic halos already
@@ -425,9 +445,27 @@ namespace
                 else           { z[q].x = Pk[x + ja*n]; z[q].y = Pk[x + jb*n]; }
             }
         }
+        double sa = 0., sb = 0.;
+        if (MODE == 3)
+        {
+            // FP64 sums of the wave's two rows in a fixed order (lane sums,
+            // then a butterfly), so the result is reproducible.  The rows go
+            // into the FP32 transform minus their means, which leaves every
+            // bin but kx = 0 unchanged and rounds the others against the
+            // fluctuations instead of the mean; bin 0 is set to the FP64 sum.
+            #pragma unroll
+            for (int q = 0; q < R; ++q) { sa += double(z[q].x); sb += double(z[q].y); }
+            #pragma unroll
+            for (int o = 32; o > 0; o >>= 1) { sa += __shfl_xor(sa, o); sb += __shfl_xor(sb, o); }
+            if (lane == 0) wsum[wv] = sa + sb;
+            const TF ma = TF(sa/n), mb = TF(sb/n);
+            #pragma unroll
+            for (int q = 0; q < R; ++q) { z[q].x -= ma; z[q].y -= mb; }
+        }
         T wtw[P_-1][R-1];
         wave_twiddles<TF, R, -1>(wtw, tw, lane);
         wave_fft<TF, R, -1>(z, A, wtw, lane);
+        if (MODE == 3 && lane == 0) { z[0].x = TF(sa); z[0].y = TF(sb); }   // x = 0 is lane 0, q = 0
 
         // the two half-spectra: Z[kx] and Z[n-kx] come from other lanes
         #pragma unroll
@@ -448,6 +486,8 @@ namespace
             if (kx < nx) stage_store<TF>(A, kx, n, xs[q][0], xs[q][1]);
         }
         __syncthreads();
+        if (MODE == 3 && tid == 0)
+            part[static_cast<size_t>(k)*gridDim.x + blockIdx.x] = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
         // C[k][kx][j0 .. j0+8): consecutive threads on consecutive j
         T* Ck = C + static_cast<size_t>(k)*nx*M;
         for (int idx = tid; idx < nx*JB; idx += 256)
@@ -776,49 +816,61 @@ namespace
     // plane means of P are its (0,0) Fourier coefficient divided by
     // itot*jtot, so the same tridiagonal system (lambda = 0, Neumann bottom,
     // p = 0 at the top face, src/pres_2.cxx:306-324) with the means as RHS gives
-    // the mean pressure directly.  One workgroup; thread 0 runs the sweep.
-    __global__ void __launch_bounds__(256) mean_mode_g(const double* __restrict__ part, double* __restrict__ pmean,
-                                                       double* __restrict__ gam, double* __restrict__ pbar,
-                                                       float* __restrict__ dpbar,
+    // the mean pressure directly.  One workgroup: the partial sums (nparts per
+    // level) are folded in parallel, thread 0 runs the sweep on LDS copies and
+    // the results go out in parallel.  Dynamic LDS: 2 ktot doubles.
+    __global__ void __launch_bounds__(256) mean_mode_g(const double* __restrict__ part, int nparts,
+                                                       double* __restrict__ pmean,
+                                                       double* __restrict__ pbar, float* __restrict__ dpbar,
                                                        const double* __restrict__ a, const double* __restrict__ c,
                                                        const double* __restrict__ ac, const double* __restrict__ dz2,
                                                        double inv_nplane, Gd g)
     {
+        extern __shared__ double mm[];
+        double* sp = mm;                // ktot: the sweep's right-hand side, then its solution
+        double* sg = mm + g.ktot;       // ktot: gamma
         const int K = g.ktot;
         for (int k = threadIdx.x; k < K; k += blockDim.x)
         {
             double s = 0.;
-            for (int q = 0; q < PM_CHUNKS; ++q) s += part[k*PM_CHUNKS + q];
-            pmean[k] = s*inv_nplane;
+            for (int q = 0; q < nparts; ++q) s += part[static_cast<size_t>(k)*nparts + q];
+            const double m = s*inv_nplane;
+            pmean[k] = m;
+            sp[k] = dz2[k]*m;
         }
         __syncthreads();
-        if (threadIdx.x != 0) return;
-        auto bcoef = [&](int k) -> double {
-            double b = -ac[k];
-            if (k == 0) b += a[0];
-            if (k == K-1) b -= c[K-1];
-            return b;
-        };
-        double w2 = bcoef(0);
-        double pk = dz2[0]*pmean[0] / w2;
-        pbar[g.kstart] = pk;
-        for (int k = 1; k < K; ++k)
+        if (threadIdx.x == 0)
         {
-            const double gk = c[k-1] / w2;
-            gam[k] = gk;
-            w2 = bcoef(k) - a[k]*gk;
-            pk = (dz2[k]*pmean[k] - a[k]*pk) / w2;
-            pbar[g.kstart + k] = pk;
+            auto bcoef = [&](int k) -> double {
+                double b = -ac[k];
+                if (k == 0) b += a[0];
+                if (k == K-1) b -= c[K-1];
+                return b;
+            };
+            double w2 = bcoef(0);
+            double pk = sp[0] / w2;
+            sp[0] = pk;
+            for (int k = 1; k < K; ++k)
+            {
+                const double gk = c[k-1] / w2;
+                sg[k] = gk;
+                w2 = bcoef(k) - a[k]*gk;
+                pk = (sp[k] - a[k]*pk) / w2;
+                sp[k] = pk;
+            }
+            for (int k = K-2; k >= 0; --k)
+            {
+                pk = sp[k] - sg[k+1]*pk;
+                sp[k] = pk;
+            }
         }
-        for (int k = K-2; k >= 0; --k)
+        __syncthreads();
+        for (int k = threadIdx.x; k < K; k += blockDim.x)
         {
-            pk = pbar[g.kstart + k] - gam[k+1]*pk;
-            pbar[g.kstart + k] = pk;
+            pbar[g.kstart + k] = sp[k];
+            dpbar[g.kstart + k] = k == 0 ? 0.f : float(sp[k] - sp[k-1]);
         }
-        pbar[g.kstart-1] = pbar[g.kstart];
-        dpbar[g.kstart] = 0.f;
-        for (int k = g.kstart+1; k < g.kend; ++k)
-            dpbar[k] = float(pbar[k] - pbar[k-1]);
+        if (threadIdx.x == 0) pbar[g.kstart-1] = sp[0];
     }
 
     // Pres_2::output (src/pres_2.cxx:364-387).
@@ -1035,9 +1087,10 @@ namespace
         const TF* u = static_cast<const TF*>(c->f.u); const TF* v = static_cast<const TF*>(c->f.v);
         const TF* w = static_cast<const TF*>(c->f.w); const TF* ut = static_cast<const TF*>(c->f.ut);
         const TF* vt = static_cast<const TF*>(c->f.vt); const TF* wt = static_cast<const TF*>(c->f.wt);
-#define MHH_XF(MODE) fft_x_fwd_wave_g<TF, R, MODE><<<grd, blk, 0, c->stream>>>(P, C, tw, g, c->pm_mean, \
+#define MHH_XF(MODE) fft_x_fwd_wave_g<TF, R, MODE><<<grd, blk, 0, c->stream>>>(P, C, tw, g, c->pm_mean, c->pm_part, \
                 u, v, w, ut, vt, wt, p.dzi, p.rhoref, p.rhorefh, dxi, dyi, dti)
-        if (mode == 2) MHH_XF(2);
+        if (mode == 3) MHH_XF(3);
+        else if (mode == 2) MHH_XF(2);
         else if (mode == 1) MHH_XF(1);
         else MHH_XF(0);
 #undef MHH_XF
@@ -1103,7 +1156,13 @@ namespace
         Pin_args<TF> pin{static_cast<const TF*>(c->f.u), static_cast<const TF*>(c->f.v), static_cast<const TF*>(c->f.w),
                          static_cast<const TF*>(c->f.ut), static_cast<const TF*>(c->f.vt), static_cast<const TF*>(c->f.wt),
                          p.dzi, p.rhoref, p.rhorefh, dxi, dyi, dti};
-        const int fuse_in = c->mixed ? 0 : 1;
+        const bool xwave = !c->fft_generic && (N == 128 || N == 256 || N == 512) && M % 8 == 0 && g.igc <= 32;
+        // The wave x transform evaluates the right-hand side itself (and, in
+        // the mixed mode, the FP64 plane sums); the general engine does so
+        // except in the mixed mode, whose plane means come first.
+        const int fuse_in = (xwave || !c->mixed) ? 1 : 0;
+        const Pres_tables<double>& d = c->td;
+        const size_t mm_lds = 2*sizeof(double)*K;
         if (!fuse_in)
         {
             const dim3 grd((N + BX - 1)/BX, (M + BY - 1)/BY, K);
@@ -1112,22 +1171,23 @@ namespace
                     static_cast<const TF*>(c->f.ut), static_cast<const TF*>(c->f.vt), static_cast<const TF*>(c->f.wt),
                     p.dzi, p.rhoref, p.rhorefh, dxi, dyi, dti, g); }
             MHH_CHECK(check_launch("pres_in"));
+            { mhh::Timer _t(c, "pres_mean"); plane_sum_g<TF><<<dim3(PM_CHUNKS, K), 256, 0, c->stream>>>(P, c->pm_part, N*M); }
+            MHH_CHECK(check_launch("plane_sum"));
+            { mhh::Timer _t(c, "pres_mean"); mean_mode_g<<<1, 256, mm_lds, c->stream>>>(c->pm_part, PM_CHUNKS, c->pm_mean,
+                    c->pbar, c->dpbar, d.a, d.c, d.ac, d.dz2, 1./(double(N)*double(M)), g); }
+            MHH_CHECK(check_launch("mean_mode"));
+        }
+        // 2. forward x transform
+        if (xwave)
+        {
+            const int mode = c->mixed ? 3 : 2;
+            MHH_CHECK(fft_x_fwd_wave<TF>(c, P, C, t.twx, mode, p, dxi, dyi, dti, "pres_in_fft_x"));
             if (c->mixed)
             {
-                { mhh::Timer _t(c, "pres_mean"); plane_sum_g<TF><<<dim3(PM_CHUNKS, K), 256, 0, c->stream>>>(P, c->pm_part, N*M); }
-                MHH_CHECK(check_launch("plane_sum"));
-                const Pres_tables<double>& d = c->td;
-                { mhh::Timer _t(c, "pres_mean"); mean_mode_g<<<1, 256, 0, c->stream>>>(c->pm_part, c->pm_mean, c->pm_work,
+                { mhh::Timer _t(c, "pres_mean"); mean_mode_g<<<1, 256, mm_lds, c->stream>>>(c->pm_part, M/8, c->pm_mean,
                         c->pbar, c->dpbar, d.a, d.c, d.ac, d.dz2, 1./(double(N)*double(M)), g); }
                 MHH_CHECK(check_launch("mean_mode"));
             }
-        }
-        // 2. forward x transform
-        const bool xwave = !c->fft_generic && (N == 128 || N == 256 || N == 512) && M % 8 == 0 && g.igc <= 32;
-        if (xwave)
-        {
-            const int mode = fuse_in ? 2 : (c->mixed ? 1 : 0);
-            MHH_CHECK(fft_x_fwd_wave<TF>(c, P, C, t.twx, mode, p, dxi, dyi, dti, fuse_in ? "pres_in_fft_x" : "fft_x_fwd"));
         }
         else
         {
@@ -1256,14 +1316,15 @@ namespace mhh
         if (c->mixed)
         {
             const size_t K = c->gd.ktot, nk = c->gd.kcells;
-            MHH_CHECK(check_hip(hipMalloc(&c->pm_part, sizeof(double)*K*PM_CHUNKS), "hipMalloc(pm_part)"));
+            // partial plane sums: PM_CHUNKS per level (plane_sum_g) or one per
+            // block of 8 rows (the wave x transform)
+            const size_t nparts = std::max<size_t>(PM_CHUNKS, (c->gd.jtot + 7)/8);
+            MHH_CHECK(check_hip(hipMalloc(&c->pm_part, sizeof(double)*K*nparts), "hipMalloc(pm_part)"));
             MHH_CHECK(check_hip(hipMalloc(&c->pm_mean, sizeof(double)*K), "hipMalloc(pm_mean)"));
-            MHH_CHECK(check_hip(hipMalloc(&c->pm_work, sizeof(double)*K), "hipMalloc(pm_work)"));
             MHH_CHECK(check_hip(hipMalloc(&c->pbar, sizeof(double)*nk), "hipMalloc(pbar)"));
             MHH_CHECK(check_hip(hipMalloc(&c->dpbar, sizeof(float)*nk), "hipMalloc(dpbar)"));
             MHH_CHECK(check_hip(hipMemset(c->pbar, 0, sizeof(double)*nk), "hipMemset(pbar)"));
             MHH_CHECK(check_hip(hipMemset(c->dpbar, 0, sizeof(float)*nk), "hipMemset(dpbar)"));
-            MHH_CHECK(check_hip(hipMemset(c->pm_work, 0, sizeof(double)*K), "hipMemset(pm_work)"));
         }
         return 0;
     }
@@ -1272,7 +1333,7 @@ namespace mhh
     {
         free_tables(c->td);
         free_tables(c->tf);
-        for (void* b : {static_cast<void*>(c->pm_part), static_cast<void*>(c->pm_mean), static_cast<void*>(c->pm_work),
+        for (void* b : {static_cast<void*>(c->pm_part), static_cast<void*>(c->pm_mean),
                         static_cast<void*>(c->pbar), static_cast<void*>(c->dpbar)})
             if (b) (void)hipFree(b);
     }
