@@ -329,26 +329,38 @@ namespace
         const int c = (threadIdx.y + 1)*VPW + (threadIdx.x + 1);
         double cm = 0., dm = 0.;
         TF ru[VNQ], rv[VNQ], rw[VNQ];
+        // CFL: w of the own column at k-2 .. k+3 in a register queue (levels
+        // outside the field repeat the nearest one; the wall stencils never
+        // reach them)
+        TF wq[6];
+        if (limits)
+        {
+            #pragma unroll
+            for (int m = 0; m < 6; ++m)
+                wq[m] = vldo(w + size_t(min(max(kb - 2 + m, 0), g.kcells - 1))*kk, oij);
+        }
         __builtin_amdgcn_s_waitcnt(0);
 
         for (int k = kb; k < ke; ++k)
         {
             // Issue order sets what a wait covers (loads complete in order): the
-            // CFL stencil of this level first, then theta of level k+2, then the
-            // planes of level k+2, which stay in flight until the end of the
-            // level.  Every level issues the same loads (clamped levels).
-            TF cu[6], cv[6], cw[6];
+            // CFL stencil values this level needs beyond the LDS planes first
+            // (u at i-2, i+2, i+3 and v at j-2, j+2, j+3, in the cache lines of
+            // the planes, and w of the own column at k+4 for the queue), then
+            // theta of level k+2, then the planes of level k+2, which stay in
+            // flight until the end of the level.  Every level issues the same
+            // loads (clamped levels).
+            TF cu[3], cv[3], wn = TF(0);
             if (limits)
             {
                 const TF* __restrict__ uk = u + size_t(k)*kk;
                 const TF* __restrict__ vk = v + size_t(k)*kk;
-                #pragma unroll
-                for (int m = 0; m < 6; ++m)
-                {
-                    cu[m] = vldo(uk, oij + uint32_t((m-2)*int(sizeof(TF))));
-                    cv[m] = vldo(vk, oij + uint32_t((m-2)*g.icells*int(sizeof(TF))));
-                    cw[m] = vldo(w + size_t(min(max(k-2+m, 0), g.kcells-1))*kk, oij);
-                }
+                const int sz = int(sizeof(TF)), jb = g.icells*int(sizeof(TF));
+                cu[0] = vldo(uk, oij - uint32_t(2*sz)); cu[1] = vldo(uk, oij + uint32_t(2*sz));
+                cu[2] = vldo(uk, oij + uint32_t(3*sz));
+                cv[0] = vldo(vk, oij - uint32_t(2*jb)); cv[1] = vldo(vk, oij + uint32_t(2*jb));
+                cv[2] = vldo(vk, oij + uint32_t(3*jb));
+                wn = vldo(w + size_t(min(k + 4, g.kcells - 1))*kk, oij);
                 __builtin_amdgcn_sched_barrier(0);
             }
             const int k2 = min(k + 2, g.kcells - 1);
@@ -390,13 +402,13 @@ namespace
                 // the 2i5 CFL rate as in cfl_g
                 TF wpart;
                 if (k == g.kstart || k == g.kend-1)
-                    wpart = fabs(i2(cw[2], cw[3]))*L.dzi;
+                    wpart = fabs(i2(wq[2], wq[3]))*L.dzi;
                 else if (k == g.kstart+1 || k == g.kend-2)
-                    wpart = fabs(i4ws(cw[1], cw[2], cw[3], cw[4]))*L.dzi;
+                    wpart = fabs(i4ws(wq[1], wq[2], wq[3], wq[4]))*L.dzi;
                 else
-                    wpart = fabs(i6ws(cw[0], cw[1], cw[2], cw[3], cw[4], cw[5]))*L.dzi;
-                const TF val = fabs(i6ws(cu[0], cu[1], cu[2], cu[3], cu[4], cu[5]))*a.dxi
-                             + fabs(i6ws(cv[0], cv[1], cv[2], cv[3], cv[4], cv[5]))*a.dyi
+                    wpart = fabs(i6ws(wq[0], wq[1], wq[2], wq[3], wq[4], wq[5]))*L.dzi;
+                const TF val = fabs(i6ws(cu[0], Uc[c-1], Uc[c], Uc[c+1], cu[1], cu[2]))*a.dxi
+                             + fabs(i6ws(cv[0], Vc[c-VPW], Vc[c], Vc[c+VPW], cv[1], cv[2]))*a.dyi
                              + wpart;
                 cm = fmax(cm, static_cast<double>(val));
             }
@@ -409,6 +421,12 @@ namespace
             vplane_store(Vp, rv, tid);
             vplane_store(Wp, rw, tid);
             t0 = t1; t1 = t2; t2 = tn;
+            if (limits)
+            {
+                #pragma unroll
+                for (int m = 0; m < 5; ++m) wq[m] = wq[m+1];
+                wq[5] = wn;
+            }
         }
         if (limits)
         {

@@ -73,6 +73,7 @@ struct mhh_ctx
     // Mixed mode: the horizontal mean of the pressure, solved in FP64.
     double* pm_part = nullptr;   // ktot x PM_CHUNKS partial plane sums of the RHS
     double* pm_mean = nullptr;   // ktot: plane mean of the RHS
+    double* pm_coef = nullptr;   // 2 ktot: reciprocal pivots and gamma of the mean-mode elimination
     double* pbar = nullptr;      // kcells: mean pressure per level
     float* dpbar = nullptr;      // kcells: pbar[k] - pbar[k-1], the mean's part of the z gradient
     hipStream_t stream = nullptr;

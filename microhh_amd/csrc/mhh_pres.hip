@@ -816,20 +816,21 @@ namespace
     // plane means of P are its (0,0) Fourier coefficient divided by
     // itot*jtot, so the same tridiagonal system (lambda = 0, Neumann bottom,
     // p = 0 at the top face, src/pres_2.cxx:306-324) with the means as RHS gives
-    // the mean pressure directly.  One workgroup: the partial sums (nparts per
-    // level) are folded in parallel, thread 0 runs the sweep on LDS copies and
-    // the results go out in parallel.  Dynamic LDS: 2 ktot doubles.
+    // the mean pressure directly.  Its elimination depends on the grid only:
+    // coef holds the reciprocal pivots (ktot) and gamma (ktot), so the sweep is
+    // a chain of multiply-adds.  One workgroup: the partial sums (nparts per
+    // level) are folded in parallel, thread 0 sweeps an LDS copy, the results
+    // go out in parallel.  Dynamic LDS: ktot doubles.
     __global__ void __launch_bounds__(256) mean_mode_g(const double* __restrict__ part, int nparts,
                                                        double* __restrict__ pmean,
                                                        double* __restrict__ pbar, float* __restrict__ dpbar,
-                                                       const double* __restrict__ a, const double* __restrict__ c,
-                                                       const double* __restrict__ ac, const double* __restrict__ dz2,
-                                                       double inv_nplane, Gd g)
+                                                       const double* __restrict__ a, const double* __restrict__ dz2,
+                                                       const double* __restrict__ coef, double inv_nplane, Gd g)
     {
-        extern __shared__ double mm[];
-        double* sp = mm;                // ktot: the sweep's right-hand side, then its solution
-        double* sg = mm + g.ktot;       // ktot: gamma
+        extern __shared__ double sp[];   // the sweep's right-hand side, then its solution
         const int K = g.ktot;
+        const double* __restrict__ wi = coef;
+        const double* __restrict__ gam = coef + K;
         for (int k = threadIdx.x; k < K; k += blockDim.x)
         {
             double s = 0.;
@@ -841,26 +842,16 @@ namespace
         __syncthreads();
         if (threadIdx.x == 0)
         {
-            auto bcoef = [&](int k) -> double {
-                double b = -ac[k];
-                if (k == 0) b += a[0];
-                if (k == K-1) b -= c[K-1];
-                return b;
-            };
-            double w2 = bcoef(0);
-            double pk = sp[0] / w2;
+            double pk = sp[0]*wi[0];
             sp[0] = pk;
             for (int k = 1; k < K; ++k)
             {
-                const double gk = c[k-1] / w2;
-                sg[k] = gk;
-                w2 = bcoef(k) - a[k]*gk;
-                pk = (sp[k] - a[k]*pk) / w2;
+                pk = (sp[k] - a[k]*pk)*wi[k];
                 sp[k] = pk;
             }
             for (int k = K-2; k >= 0; --k)
             {
-                pk = sp[k] - sg[k+1]*pk;
+                pk = sp[k] - gam[k+1]*pk;
                 sp[k] = pk;
             }
         }
@@ -1162,7 +1153,7 @@ namespace
         // except in the mixed mode, whose plane means come first.
         const int fuse_in = (xwave || !c->mixed) ? 1 : 0;
         const Pres_tables<double>& d = c->td;
-        const size_t mm_lds = 2*sizeof(double)*K;
+        const size_t mm_lds = sizeof(double)*K;
         if (!fuse_in)
         {
             const dim3 grd((N + BX - 1)/BX, (M + BY - 1)/BY, K);
@@ -1174,7 +1165,7 @@ namespace
             { mhh::Timer _t(c, "pres_mean"); plane_sum_g<TF><<<dim3(PM_CHUNKS, K), 256, 0, c->stream>>>(P, c->pm_part, N*M); }
             MHH_CHECK(check_launch("plane_sum"));
             { mhh::Timer _t(c, "pres_mean"); mean_mode_g<<<1, 256, mm_lds, c->stream>>>(c->pm_part, PM_CHUNKS, c->pm_mean,
-                    c->pbar, c->dpbar, d.a, d.c, d.ac, d.dz2, 1./(double(N)*double(M)), g); }
+                    c->pbar, c->dpbar, d.a, d.dz2, c->pm_coef, 1./(double(N)*double(M)), g); }
             MHH_CHECK(check_launch("mean_mode"));
         }
         // 2. forward x transform
@@ -1185,7 +1176,7 @@ namespace
             if (c->mixed)
             {
                 { mhh::Timer _t(c, "pres_mean"); mean_mode_g<<<1, 256, mm_lds, c->stream>>>(c->pm_part, M/8, c->pm_mean,
-                        c->pbar, c->dpbar, d.a, d.c, d.ac, d.dz2, 1./(double(N)*double(M)), g); }
+                        c->pbar, c->dpbar, d.a, d.dz2, c->pm_coef, 1./(double(N)*double(M)), g); }
                 MHH_CHECK(check_launch("mean_mode"));
             }
         }
@@ -1325,6 +1316,34 @@ namespace mhh
             MHH_CHECK(check_hip(hipMalloc(&c->dpbar, sizeof(float)*nk), "hipMalloc(dpbar)"));
             MHH_CHECK(check_hip(hipMemset(c->pbar, 0, sizeof(double)*nk), "hipMemset(pbar)"));
             MHH_CHECK(check_hip(hipMemset(c->dpbar, 0, sizeof(float)*nk), "hipMemset(dpbar)"));
+            // elimination of the mean mode (lambda = 0; Neumann bottom, Dirichlet
+            // top, src/pres_2.cxx:306-324): reciprocal pivots and gamma, FP64
+            const Gd& g = c->gd;
+            const int Kc = g.ktot;
+            std::vector<double> av(Kc), cv(Kc), coef(2*static_cast<size_t>(Kc), 0.);
+            for (int k = 0; k < Kc; ++k)
+            {
+                av[k] = c->h_dz[k+g.kgc] * c->h_rhorefh[k+g.kgc  ]*c->h_dzhi[k+g.kgc  ];
+                cv[k] = c->h_dz[k+g.kgc] * c->h_rhorefh[k+g.kgc+1]*c->h_dzhi[k+g.kgc+1];
+            }
+            auto bcoef = [&](int k) {
+                double b = -(av[k] + cv[k]);
+                if (k == 0) b += av[0];
+                if (k == Kc-1) b -= cv[Kc-1];
+                return b;
+            };
+            double w2 = bcoef(0);
+            coef[0] = 1./w2;
+            for (int k = 1; k < Kc; ++k)
+            {
+                const double gk = cv[k-1]/w2;
+                coef[Kc + k] = gk;
+                w2 = bcoef(k) - av[k]*gk;
+                coef[k] = 1./w2;
+            }
+            MHH_CHECK(check_hip(hipMalloc(&c->pm_coef, sizeof(double)*coef.size()), "hipMalloc(pm_coef)"));
+            MHH_CHECK(check_hip(hipMemcpy(c->pm_coef, coef.data(), sizeof(double)*coef.size(), hipMemcpyHostToDevice),
+                                "hipMemcpy(pm_coef)"));
         }
         return 0;
     }
@@ -1333,7 +1352,7 @@ namespace mhh
     {
         free_tables(c->td);
         free_tables(c->tf);
-        for (void* b : {static_cast<void*>(c->pm_part), static_cast<void*>(c->pm_mean),
+        for (void* b : {static_cast<void*>(c->pm_part), static_cast<void*>(c->pm_mean), static_cast<void*>(c->pm_coef),
                         static_cast<void*>(c->pbar), static_cast<void*>(c->dpbar)})
             if (b) (void)hipFree(b);
     }
