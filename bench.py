@@ -43,8 +43,8 @@ HBM_PEAK_GBS = 8000.0
 #   buoyancy: th, wt; wt = 3.
 #   pres_in_fft_x (right-hand side evaluated inside the forward x transform):
 #       u, v, w, ut, vt, wt in; spectrum out = 7.
-#   pres_out_rk3 (gradient + RK3 of u, v, w): p, u, v, w, ut, vt, wt in;
-#       u, v, w, ut, vt, wt out = 13.
+#   pres_out_rk3 (gradient + RK3 of u, v, w and scalar 0): p, u, v, w, th,
+#       ut, vt, wt, tht in; u, v, w, th, ut, vt, wt, tht out = 17.
 #   tend_2i5_smag2 (fused buoyancy + advection + diffusion of u, v, w, th):
 #       u, v, w, th, evisc, ut, vt, wt, tht in; ut, vt, wt, tht out = 13.
 PASSES = {
@@ -52,11 +52,9 @@ PASSES = {
     "advec_2i5_u": 5, "advec_2i5_v": 5, "advec_2i5_w": 5, "advec_2i5_s": 6,
     "evisc": 5, "diff_u": 6, "diff_v": 6, "diff_w": 6, "diff_c": 4,
     "pres_in": 7, "pres_in_fft_x": 7, "fft_x_fwd": 2, "fft_y_fwd": 2, "tdma": 2, "fft_y_bwd": 2, "fft_x_bwd": 2,
-    # gradient + RK3 of u, v, w and scalar 0: reads p, u, v, w, s and the four
-    # tendencies, writes the eight prognostic arrays
     "pres_out": 7, "pres_out_rk3": 17, "rk3": 4, "buoyancy": 3,
 }
-KERNELS = list(PASSES) + ["cyclic", "ghost_cells", "mirror_walls", "rk3_zero_ghosts", "cfl", "dnmul"]
+KERNELS = list(PASSES) + ["cyclic", "ghost_cells", "mirror_walls", "rk3_zero_ghosts", "cfl", "dnmul", "pres_mean"]
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (scaled by the copy-kernel calibration: x2.000 / x1.000) on the default
 # workload; profiles/dycore_fp64_512x512x256_hbm_traffic.md holds the
@@ -131,6 +129,26 @@ def copy_calibration(nbytes=1 << 30, reps=5):
                       "ms_per_copy": round(el * 1e3, 4), "gbs_read_plus_write": round(2 * nbytes / el / 1e9, 1)}))
 
 
+def kernel_table(L, abi, ctx, npts, esize, itot):
+    """Per-kernel HIP-event times of the timed region (mhh_timing_*), with the
+    algorithmic bytes and rate of the kernels in PASSES."""
+    kernels = {}
+    for k in KERNELS:
+        tot, cnt = C.c_double(), C.c_int()
+        abi.check(L.mhh_timing_query(ctx, k.encode(), C.byref(tot), C.byref(cnt)), "timing_query")
+        if cnt.value:
+            entry = {"ms_total": round(tot.value, 3), "launches": cnt.value,
+                     "ms_mean": round(tot.value / cnt.value, 4)}
+            if k in PASSES:
+                nbytes = PASSES[k] * npts * esize
+                if k.startswith("fft") or k == "tdma":
+                    nbytes = int(nbytes * (itot + 2) / itot)
+                entry["alg_bytes"] = nbytes
+                entry["gbs"] = round(nbytes / (tot.value / cnt.value * 1e-3) / 1e9, 1)
+            kernels[k] = entry
+    return kernels
+
+
 def mixed_precision(mm, abi, args, fp64_model, fp64_ms_per_step):
     """The mixed-precision half of the metric: the same workload with FP32
     fields and transforms (FP64 tridiagonal, mean pressure and divergence),
@@ -145,16 +163,24 @@ def mixed_precision(mm, abi, args, fp64_model, fp64_ms_per_step):
     log("bench: mixed-precision model")
     m = mm.Model(mm.dry_cbl_case(precision=mm.MHH_MIXED, **kw))
     m.steps(args.warmup)
+    ctx = m.ctx()
+    timing = not args.no_timing
+    if timing:
+        abi.check(L.mhh_timing_enable(ctx, 1), "timing_enable")
+        abi.check(L.mhh_timing_reset(ctx), "timing_reset")
     abi.check(L.mhh_device_synchronize(), "sync")
     t0 = time.perf_counter()
     m.steps(args.steps)
     abi.check(L.mhh_device_synchronize(), "sync")
     el = time.perf_counter() - t0
-    m.close()
     npts = itot * jtot * ktot
+    kernels = kernel_table(L, abi, ctx, npts, 4, itot) if timing else {}
+    m.close()
     ms = el / args.steps * 1e3
     out = {"value": round(npts * 3 * args.steps / el / 1e6, 2), "unit": UNIT, "ms_per_step": round(ms, 3),
            "speedup_vs_fp64": round(fp64_ms_per_step / ms, 3), "dtype": "f32 (f64 tridiagonal, mean pressure, divergence)"}
+    if kernels:
+        out["kernels_ms_mean"] = {k: v["ms_mean"] for k, v in kernels.items()}
     # one substep from the FP64 state, with a fixed dt so both take the same step
     log("bench: mixed-precision one-substep error against FP64")
     t, dt, substep, _ = fp64_model.time()
@@ -248,19 +274,7 @@ def main():
 
     kernels = {}
     if timing:
-        for k in KERNELS:
-            tot, cnt = C.c_double(), C.c_int()
-            abi.check(L.mhh_timing_query(ctx, k.encode(), C.byref(tot), C.byref(cnt)), "timing_query")
-            if cnt.value:
-                entry = {"ms_total": round(tot.value, 3), "launches": cnt.value,
-                         "ms_mean": round(tot.value / cnt.value, 4)}
-                if k in PASSES:
-                    nbytes = PASSES[k] * npts * esize
-                    if k.startswith("fft") or k == "tdma":
-                        nbytes = int(nbytes * (itot + 2) / itot)
-                    entry["alg_bytes"] = nbytes
-                    entry["gbs"] = round(nbytes / (tot.value / cnt.value * 1e-3) / 1e9, 1)
-                kernels[k] = entry
+        kernels = kernel_table(L, abi, ctx, npts, esize, itot)
         abi.check(L.mhh_timing_enable(ctx, 0), "timing_disable")
 
     roof = None

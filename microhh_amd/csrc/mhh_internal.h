@@ -127,6 +127,14 @@ namespace mhh
         return dim3((g.itot + BX - 1)/BX, (g.jtot + BY - 1)/BY, klast > kfirst ? klast - kfirst : 0);
     }
 
+    // XCD-contiguous block order.  Workgroups are dealt to the 8 XCDs round
+    // robin (block b to XCD b % 8), each with its own L2; a 1-D grid of nb
+    // blocks (nb a multiple of 8) mapped through xcd_tile gives every XCD a
+    // contiguous range of tiles, so the rows and planes neighbouring tiles
+    // share are fetched into one L2, not eight.
+    __host__ __device__ inline int xcd_tile(int b, int nb) { return (b % 8)*(nb/8) + b/8; }
+    inline int xcd_blocks(int ntiles) { return (ntiles + 7)/8*8; }
+
     // Max-reductions: each thread folds a chunk of RED_KCHUNK levels of its
     // column, each block writes one partial, one block folds the partials.
     constexpr int RED_KCHUNK = 32;

@@ -402,8 +402,8 @@ namespace
         __shared__ T rows[4][wave_row_elems<R>()];
         __shared__ double wsum[4];
         const int lane = threadIdx.x, wv = threadIdx.y, tid = lane + 64*wv;
-        const int M = g.jtot, k = blockIdx.y;
-        const int j0 = blockIdx.x*JB, ja = j0 + 2*wv, jb = ja + 1;
+        const int M = g.jtot, k = blockIdx.y, bj = blockIdx.x, nbj = gridDim.x;
+        const int j0 = bj*JB, ja = j0 + 2*wv, jb = ja + 1;
         T* A = rows[wv];
 
         T z[R];
@@ -487,7 +487,7 @@ namespace
         }
         __syncthreads();
         if (MODE == 3 && tid == 0)
-            part[static_cast<size_t>(k)*gridDim.x + blockIdx.x] = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+            part[static_cast<size_t>(k)*nbj + bj] = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
         // C[k][kx][j0 .. j0+8): consecutive threads on consecutive j
         T* Ck = C + static_cast<size_t>(k)*nx*M;
         for (int idx = tid; idx < nx*JB; idx += 256)
@@ -895,11 +895,15 @@ namespace
             TF* __restrict__ ut, TF* __restrict__ vt, TF* __restrict__ wt,
             TF* __restrict__ s, TF* __restrict__ st,
             const TF* __restrict__ p, const TF* __restrict__ dzhi, const TF* __restrict__ dpbar,
-            TF dxi, TF dyi, TF cBdt, TF cAn, int zero, Gd g)
+            TF dxi, TF dyi, TF cBdt, TF cAn, int zero, Gd g, int nbx, int nby)
     {
-        const int i = blockIdx.x*BX + threadIdx.x + g.istart;
-        const int j = blockIdx.y*BY + threadIdx.y + g.jstart;
-        const int k = blockIdx.z + g.kstart;
+        // 1-D grid in XCD-contiguous order: the p rows and plane a tile reads
+        // from its neighbours sit in the same L2
+        const int tile = xcd_tile(blockIdx.x, gridDim.x);
+        if (tile >= nbx*nby*g.ktot) return;
+        const int i = (tile % nbx)*BX + threadIdx.x + g.istart;
+        const int j = ((tile / nbx) % nby)*BY + threadIdx.y + g.jstart;
+        const int k = tile/(nbx*nby) + g.kstart;
         if (i >= g.iend || j >= g.jend) return;
         const int ijk = i + j*g.icells + k*g.ijcells;
         const TF pc = p[ijk];
@@ -1245,10 +1249,12 @@ namespace
             TF* s0 = ns > 0 ? static_cast<TF*>(c->f.s[0]) : nullptr;
             TF* st0 = ns > 0 ? static_cast<TF*>(c->f.st[0]) : nullptr;
             mhh::Timer _t(c, "pres_out_rk3");
-#define MHH_PO(NS) pres_out_rk3_g<TF, NS><<<grid3(g, g.kstart, g.kend), dim3(BX, BY), 0, c->stream>>>( \
+            const int nbx = (g.itot + BX - 1)/BX, nby = (g.jtot + BY - 1)/BY;
+            const int nb = xcd_blocks(nbx*nby*g.ktot);
+#define MHH_PO(NS) pres_out_rk3_g<TF, NS><<<nb, dim3(BX, BY), 0, c->stream>>>( \
                 static_cast<TF*>(c->f.u), static_cast<TF*>(c->f.v), static_cast<TF*>(c->f.w), \
                 static_cast<TF*>(c->f.ut), static_cast<TF*>(c->f.vt), static_cast<TF*>(c->f.wt), s0, st0, \
-                P, p.dzhi, dpb, dxi, dyi, cBdt, cA[substepn], substepn == 0, g)
+                P, p.dzhi, dpb, dxi, dyi, cBdt, cA[substepn], substepn == 0, g, nbx, nby)
             if (ns > 0) MHH_PO(1); else MHH_PO(0);
 #undef MHH_PO
         }
