@@ -62,9 +62,10 @@ KERNELS = list(PASSES) + ["cyclic", "ghost_cells", "mirror_walls", "rk3_zero_gho
 # workload and the dominant kernel match.
 MEASURED_TRAFFIC = {
     (512, 512, 256, 8): {
-        "tend_2i5_smag2": 9053.3e6, "pres_out_rk3": 8322.1e6, "pres_in": 4687.4e6, "evisc": 4569.1e6,
-        "tdma": 2671.8e6, "rk3": 2483.0e6, "fft_x_bwd": 1091.7e6, "fft_y_fwd": 1078.1e6,
-        "fft_y_bwd": 1078.1e6, "fft_x_fwd": 1076.2e6,
+        "tend_2i5_smag2": 9904.5e6, "pres_out_rk3": 9777.6e6, "pres_in_fft_x": 5205.1e6,
+        # one launch with the CFL/dnmul maxima per three: (4432.3 + 2 x 3248.7)/3
+        "evisc": 3643.2e6,
+        "tdma": 2673.2e6, "fft_x_bwd": 1091.5e6, "fft_y_fwd": 1078.4e6, "fft_y_bwd": 1078.4e6,
     },
 }
 # The canonical fused minimum per grid-point-substep (40 passes).
