@@ -62,6 +62,13 @@ namespace mhh
         TF* dz2rho = nullptr;   // kmax: dz*dz*rhoref
         double* twx = nullptr;  // itot complex roots exp(-2 pi i t / itot), (re, im) pairs
         double* twy = nullptr;  // jtot complex roots
+        // Elimination of the tridiagonal systems, which depend on the mode
+        // only through lambda = bmati[kx] + bmatj[ky] (bmatj[ky] =
+        // bmatj[jtot-ky]): one system per l = kx*(jtot/2+1) + min(ky, jtot-ky),
+        // nsys of them, stored [k][l].
+        int nsys = 0;
+        TF* tri_wi = nullptr;   // ktot x nsys: reciprocal pivots
+        TF* tri_gam = nullptr;  // ktot x nsys: gamma (k >= 1)
     };
 }
 

@@ -611,57 +611,45 @@ namespace
 
     // Tridiagonal solve per (kx, ky) mode (Pres_2::solve + tdma,
     // src/pres_2.cxx:266-324, 200-263), one thread per mode marching in k.
-    // TF is the element type of the spectrum, TC the arithmetic type of the
-    // sweep (double in the mixed mode).
+    // The elimination (reciprocal pivots, gamma) depends on the mode only
+    // through lambda and is precomputed per system (Pres_tables::tri_*), so
+    // the sweeps read it instead of writing and re-reading gamma, and divide
+    // nowhere.  TF is the element type of the spectrum, TC the arithmetic type
+    // of the sweep (double in the mixed mode).
     template<typename TF, typename TC>
-    __global__ void __launch_bounds__(256) tdma_g(cplx<TF>* __restrict__ C, TC* __restrict__ gam,
-                                                  const TC* __restrict__ bmati, const TC* __restrict__ bmatj,
-                                                  const TC* __restrict__ a, const TC* __restrict__ c,
-                                                  const TC* __restrict__ ac, const TC* __restrict__ dz2,
-                                                  const TC* __restrict__ dz2rho, int nx, Gd g, bool skip_mean)
+    __global__ void __launch_bounds__(256) tdma_g(cplx<TF>* __restrict__ C,
+                                                  const TC* __restrict__ wi, const TC* __restrict__ gam,
+                                                  const TC* __restrict__ a, const TC* __restrict__ dz2,
+                                                  int nx, int nsys, Gd g, bool skip_mean)
     {
         const int M = g.jtot, K = g.ktot;
         const int mode = blockIdx.x*blockDim.x + threadIdx.x;
         const int nmodes = nx*M;
         if (mode >= nmodes) return;
         const int kx = mode / M, ky = mode % M;
-        const TC lam = bmati[kx] + bmatj[ky];
-        const size_t ks = static_cast<size_t>(nmodes);
-        const bool mean_mode = (kx == 0 && ky == 0);
-        if (mean_mode && skip_mean)
+        const int l = kx*(M/2 + 1) + (ky <= M/2 ? ky : M - ky);
+        const size_t ks = static_cast<size_t>(nmodes), ls = static_cast<size_t>(nsys);
+        if (kx == 0 && ky == 0 && skip_mean)
         {
             // the mixed mode solves the horizontal mean in FP64 (mean_mode_g)
             for (int k = 0; k < K; ++k) C[k*ks] = cplx<TF>{TF(0), TF(0)};
             return;
         }
-
-        // b of src/pres_2.cxx:306-324: dz*dz*rho*(bmati+bmatj) - (a+c), the
-        // bottom Neumann row and the top row (Dirichlet for the mean mode).
-        auto bcoef = [&](int k) -> TC {
-            TC b = dz2rho[k]*lam - ac[k];
-            if (k == 0) b += a[0];
-            if (k == K-1) { if (mean_mode) b -= c[K-1]; else b += c[K-1]; }
-            return b;
-        };
-
         // forward elimination (src/pres_2.cxx:200-252)
-        // one reciprocal of the pivot per level instead of three divisions
-        TC wi = TC(1)/bcoef(0);
         TC px, py;
         {
             const cplx<TF> q = C[mode];
-            px = dz2[0]*TC(q.x)*wi; py = dz2[0]*TC(q.y)*wi;
+            const TC w0 = wi[l];
+            px = dz2[0]*TC(q.x)*w0; py = dz2[0]*TC(q.y)*w0;
             C[mode] = cplx<TF>{TF(px), TF(py)};
         }
         for (int k = 1; k < K; ++k)
         {
             const cplx<TF> q = C[k*ks + mode];
-            const TC gk = c[k-1]*wi;
-            gam[k*ks + mode] = gk;
-            wi = TC(1)/(bcoef(k) - a[k]*gk);
+            const TC wk = wi[k*ls + l];
             TC qx = dz2[k]*TC(q.x), qy = dz2[k]*TC(q.y);
             qx -= a[k]*px; qy -= a[k]*py;
-            qx *= wi; qy *= wi;
+            qx *= wk; qy *= wk;
             C[k*ks + mode] = cplx<TF>{TF(qx), TF(qy)};
             px = qx; py = qy;
         }
@@ -669,7 +657,7 @@ namespace
         for (int k = K-2; k >= 0; --k)
         {
             const cplx<TF> q = C[k*ks + mode];
-            const TC gk1 = gam[(k+1)*ks + mode];
+            const TC gk1 = gam[(k+1)*ls + l];
             TC qx = TC(q.x), qy = TC(q.y);
             qx -= gk1*px; qy -= gk1*py;
             C[k*ks + mode] = cplx<TF>{TF(qx), TF(qy)};
@@ -1007,6 +995,39 @@ namespace
         MHH_CHECK(up(&t.ac, acv));
         MHH_CHECK(up(&t.dz2, dz2v));
         MHH_CHECK(up(&t.dz2rho, dz2rhov));
+        // Elimination of every distinct system (src/pres_2.cxx:200-252, 306-324):
+        // b = dz*dz*rho*lambda - (a + c), the bottom Neumann row, the top row
+        // Dirichlet for the mean mode (l = 0) and Neumann otherwise.
+        {
+            const int NY = M/2 + 1;
+            t.nsys = (N/2 + 1)*NY;
+            const size_t ns = static_cast<size_t>(t.nsys);
+            std::vector<TF> twi(ns*K), tgam(ns*K, TF(0));
+            for (int kx = 0; kx <= N/2; ++kx)
+                for (int ky = 0; ky < NY; ++ky)
+                {
+                    const size_t l = static_cast<size_t>(kx)*NY + ky;
+                    const TF lam = bi[kx] + bj[ky];
+                    const bool mean_mode = kx == 0 && ky == 0;
+                    auto bcoef = [&](int k) -> TF {
+                        TF b = dz2rhov[k]*lam - acv[k];
+                        if (k == 0) b += av[0];
+                        if (k == K-1) { if (mean_mode) b -= cv[K-1]; else b += cv[K-1]; }
+                        return b;
+                    };
+                    TF w = TF(1)/bcoef(0);
+                    twi[l] = w;
+                    for (int k = 1; k < K; ++k)
+                    {
+                        const TF gk = cv[k-1]*w;
+                        tgam[k*ns + l] = gk;
+                        w = TF(1)/(bcoef(k) - av[k]*gk);
+                        twi[k*ns + l] = w;
+                    }
+                }
+            MHH_CHECK(up(&t.tri_wi, twi));
+            MHH_CHECK(up(&t.tri_gam, tgam));
+        }
         // Twiddles exp(-2 pi i t/n) in long double, rounded once.
         auto roots = [](double** d, int n) -> int {
             std::vector<double> h(2*static_cast<size_t>(n));
@@ -1037,6 +1058,8 @@ namespace
         if (t.dz2rho) (void)hipFree(t.dz2rho);
         if (t.twx) (void)hipFree(t.twx);
         if (t.twy) (void)hipFree(t.twy);
+        if (t.tri_wi) (void)hipFree(t.tri_wi);
+        if (t.tri_gam) (void)hipFree(t.tri_gam);
         t = Pres_tables<TF>();
     }
 
@@ -1143,7 +1166,6 @@ namespace
 
         TF* P = static_cast<TF*>(c->f.p);
         T* C = static_cast<T*>(c->f.tmp1);
-        TF* gam = static_cast<TF*>(c->f.tmp2);
         const TF dxi = TF(1.)/TF(g.dx), dyi = TF(1.)/TF(g.dy), dti = TF(1.)/TF(dt);
 
         // 1. divergence into the dense array P (stored in p); without the mixed
@@ -1203,14 +1225,14 @@ namespace
             {
                 const Pres_tables<double>& d = c->td;
                 mhh::Timer _t(c, "tdma");
-                tdma_g<TF, double><<<(nmodes + 255)/256, 256, 0, c->stream>>>(C, reinterpret_cast<double*>(c->f.tmp2),
-                        d.bmati, d.bmatj, d.a, d.c, d.ac, d.dz2, d.dz2rho, nx, g, true);
+                tdma_g<TF, double><<<(nmodes + 255)/256, 256, 0, c->stream>>>(C, d.tri_wi, d.tri_gam, d.a, d.dz2,
+                        nx, d.nsys, g, true);
             }
             else
             {
                 mhh::Timer _t(c, "tdma");
-                tdma_g<TF, TF><<<(nmodes + 255)/256, 256, 0, c->stream>>>(C, gam, t.bmati, t.bmatj, t.a, t.c,
-                        t.ac, t.dz2, t.dz2rho, nx, g, false);
+                tdma_g<TF, TF><<<(nmodes + 255)/256, 256, 0, c->stream>>>(C, t.tri_wi, t.tri_gam, t.a, t.dz2,
+                        nx, t.nsys, g, false);
             }
             MHH_CHECK(check_launch("tdma"));
         }
