@@ -53,6 +53,9 @@ PASSES = {
     "evisc": 5, "diff_u": 6, "diff_v": 6, "diff_w": 6, "diff_c": 4,
     "pres_in": 7, "pres_in_fft_x": 7, "fft_x_fwd": 2, "fft_y_fwd": 2, "tdma": 2, "fft_y_bwd": 2, "fft_x_bwd": 2,
     "pres_out": 7, "pres_out_rk3": 17, "rk3": 4, "buoyancy": 3,
+    # y transform + forward elimination, back substitution + inverse y
+    # transform (fused): spectrum in and out = 2 each
+    "ytri_fwd": 2, "ytri_bwd": 2,
 }
 KERNELS = list(PASSES) + ["cyclic", "ghost_cells", "mirror_walls", "rk3_zero_ghosts", "cfl", "dnmul", "pres_mean"]
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
@@ -142,7 +145,7 @@ def kernel_table(L, abi, ctx, npts, esize, itot):
                      "ms_mean": round(tot.value / cnt.value, 4)}
             if k in PASSES:
                 nbytes = PASSES[k] * npts * esize
-                if k.startswith("fft") or k == "tdma":
+                if k.startswith("fft") or k == "tdma" or k.startswith("ytri"):
                     nbytes = int(nbytes * (itot + 2) / itot)
                 entry["alg_bytes"] = nbytes
                 entry["gbs"] = round(nbytes / (tot.value / cnt.value * 1e-3) / 1e9, 1)

@@ -224,6 +224,7 @@ int mhh_ctx_create(const mhh_grid_desc* g, int precision, void* hip_stream, mhh_
     // levels per block of the fused tendency kernel (a tuning choice, same results)
     if (const char* e = std::getenv("MHH_TEND_KCHUNK")) c->tend_kchunk = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("MHH_FFT_GENERIC")) c->fft_generic = e[0] == '1';
+    if (const char* e = std::getenv("MHH_YTRI_SPLIT")) c->ytri_split = e[0] == '1';
     c->precision = c->mixed ? MHH_FP32 : precision;
     c->stream = static_cast<hipStream_t>(hip_stream);
     (void)hipGetDevice(&c->device);

@@ -99,6 +99,9 @@ struct mhh_ctx
     // Spectral transforms through the general LDS engine only (no kernels
     // specialised for n = 128, 256, 512): a check of the specialised ones.
     bool fft_generic = false;
+    // y transforms and tridiagonal sweep as separate passes even where the
+    // fused ones apply (a check of the fused kernels).
+    bool ytri_split = false;
     // Scalars advected with the Koren flux limiter ([advec] fluxlimit_list).
     int fluxlimit[MHH_MAX_SCALARS] = {};
     // Reduction scratch: per-block partial maxima and one result slot.

@@ -665,6 +665,148 @@ namespace
         }
     }
 
+    // y transform and tridiagonal sweep fused, for jtot = 64 R.  A block of R
+    // waves owns one kx and runs through the levels in groups of R: each wave
+    // transforms one level's y row (the wave FFT) into its LDS row, then each
+    // thread, one per ky, takes the group's R levels of its mode through the
+    // forward elimination (precomputed per system, see tdma_g) and stores them.
+    // The spectrum is read and written once instead of twice.
+    template<typename TF, typename TC, int R>
+    __global__ void __launch_bounds__(64*R) ytri_fwd_g(cplx<TF>* __restrict__ C, const double2* __restrict__ tw,
+                                                       const TC* __restrict__ wi, const TC* __restrict__ a,
+                                                       const TC* __restrict__ dz2, int nx, int nsys, Gd g,
+                                                       bool skip_mean)
+    {
+        using T = cplx<TF>;
+        constexpr int n = 64*R, P_ = wave_passes<R>();
+        __shared__ T rows[R][wave_row_elems<R>()];
+        const int lane = threadIdx.x, wv = threadIdx.y, ky = lane + 64*wv;
+        const int kx = blockIdx.x, K = g.ktot;
+        const size_t ks = static_cast<size_t>(nx)*n, ls = static_cast<size_t>(nsys);
+        const int l = kx*(n/2 + 1) + (ky <= n/2 ? ky : n - ky);
+        const bool zero_mode = skip_mean && kx == 0 && ky == 0;   // mixed mode: the mean is solved in FP64
+        T* A = rows[wv];
+        T* __restrict__ Ck = C + static_cast<size_t>(kx)*n;      // C[k][kx][.] = Ck[k*ks + .]
+        T wtw[P_-1][R-1];
+        wave_twiddles<TF, R, -1>(wtw, tw, lane);
+
+        T cur[R];
+        #pragma unroll
+        for (int q = 0; q < R; ++q) cur[q] = Ck[min(wv, K-1)*ks + lane + 64*q];
+        TC px = TC(0), py = TC(0);
+        for (int k0 = 0; k0 < K; k0 += R)
+        {
+            T nxt[R];
+            const size_t kn = static_cast<size_t>(min(k0 + R + wv, K-1));
+            #pragma unroll
+            for (int q = 0; q < R; ++q) nxt[q] = Ck[kn*ks + lane + 64*q];
+            // the y transform of level k0 + wv (every wave transforms; rows
+            // past the top are clamped copies, never used).  FP32: the row
+            // minus its FP64 mean, bin 0 set to the FP64 sum afterwards, as in
+            // fft_y_wave_g.
+            constexpr bool demean = sizeof(TF) == 4;
+            double sx = 0., sy = 0.;
+            if (demean)
+            {
+                #pragma unroll
+                for (int q = 0; q < R; ++q) { sx += double(cur[q].x); sy += double(cur[q].y); }
+                #pragma unroll
+                for (int o = 32; o > 0; o >>= 1) { sx += __shfl_xor(sx, o); sy += __shfl_xor(sy, o); }
+                const TF mx = TF(sx/n), my = TF(sy/n);
+                #pragma unroll
+                for (int q = 0; q < R; ++q) { cur[q].x -= mx; cur[q].y -= my; }
+            }
+            wave_fft<TF, R, -1>(cur, A, wtw, lane);
+            if (demean && lane == 0) { cur[0].x = TF(sx); cur[0].y = TF(sy); }
+            #pragma unroll
+            for (int q = 0; q < R; ++q) A[lpad(lane + 64*q)] = cur[q];
+            __syncthreads();
+            // forward elimination of mode (kx, ky) over the group (src/pres_2.cxx:200-252)
+            #pragma unroll
+            for (int m = 0; m < R; ++m)
+            {
+                const int k = k0 + m;
+                if (k >= K) break;
+                const T q = rows[m][lpad(ky)];
+                const TC w = wi[k*ls + l];
+                TC qx = dz2[k]*TC(q.x), qy = dz2[k]*TC(q.y);
+                if (k > 0) { qx -= a[k]*px; qy -= a[k]*py; }
+                qx *= w; qy *= w;
+                if (zero_mode) { qx = TC(0); qy = TC(0); }
+                Ck[k*ks + ky] = T{TF(qx), TF(qy)};
+                px = qx; py = qy;
+            }
+            __syncthreads();
+            #pragma unroll
+            for (int q = 0; q < R; ++q) cur[q] = nxt[q];
+        }
+    }
+
+    // Back substitution and inverse y transform fused (the counterpart of
+    // ytri_fwd_g): from the top down in groups of R levels, each thread takes
+    // its mode through the group's back substitution into the LDS rows, then
+    // each wave inverse-transforms one level's row (dividing by jtot as
+    // src/fft.cxx:420 does) and stores it.
+    template<typename TF, typename TC, int R>
+    __global__ void __launch_bounds__(64*R) ytri_bwd_g(cplx<TF>* __restrict__ C, const double2* __restrict__ tw,
+                                                       const TC* __restrict__ gam, int nx, int nsys, Gd g)
+    {
+        using T = cplx<TF>;
+        constexpr int n = 64*R, P_ = wave_passes<R>();
+        __shared__ T rows[R][wave_row_elems<R>()];
+        const int lane = threadIdx.x, wv = threadIdx.y, ky = lane + 64*wv;
+        const int kx = blockIdx.x, K = g.ktot;
+        const size_t ks = static_cast<size_t>(nx)*n, ls = static_cast<size_t>(nsys);
+        const int l = kx*(n/2 + 1) + (ky <= n/2 ? ky : n - ky);
+        T* A = rows[wv];
+        T* __restrict__ Ck = C + static_cast<size_t>(kx)*n;
+        T wtw[P_-1][R-1];
+        wave_twiddles<TF, R, +1>(wtw, tw, lane);
+
+        // the forward sweep's value at the top level is the solution there
+        TC px = TC(0), py = TC(0);
+        for (int k0 = K-1; k0 >= 0; k0 -= R)
+        {
+            // back substitution (src/pres_2.cxx:254-262) of levels k0, k0-1, ...
+            T qv[R]; TC gv[R];
+            #pragma unroll
+            for (int m = 0; m < R; ++m)
+            {
+                const int k = max(k0 - m, 0);
+                qv[m] = Ck[k*ks + ky];
+                gv[m] = gam[min(k + 1, K-1)*ls + l];
+            }
+            #pragma unroll
+            for (int m = 0; m < R; ++m)
+            {
+                const int k = k0 - m;
+                if (k < 0) break;
+                TC qx = TC(qv[m].x), qy = TC(qv[m].y);
+                if (k < K-1) { qx -= gv[m]*px; qy -= gv[m]*py; }
+                rows[m][lpad(ky)] = T{TF(qx), TF(qy)};
+                px = qx; py = qy;
+            }
+            __syncthreads();
+            const int k = k0 - wv;
+            T v[R];
+            #pragma unroll
+            for (int q = 0; q < R; ++q) v[q] = A[lpad(lane + 64*q)];
+            wave_lds_sync();
+            wave_fft<TF, R, +1>(v, A, wtw, lane);
+            if (k >= 0)
+            {
+                #pragma unroll
+                for (int q = 0; q < R; ++q)
+                {
+                    T z = v[q];
+                    z.x *= TF(1)/TF(n); z.y *= TF(1)/TF(n);
+                    Ck[static_cast<size_t>(k)*ks + lane + 64*q] = z;
+                }
+            }
+            __syncthreads();
+        }
+    }
+
     // Inverse x transform and scatter into the ghosted p (Pres_2::solve tail,
     // src/pres_2.cxx:336-361): interior, the periodic halos of the rows the
     // block owns, and the bottom ghost level p[kstart-1] = p[kstart].
@@ -1094,6 +1236,29 @@ namespace
         return check_launch(name);
     }
 
+    // Fused y transforms and tridiagonal sweep (jtot = 64 R, R = 2, 4, 8).
+    template<typename TF, typename TC, int R>
+    void launch_ytri(mhh_ctx* c, cplx<TF>* C, const double2* twy, const Pres_tables<TC>& d, int nx, bool skip_mean)
+    {
+        const dim3 blk(64, R);
+        { mhh::Timer _t(c, "ytri_fwd"); ytri_fwd_g<TF, TC, R><<<nx, blk, 0, c->stream>>>(C, twy, d.tri_wi, d.a, d.dz2,
+                nx, d.nsys, c->gd, skip_mean); }
+        { mhh::Timer _t(c, "ytri_bwd"); ytri_bwd_g<TF, TC, R><<<nx, blk, 0, c->stream>>>(C, twy, d.tri_gam,
+                nx, d.nsys, c->gd); }
+    }
+
+    template<typename TF, typename TC>
+    int ytri(mhh_ctx* c, cplx<TF>* C, const double2* twy, const Pres_tables<TC>& d, int nx, bool skip_mean)
+    {
+        switch (c->gd.jtot)
+        {
+            case 512: launch_ytri<TF, TC, 8>(c, C, twy, d, nx, skip_mean); break;
+            case 256: launch_ytri<TF, TC, 4>(c, C, twy, d, nx, skip_mean); break;
+            default:  launch_ytri<TF, TC, 2>(c, C, twy, d, nx, skip_mean); break;
+        }
+        return check_launch("ytri");
+    }
+
     // Forward and inverse x transforms through the wave kernels (itot = 64 R,
     // R = 2, 4, 8; jtot a multiple of 8).
     template<typename TF, int R>
@@ -1215,8 +1380,24 @@ namespace
                     c->mixed ? c->pm_mean : nullptr, pin, fuse_in); }
             MHH_CHECK(check_launch("fft_x_fwd"));
         }
-        // 3. forward y transform
         const int nrows = nx*K;
+        const bool ywave = !c->fft_generic && !c->ytri_split && (M == 128 || M == 256 || M == 512);
+        if (ywave)
+        {
+            // 3-5. y transform, tridiagonal sweep, inverse y transform: two
+            // fused passes
+            const double2* twy = reinterpret_cast<const double2*>(t.twy);
+            if (c->mixed)
+            {
+                const Pres_tables<double>& d = c->td;
+                MHH_CHECK((ytri<TF, double>(c, C, twy, d, nx, true)));
+            }
+            else
+                MHH_CHECK((ytri<TF, TF>(c, C, twy, t, nx, false)));
+        }
+        else
+        {
+        // 3. forward y transform
         MHH_CHECK(fft_y<TF>(c, C, py, t.twy, -1, nrows, "fft_y_fwd"));
         // 4. tridiagonal per mode
         {
@@ -1238,6 +1419,7 @@ namespace
         }
         // 5. inverse y transform (divides by jtot)
         MHH_CHECK(fft_y<TF>(c, C, py, t.twy, +1, nrows, "fft_y_bwd"));
+        }
         // 6. inverse x transform (divides by itot) into the ghosted p with halos
         if (xwave)
             MHH_CHECK(fft_x_bwd_wave<TF>(c, C, P, t.twx));
