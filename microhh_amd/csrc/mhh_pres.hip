@@ -691,15 +691,22 @@ namespace
         wave_twiddles<TF, R, -1>(wtw, tw, lane);
 
         T cur[R];
+        TC wcur[R];
         #pragma unroll
         for (int q = 0; q < R; ++q) cur[q] = Ck[min(wv, K-1)*ks + lane + 64*q];
+        #pragma unroll
+        for (int m = 0; m < R; ++m) wcur[m] = wi[min(m, K-1)*ls + l];
         TC px = TC(0), py = TC(0);
         for (int k0 = 0; k0 < K; k0 += R)
         {
+            // the next group's row and pivots load while this group is solved
             T nxt[R];
+            TC wnxt[R];
             const size_t kn = static_cast<size_t>(min(k0 + R + wv, K-1));
             #pragma unroll
             for (int q = 0; q < R; ++q) nxt[q] = Ck[kn*ks + lane + 64*q];
+            #pragma unroll
+            for (int m = 0; m < R; ++m) wnxt[m] = wi[min(k0 + R + m, K-1)*ls + l];
             // the y transform of level k0 + wv (every wave transforms; rows
             // past the top are clamped copies, never used).  FP32: the row
             // minus its FP64 mean, bin 0 set to the FP64 sum afterwards, as in
@@ -728,7 +735,7 @@ namespace
                 const int k = k0 + m;
                 if (k >= K) break;
                 const T q = rows[m][lpad(ky)];
-                const TC w = wi[k*ls + l];
+                const TC w = wcur[m];
                 TC qx = dz2[k]*TC(q.x), qy = dz2[k]*TC(q.y);
                 if (k > 0) { qx -= a[k]*px; qy -= a[k]*py; }
                 qx *= w; qy *= w;
@@ -739,6 +746,8 @@ namespace
             __syncthreads();
             #pragma unroll
             for (int q = 0; q < R; ++q) cur[q] = nxt[q];
+            #pragma unroll
+            for (int m = 0; m < R; ++m) wcur[m] = wnxt[m];
         }
     }
 
@@ -765,17 +774,24 @@ namespace
 
         // the forward sweep's value at the top level is the solution there
         TC px = TC(0), py = TC(0);
-        for (int k0 = K-1; k0 >= 0; k0 -= R)
-        {
-            // back substitution (src/pres_2.cxx:254-262) of levels k0, k0-1, ...
-            T qv[R]; TC gv[R];
+        T qv[R]; TC gv[R];
+        auto load_group = [&](T (&qd)[R], TC (&gd)[R], int kt) {
             #pragma unroll
             for (int m = 0; m < R; ++m)
             {
-                const int k = max(k0 - m, 0);
-                qv[m] = Ck[k*ks + ky];
-                gv[m] = gam[min(k + 1, K-1)*ls + l];
+                const int k = max(kt - m, 0);
+                qd[m] = Ck[k*ks + ky];
+                gd[m] = gam[min(k + 1, K-1)*ls + l];
             }
+        };
+        load_group(qv, gv, K-1);
+        for (int k0 = K-1; k0 >= 0; k0 -= R)
+        {
+            // the next group's values load while this one is solved: the
+            // levels below k0 - R + 1 are not written before their group
+            T qn[R]; TC gn[R];
+            load_group(qn, gn, k0 - R);
+            // back substitution (src/pres_2.cxx:254-262) of levels k0, k0-1, ...
             #pragma unroll
             for (int m = 0; m < R; ++m)
             {
@@ -804,6 +820,8 @@ namespace
                 }
             }
             __syncthreads();
+            #pragma unroll
+            for (int m = 0; m < R; ++m) { qv[m] = qn[m]; gv[m] = gn[m]; }
         }
     }
 
