@@ -387,8 +387,13 @@ namespace
     // FP64 sum of the right-hand side into part[k*gridDim.x + blockIdx.x]
     // (mixed mode: the plane means, whose mode the FP64 path solves).
     // Writes C[k][kx][j].
+    // Waves per block of the x wave kernels: each wave owns two rows, so a
+    // block writes (reads) 2*XW consecutive j of every kx in C, a 128-byte
+    // line in either precision.
+    template<typename TF> constexpr int xwaves() { return sizeof(TF) == 8 ? 4 : 8; }
+
     template<typename TF, int R, int MODE>
-    __global__ void __launch_bounds__(256) fft_x_fwd_wave_g(
+    __global__ void __launch_bounds__(64*xwaves<TF>()) fft_x_fwd_wave_g(
             const TF* __restrict__ P, cplx<TF>* __restrict__ C, const double2* __restrict__ tw, Gd g,
             const double* __restrict__ pmean, double* __restrict__ part,
             const TF* __restrict__ u, const TF* __restrict__ v, const TF* __restrict__ w,
@@ -397,10 +402,11 @@ namespace
             TF dxi, TF dyi, TF dti)
     {
         using T = cplx<TF>;
-        constexpr int n = 64*R, nx = n/2 + 1, JB = 8, P_ = wave_passes<R>();
+        constexpr int XW = xwaves<TF>(), NT = 64*XW;
+        constexpr int n = 64*R, nx = n/2 + 1, JB = 2*XW, P_ = wave_passes<R>();
         constexpr int NQ = (nx + 63)/64;
-        __shared__ T rows[4][wave_row_elems<R>()];
-        __shared__ double wsum[4];
+        __shared__ T rows[XW][wave_row_elems<R>()];
+        __shared__ double wsum[XW];
         const int lane = threadIdx.x, wv = threadIdx.y, tid = lane + 64*wv;
         const int M = g.jtot, k = blockIdx.y, bj = blockIdx.x, nbj = gridDim.x;
         const int j0 = bj*JB, ja = j0 + 2*wv, jb = ja + 1;
@@ -487,10 +493,15 @@ namespace
         }
         __syncthreads();
         if (MODE == 3 && tid == 0)
-            part[static_cast<size_t>(k)*nbj + bj] = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
-        // C[k][kx][j0 .. j0+8): consecutive threads on consecutive j
+        {
+            double sm = 0.;
+            #pragma unroll
+            for (int q = 0; q < XW; ++q) sm += wsum[q];
+            part[static_cast<size_t>(k)*nbj + bj] = sm;
+        }
+        // C[k][kx][j0 .. j0+JB): consecutive threads on consecutive j
         T* Ck = C + static_cast<size_t>(k)*nx*M;
-        for (int idx = tid; idx < nx*JB; idx += 256)
+        for (int idx = tid; idx < nx*JB; idx += NT)
         {
             const int kx = idx / JB, jl = idx % JB;
             const int r = jl >> 1, part = jl & 1;
@@ -511,12 +522,14 @@ namespace
     // into the ghosted p: interior rows, their periodic images in the y halo,
     // the x halo columns and, at the lowest level, the ghost level below.
     template<typename TF, int R>
-    __global__ void __launch_bounds__(256) fft_x_bwd_wave_g(const cplx<TF>* __restrict__ C, TF* __restrict__ p,
-                                                            const double2* __restrict__ tw, Gd g)
+    __global__ void __launch_bounds__(64*xwaves<TF>()) fft_x_bwd_wave_g(const cplx<TF>* __restrict__ C,
+                                                                         TF* __restrict__ p,
+                                                                         const double2* __restrict__ tw, Gd g)
     {
         using T = cplx<TF>;
-        constexpr int n = 64*R, nx = n/2 + 1, JB = 8, P_ = wave_passes<R>();
-        __shared__ T rows[4][wave_row_elems<R>()];
+        constexpr int XW = xwaves<TF>(), NT = 64*XW;
+        constexpr int n = 64*R, nx = n/2 + 1, JB = 2*XW, P_ = wave_passes<R>();
+        __shared__ T rows[XW][wave_row_elems<R>()];
         const int lane = threadIdx.x, wv = threadIdx.y, tid = lane + 64*wv;
         const int M = g.jtot, k = blockIdx.y;
         const int j0 = blockIdx.x*JB;
@@ -526,7 +539,7 @@ namespace
 
         // stage the block's spectra packed as in the forward kernel
         const T* Ck = C + static_cast<size_t>(k)*nx*M;
-        for (int idx = tid; idx < nx*JB; idx += 256)
+        for (int idx = tid; idx < nx*JB; idx += NT)
         {
             const int kx = idx / JB, jl = idx % JB;
             const int r = jl >> 1, part = jl & 1;
@@ -1284,7 +1297,7 @@ namespace
                                const Prof<TF>& p, TF dxi, TF dyi, TF dti)
     {
         const Gd& g = c->gd;
-        const dim3 grd(g.jtot/8, g.ktot), blk(64, 4);
+        const dim3 grd(g.jtot/(2*xwaves<TF>()), g.ktot), blk(64, xwaves<TF>());
         const TF* u = static_cast<const TF*>(c->f.u); const TF* v = static_cast<const TF*>(c->f.v);
         const TF* w = static_cast<const TF*>(c->f.w); const TF* ut = static_cast<const TF*>(c->f.ut);
         const TF* vt = static_cast<const TF*>(c->f.vt); const TF* wt = static_cast<const TF*>(c->f.wt);
@@ -1319,7 +1332,7 @@ namespace
     {
         const Gd& g = c->gd;
         const double2* t2 = reinterpret_cast<const double2*>(tw);
-        const dim3 grd(g.jtot/8, g.ktot), blk(64, 4);
+        const dim3 grd(g.jtot/(2*xwaves<TF>()), g.ktot), blk(64, xwaves<TF>());
         {
             mhh::Timer _t(c, "fft_x_bwd");
             switch (g.itot)
@@ -1356,7 +1369,8 @@ namespace
         Pin_args<TF> pin{static_cast<const TF*>(c->f.u), static_cast<const TF*>(c->f.v), static_cast<const TF*>(c->f.w),
                          static_cast<const TF*>(c->f.ut), static_cast<const TF*>(c->f.vt), static_cast<const TF*>(c->f.wt),
                          p.dzi, p.rhoref, p.rhorefh, dxi, dyi, dti};
-        const bool xwave = !c->fft_generic && (N == 128 || N == 256 || N == 512) && M % 8 == 0 && g.igc <= 32;
+        const bool xwave = !c->fft_generic && (N == 128 || N == 256 || N == 512) && M % (2*xwaves<TF>()) == 0
+                        && g.igc <= 32;
         // The wave x transform evaluates the right-hand side itself (and, in
         // the mixed mode, the FP64 plane sums); the general engine does so
         // except in the mixed mode, whose plane means come first.
@@ -1384,7 +1398,7 @@ namespace
             MHH_CHECK(fft_x_fwd_wave<TF>(c, P, C, t.twx, mode, p, dxi, dyi, dti, "pres_in_fft_x"));
             if (c->mixed)
             {
-                { mhh::Timer _t(c, "pres_mean"); mean_mode_g<<<1, 256, mm_lds, c->stream>>>(c->pm_part, M/8, c->pm_mean,
+                { mhh::Timer _t(c, "pres_mean"); mean_mode_g<<<1, 256, mm_lds, c->stream>>>(c->pm_part, M/(2*xwaves<TF>()), c->pm_mean,
                         c->pbar, c->dpbar, d.a, d.dz2, c->pm_coef, 1./(double(N)*double(M)), g); }
                 MHH_CHECK(check_launch("mean_mode"));
             }
