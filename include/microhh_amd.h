@@ -224,6 +224,9 @@ int  mhh_device_synchronize(void);
  * events; mhh_timing_query returns the summed milliseconds and the launch count
  * since the last reset (it synchronises the stream). */
 int  mhh_timing_enable(mhh_ctx* ctx, int on);
+/* Restrict the timing to one kernel family (NULL or "": every family), so a
+ * timed run carries events around that kernel only. */
+int  mhh_timing_select(mhh_ctx* ctx, const char* kernel);
 int  mhh_timing_reset(mhh_ctx* ctx);
 int  mhh_timing_query(mhh_ctx* ctx, const char* kernel, double* total_ms, int* count);
 

@@ -90,6 +90,7 @@ SIGNATURES = {
     "mhh_copy_device": (C.c_int, [VP, VP, C.c_size_t]),
     "mhh_device_synchronize": (C.c_int, []),
     "mhh_timing_enable": (C.c_int, [VP, C.c_int]),
+    "mhh_timing_select": (C.c_int, [VP, C.c_char_p]),
     "mhh_timing_reset": (C.c_int, [VP]),
     "mhh_timing_query": (C.c_int, [VP, C.c_char_p, DP, C.POINTER(C.c_int)]),
 }

@@ -129,6 +129,7 @@ namespace mhh
     Timer::Timer(mhh_ctx* ctx, const char* name) : c(ctx)
     {
         if (!c || !c->timing) return;
+        if (!c->timing_only.empty() && c->timing_only != name) return;
         auto& pool = c->timers[name];
         if (pool.used == static_cast<int>(pool.ev.size()))
         {
@@ -378,6 +379,13 @@ int mhh_timing_enable(mhh_ctx* c, int on)
 {
     if (!c) return mhh::fail(MHH_EINVAL, "null context");
     c->timing = on != 0;
+    return 0;
+}
+
+int mhh_timing_select(mhh_ctx* c, const char* kernel)
+{
+    if (!c) return mhh::fail(MHH_EINVAL, "null context");
+    c->timing_only = kernel ? kernel : "";
     return 0;
 }
 

@@ -116,6 +116,7 @@ struct mhh_ctx
     // Kernel timing: per kernel name, a pool of (start, stop) event pairs and
     // the number recorded since the last reset.
     bool timing = false;
+    std::string timing_only;   // non-empty: time only the kernel family of this name
     struct Event_pool { std::vector<std::pair<hipEvent_t, hipEvent_t>> ev; int used = 0; };
     std::map<std::string, Event_pool> timers;
 };
