@@ -24,19 +24,24 @@ def _inputs(n, seed):
     return make_inputs(dict(itot=n, jtot=n, ktot=6, stretch=True, anelastic=True, dx=50.0, dy=40.0, seed=seed))
 
 
-def _gpu_pres(p, i, dt, generic, precision=None):
+def _gpu_pres(p, i, dt, generic, precision=None, split=False):
+    """The projection on the GPU; generic: the general transform engine only;
+    split: y transforms and tridiagonal sweep as three passes instead of the
+    two fused ones (both switches are read when the context is created)."""
     from microhh_amd import abi
     from tests._gpu import GpuCase
-    old = os.environ.get("MHH_FFT_GENERIC")
-    os.environ["MHH_FFT_GENERIC"] = "1" if generic else "0"
+    env = {"MHH_FFT_GENERIC": "1" if generic else "0", "MHH_YTRI_SPLIT": "1" if split else "0"}
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
     try:
         g = GpuCase(p["itot"], p["jtot"], p["ktot"], p["igc"], p["jgc"], p["kgc"], p["dx"], p["dy"], i,
                     nscalars=1, precision=abi.MHH_FP64 if precision is None else precision)
     finally:
-        if old is None:
-            del os.environ["MHH_FFT_GENERIC"]
-        else:
-            os.environ["MHH_FFT_GENERIC"] = old
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
     try:
         for n in ("u", "v", "w", "ut", "vt", "wt"):
             g.put(n, i[n])
@@ -68,11 +73,12 @@ def _oracle_pres(p, i, uvw, dt):
     return d, {"ut": ut, "vt": vt, "wt": wt, "p": pf}
 
 
+@pytest.mark.parametrize("split", [False, True])
 @pytest.mark.parametrize("n", [128, 256, 512])
-def test_pres_specialised_lengths(n):
+def test_pres_specialised_lengths(n, split):
     p, i = _inputs(n, seed=31 + n)
     dt = 0.41
-    out, uvw = _gpu_pres(p, i, dt, generic=False)
+    out, uvw = _gpu_pres(p, i, dt, generic=False, split=split)
     d, ref = _oracle_pres(p, i, uvw, dt)
     m = interior(d)
     for name in ("ut", "vt", "wt"):
