@@ -45,6 +45,20 @@ namespace mhh
     template<typename TF>
     __device__ __forceinline__ TF hsum(Hterms<TF> x, Hterms<TF> y) { return ((x.c + x.u) + y.c) + y.u; }
 
+    // The same horizontal 2i5 flux divergence as one value, with the face
+    // velocities given times a factor that di carries back (velocities
+    // interpolated to the face are passed as the sum of the two values with
+    // di/2, which saves the multiplication of the interpolation).
+    template<typename TF>
+    __device__ __forceinline__ TF hflux_2i5_one(TF fp, TF fm, const TF* __restrict__ s, int ijk, int st, TF di)
+    {
+        const TF sm3 = s[ijk-3*st], sm2 = s[ijk-2*st], sm1 = s[ijk-st], s0 = s[ijk], sp1 = s[ijk+st],
+                 sp2 = s[ijk+2*st], sp3 = s[ijk+3*st];
+        const TF c = fm * i6ws(sm3, sm2, sm1, s0, sp1, sp2) - fp * i6ws(sm2, sm1, s0, sp1, sp2, sp3);
+        const TF u = fabs(fp) * i5ws(sm2, sm1, s0, sp1, sp2, sp3) - fabs(fm) * i5ws(sm3, sm2, sm1, s0, sp1, sp2);
+        return (c + u) * di;
+    }
+
     // Level classes of the 2i5 vertical stencil for cell-centred fields:
     // 0=kstart, 1=kstart+1, 2=kstart+2, 3=interior, 4=kend-3, 5=kend-2, 6=kend-1
     // (src/advec_2i5.cxx:215-299).

@@ -118,41 +118,43 @@ namespace
     // face (src/advec_2i5.cxx:215-299): zero at the walls, 2nd order next to
     // them, 4th/3rd order one further, 6th/5th order inside.  Both cells that
     // share a face use the same products, so each face is computed once.
+    // rw is the face mass flux rf*wf (rf > 0, so |rw| = rf*|wf|).
     template<typename TF>
-    __device__ __forceinline__ void vface_c(int f, int ks, int ke, TF rf, TF wf,
+    __device__ __forceinline__ void vface_c(int f, int ks, int ke, TF rw,
                                             TF s0, TF s1, TF s2, TF s3, TF s4, TF s5, TF& fc, TF& fu)
     {
         if (f == ks || f == ke) { fc = TF(0); fu = TF(0); }
-        else if (f == ks+1 || f == ke-1) { fc = rf*wf*i2(s2, s3); fu = TF(0); }
+        else if (f == ks+1 || f == ke-1) { fc = rw*i2(s2, s3); fu = TF(0); }
         else if (f == ks+2 || f == ke-2)
         {
-            fc = rf*wf*i4ws(s1, s2, s3, s4);
-            fu = rf*fabs(wf)*i3ws(s1, s2, s3, s4);
+            fc = rw*i4ws(s1, s2, s3, s4);
+            fu = fabs(rw)*i3ws(s1, s2, s3, s4);
         }
         else
         {
-            fc = rf*wf*i6ws(s0, s1, s2, s3, s4, s5);
-            fu = rf*fabs(wf)*i5ws(s0, s1, s2, s3, s4, s5);
+            fc = rw*i6ws(s0, s1, s2, s3, s4, s5);
+            fu = fabs(rw)*i5ws(s0, s1, s2, s3, s4, s5);
         }
     }
 
     // The same for w at cell-centre level c (the face between w levels c and
     // c+1), w0..w5 = w[c-2 .. c+3] (src/advec_2i5.cxx:501-578).
+    // rfh is half the cell density (the 1/2 of the velocity interpolation).
     template<typename TF>
-    __device__ __forceinline__ void vface_w(int c, int ks, int ke, TF rf,
+    __device__ __forceinline__ void vface_w(int c, int ks, int ke, TF rfh,
                                             TF w0, TF w1, TF w2, TF w3, TF w4, TF w5, TF& fc, TF& fu)
     {
-        const TF wf = i2(w2, w3);
-        if (c == ks || c == ke-1) { fc = rf*wf*i2(w2, w3); fu = TF(0); }
+        const TF rw = rfh*(w2 + w3);
+        if (c == ks || c == ke-1) { fc = rw*i2(w2, w3); fu = TF(0); }
         else if (c == ks+1 || c == ke-2)
         {
-            fc = rf*wf*i4ws(w1, w2, w3, w4);
-            fu = rf*fabs(wf)*i3ws(w1, w2, w3, w4);
+            fc = rw*i4ws(w1, w2, w3, w4);
+            fu = fabs(rw)*i3ws(w1, w2, w3, w4);
         }
         else
         {
-            fc = rf*wf*i6ws(w0, w1, w2, w3, w4, w5);
-            fu = rf*fabs(wf)*i5ws(w0, w1, w2, w3, w4, w5);
+            fc = rw*i6ws(w0, w1, w2, w3, w4, w5);
+            fu = fabs(rw)*i5ws(w0, w1, w2, w3, w4, w5);
         }
     }
 
@@ -235,13 +237,14 @@ namespace
         __syncthreads();
         {
             const Level<TF> L = lev[kb], Lm = lev[kb-1];
-            vface_c(kb, g.kstart, g.kend, L.rhoh, i2(Wc[c-1], Wc[c]),
+            const TF rhh = TF(0.5)*L.rhoh;
+            vface_c(kb, g.kstart, g.kend, rhh*(Wc[c-1] + Wc[c]),
                     qu[0], qu[1], qu[2], qu[3], qu[4], qu[5], bu_c, bu_u);
-            vface_c(kb, g.kstart, g.kend, L.rhoh, i2(Wc[c-PW], Wc[c]),
+            vface_c(kb, g.kstart, g.kend, rhh*(Wc[c-PW] + Wc[c]),
                     qv[0], qv[1], qv[2], qv[3], qv[4], qv[5], bv_c, bv_u);
-            vface_c(kb, g.kstart, g.kend, L.rhoh, Wc[c],
+            vface_c(kb, g.kstart, g.kend, L.rhoh*Wc[c],
                     qs[0], qs[1], qs[2], qs[3], qs[4], qs[5], bs_c, bs_u);
-            vface_w(kb-1, g.kstart, g.kend, Lm.rho,
+            vface_w(kb-1, g.kstart, g.kend, TF(0.5)*Lm.rho,
                     qw[0], qw[1], qw[2], qw[3], qw[4], qw[5], bw_c, bw_u);
             const TF eu = TF(0.25)*(Ep[ce-1] + Ep[ce] + Ec[ce-1] + Ec[ce]) + visc;
             du_b = L.rhoh * eu*((qu[3]-qu[2])*L.dzhi + (Wc[c]-Wc[c-1])*dxi);
@@ -285,13 +288,16 @@ namespace
             // L.cfac, L.wfac: dzi/rhoref and dzhi/rhorefh, the divisions of the
             // reference's "/ rhoref[k] * dzi[k]" taken out of the per-point work
             const TF cf = L.cfac, wf = L.wfac;
+            // the 1/2 of the face-velocity interpolations, folded into the
+            // grid factors and densities
+            const TF dxih = TF(0.5)*dxi, dyih = TF(0.5)*dyi, rhph = TF(0.5)*L.rhohp;
 
             // -------------------------------------------------------------- u
             {
-                tu += hsum(hflux_2i5(i2(Uc[c], Uc[c+1]), i2(Uc[c-1], Uc[c]), Uc, c, 1, dxi),
-                           hflux_2i5(i2(Vc[c-1+PW], Vc[c+PW]), i2(Vc[c-1], Vc[c]), Uc, c, PW, dyi));
+                tu += hflux_2i5_one(Uc[c] + Uc[c+1], Uc[c-1] + Uc[c], Uc, c, 1, dxih)
+                    + hflux_2i5_one(Vc[c-1+PW] + Vc[c+PW], Vc[c-1] + Vc[c], Uc, c, PW, dyih);
                 TF tc, tq;
-                vface_c(k+1, g.kstart, g.kend, L.rhohp, i2(Wn[c-1], Wn[c]),
+                vface_c(k+1, g.kstart, g.kend, rhph*(Wn[c-1] + Wn[c]),
                         qu[1], qu[2], qu[3], qu[4], qu[5], qu[6], tc, tq);
                 tu += - ( tc - bu_c ) * cf + ( tq - bu_u ) * cf;
                 bu_c = tc; bu_u = tq;
@@ -313,10 +319,10 @@ namespace
             }
             // -------------------------------------------------------------- v
             {
-                tv += hsum(hflux_2i5(i2(Uc[c+1-PW], Uc[c+1]), i2(Uc[c-PW], Uc[c]), Vc, c, 1, dxi),
-                           hflux_2i5(i2(Vc[c], Vc[c+PW]), i2(Vc[c-PW], Vc[c]), Vc, c, PW, dyi));
+                tv += hflux_2i5_one(Uc[c+1-PW] + Uc[c+1], Uc[c-PW] + Uc[c], Vc, c, 1, dxih)
+                    + hflux_2i5_one(Vc[c] + Vc[c+PW], Vc[c-PW] + Vc[c], Vc, c, PW, dyih);
                 TF tc, tq;
-                vface_c(k+1, g.kstart, g.kend, L.rhohp, i2(Wn[c-PW], Wn[c]),
+                vface_c(k+1, g.kstart, g.kend, rhph*(Wn[c-PW] + Wn[c]),
                         qv[1], qv[2], qv[3], qv[4], qv[5], qv[6], tc, tq);
                 tv += - ( tc - bv_c ) * cf + ( tq - bv_u ) * cf;
                 bv_c = tc; bv_u = tq;
@@ -340,14 +346,14 @@ namespace
             {
                 // faces at the cell centres k (top) and k-1 (bottom, carried)
                 TF tc, tq;
-                vface_w(k, g.kstart, g.kend, L.rho, qw[1], qw[2], qw[3], qw[4], qw[5], qw[6], tc, tq);
+                vface_w(k, g.kstart, g.kend, TF(0.5)*L.rho, qw[1], qw[2], qw[3], qw[4], qw[5], qw[6], tc, tq);
                 const TF dt = L.rho * (Ec[ce] + visc)*(qw[4]-qw[3])*L.dzi;
                 if (k > g.kstart)
                 {
                     if (a.buoyancy)
                         tw += L.gthi * (i2(qs[2], qs[3]) - L.threfh_rel);
-                    tw += hsum(hflux_2i5(i2(Up[c+1], Uc[c+1]), i2(Up[c], Uc[c]), Wc, c, 1, dxi),
-                               hflux_2i5(i2(Vp[c+PW], Vc[c+PW]), i2(Vp[c], Vc[c]), Wc, c, PW, dyi));
+                    tw += hflux_2i5_one(Up[c+1] + Uc[c+1], Up[c] + Uc[c], Wc, c, 1, dxih)
+                        + hflux_2i5_one(Vp[c+PW] + Vc[c+PW], Vp[c] + Vc[c], Wc, c, PW, dyih);
                     tw += - ( tc - bw_c ) * wf + ( tq - bw_u ) * wf;
                     const TF evisce = TF(0.25)*(Ep[ce] + Ec[ce] + Ep[ce+1] + Ec[ce+1]) + visc;
                     const TF eviscw = TF(0.25)*(Ep[ce-1] + Ec[ce-1] + Ep[ce] + Ec[ce]) + visc;
@@ -364,10 +370,10 @@ namespace
             }
             // -------------------------------------------------------------- s
             {
-                ts += hsum(hflux_2i5(Uc[c+1], Uc[c], Sc, c, 1, dxi),
-                           hflux_2i5(Vc[c+PW], Vc[c], Sc, c, PW, dyi));
+                ts += hflux_2i5_one(Uc[c+1], Uc[c], Sc, c, 1, dxi)
+                    + hflux_2i5_one(Vc[c+PW], Vc[c], Sc, c, PW, dyi);
                 TF tc, tq;
-                vface_c(k+1, g.kstart, g.kend, L.rhohp, Wn[c],
+                vface_c(k+1, g.kstart, g.kend, L.rhohp*Wn[c],
                         qs[1], qs[2], qs[3], qs[4], qs[5], qs[6], tc, tq);
                 ts += - ( tc - bs_c ) * cf + ( tq - bs_u ) * cf;
                 bs_c = tc; bs_u = tq;
