@@ -234,6 +234,7 @@ namespace
         // takes them from the top face of the level below.
         TF bu_c, bu_u, bv_c, bv_u, bw_c, bw_u, bs_c, bs_u;   // advective
         TF du_b, dv_b, dw_b, ds_b;                          // diffusive
+        TF ew_w, ew_s;                                      // w's west and south edge viscosities
         __syncthreads();
         {
             const Level<TF> L = lev[kb], Lm = lev[kb-1];
@@ -253,6 +254,10 @@ namespace
             dw_b = Lm.rho * (Ep[ce] + visc)*(qw[3]-qw[2])*Lm.dzi;
             const TF es = TF(0.5)*(Ep[ce]+Ec[ce]) * a.tPr_i + a.svisc;
             ds_b = L.rhoh * es*(qs[3]-qs[2])*L.dzhi;
+            // edge viscosities at (i-1/2, k-1/2) and (j-1/2, k-1/2): the top
+            // edges of u and v one level down, carried from level to level
+            ew_w = TF(0.25)*(Ep[ce-1] + Ec[ce-1] + Ep[ce] + Ec[ce]) + visc;
+            ew_s = TF(0.25)*(Ep[ce-EW] + Ec[ce-EW] + Ep[ce] + Ec[ce]) + visc;
         }
         // Every load of the preamble has landed: the waits inside the loop then
         // only count the loads issued in the loop.
@@ -291,6 +296,10 @@ namespace
             // the 1/2 of the face-velocity interpolations, folded into the
             // grid factors and densities
             const TF dxih = TF(0.5)*dxi, dyih = TF(0.5)*dyi, rhph = TF(0.5)*L.rhohp;
+            const TF dxi2 = TF(2.)*a.dxidxi, dyi2 = TF(2.)*a.dyidyi;
+            // the south-west edge viscosity (i-1/2, j-1/2) is u's south and v's west edge
+            const TF evsw = TF(0.25)*(Ec[ce-1-EW] + Ec[ce-EW] + Ec[ce-1] + Ec[ce]) + visc;
+            TF ewn_w, ewn_s;   // this level's top edges of u and v: w's west and south edges one level up
 
             // -------------------------------------------------------------- u
             {
@@ -299,22 +308,21 @@ namespace
                 TF tc, tq;
                 vface_c(k+1, g.kstart, g.kend, rhph*(Wn[c-1] + Wn[c]),
                         qu[1], qu[2], qu[3], qu[4], qu[5], qu[6], tc, tq);
-                tu += - ( tc - bu_c ) * cf + ( tq - bu_u ) * cf;
-                bu_c = tc; bu_u = tq;
                 const TF evisce = Ec[ce] + visc;
                 const TF eviscw = Ec[ce-1] + visc;
                 const TF eviscn = TF(0.25)*(Ec[ce-1] + Ec[ce] + Ec[ce-1+EW] + Ec[ce+EW]) + visc;
-                const TF eviscs = TF(0.25)*(Ec[ce-1-EW] + Ec[ce-EW] + Ec[ce-1] + Ec[ce]) + visc;
+                const TF eviscs = evsw;
                 const TF evisct = TF(0.25)*(Ec[ce-1] + Ec[ce] + En[ce-1] + En[ce]) + visc;
                 const TF hor =
-                         + ( evisce*(Uc[c+1]-Uc[c  ])*dxi
-                           - eviscw*(Uc[c  ]-Uc[c-1])*dxi ) * TF(2.)*dxi
+                         + ( evisce*(Uc[c+1]-Uc[c  ])
+                           - eviscw*(Uc[c  ]-Uc[c-1]) ) * dxi2
                          + ( eviscn*((Uc[c+PW]-Uc[c   ])*dyi + (Vc[c+PW]-Vc[c-1+PW])*dxi)
                            - eviscs*((Uc[c   ]-Uc[c-PW])*dyi + (Vc[c   ]-Vc[c-1   ])*dxi) ) * dyi;
                 const TF dt = L.rhohp * evisct*((qu[4]-qu[3])*L.dzhip + (Wn[c]-Wn[c-1])*dxi);
-                const TF ver = ( dt - du_b ) * cf;
-                du_b = dt;
-                tu += hor + ver;
+                // vertical advection (centred, upwind) and diffusion share the factor
+                tu += hor + ( ( tq - bu_u ) - ( tc - bu_c ) + ( dt - du_b ) ) * cf;
+                bu_c = tc; bu_u = tq; du_b = dt;
+                ewn_w = evisct;
                 if (active) sto(utk, oij, tu);
             }
             // -------------------------------------------------------------- v
@@ -324,22 +332,20 @@ namespace
                 TF tc, tq;
                 vface_c(k+1, g.kstart, g.kend, rhph*(Wn[c-PW] + Wn[c]),
                         qv[1], qv[2], qv[3], qv[4], qv[5], qv[6], tc, tq);
-                tv += - ( tc - bv_c ) * cf + ( tq - bv_u ) * cf;
-                bv_c = tc; bv_u = tq;
                 const TF evisce = TF(0.25)*(Ec[ce-EW] + Ec[ce] + Ec[ce+1-EW] + Ec[ce+1]) + visc;
-                const TF eviscw = TF(0.25)*(Ec[ce-1-EW] + Ec[ce-1] + Ec[ce-EW] + Ec[ce]) + visc;
+                const TF eviscw = evsw;
                 const TF eviscn = Ec[ce] + visc;
                 const TF eviscs = Ec[ce-EW] + visc;
                 const TF evisct = TF(0.25)*(Ec[ce-EW] + Ec[ce] + En[ce-EW] + En[ce]) + visc;
                 const TF hor =
                          + ( evisce*((Vc[c+1]-Vc[c  ])*dxi + (Uc[c+1]-Uc[c+1-PW])*dyi)
                            - eviscw*((Vc[c  ]-Vc[c-1])*dxi + (Uc[c  ]-Uc[c  -PW])*dyi) ) * dxi
-                         + ( eviscn*(Vc[c+PW]-Vc[c   ])*dyi
-                           - eviscs*(Vc[c   ]-Vc[c-PW])*dyi ) * TF(2.)*dyi;
+                         + ( eviscn*(Vc[c+PW]-Vc[c   ])
+                           - eviscs*(Vc[c   ]-Vc[c-PW]) ) * dyi2;
                 const TF dt = L.rhohp * evisct*((qv[4]-qv[3])*L.dzhip + (Wn[c]-Wn[c-PW])*dyi);
-                const TF ver = ( dt - dv_b ) * cf;
-                dv_b = dt;
-                tv += hor + ver;
+                tv += hor + ( ( tq - bv_u ) - ( tc - bv_c ) + ( dt - dv_b ) ) * cf;
+                bv_c = tc; bv_u = tq; dv_b = dt;
+                ewn_s = evisct;
                 if (active) sto(vtk, oij, tv);
             }
             // -------------------------------------------------------------- w
@@ -354,19 +360,19 @@ namespace
                         tw += L.gthi * (i2(qs[2], qs[3]) - L.threfh_rel);
                     tw += hflux_2i5_one(Up[c+1] + Uc[c+1], Up[c] + Uc[c], Wc, c, 1, dxih)
                         + hflux_2i5_one(Vp[c+PW] + Vc[c+PW], Vp[c] + Vc[c], Wc, c, PW, dyih);
-                    tw += - ( tc - bw_c ) * wf + ( tq - bw_u ) * wf;
                     const TF evisce = TF(0.25)*(Ep[ce] + Ec[ce] + Ep[ce+1] + Ec[ce+1]) + visc;
-                    const TF eviscw = TF(0.25)*(Ep[ce-1] + Ec[ce-1] + Ep[ce] + Ec[ce]) + visc;
+                    const TF eviscw = ew_w;
                     const TF eviscn = TF(0.25)*(Ep[ce] + Ec[ce] + Ep[ce+EW] + Ec[ce+EW]) + visc;
-                    const TF eviscs = TF(0.25)*(Ep[ce-EW] + Ec[ce-EW] + Ep[ce] + Ec[ce]) + visc;
+                    const TF eviscs = ew_s;
                     tw += + ( evisce*((Wc[c+1]-Wc[c  ])*dxi + (Uc[c+1]-Up[c+1])*L.dzhi)
                             - eviscw*((Wc[c  ]-Wc[c-1])*dxi + (Uc[c  ]-Up[c  ])*L.dzhi) ) * dxi
                           + ( eviscn*((Wc[c+PW]-Wc[c   ])*dyi + (Vc[c+PW]-Vp[c+PW])*L.dzhi)
                             - eviscs*((Wc[c   ]-Wc[c-PW])*dyi + (Vc[c   ]-Vp[c   ])*L.dzhi) ) * dyi
-                          + ( dt - dw_b ) * TF(2.)*wf;
+                          + ( ( tq - bw_u ) - ( tc - bw_c ) + TF(2.)*( dt - dw_b ) ) * wf;
                     if (active) sto(wtk, oij, tw);
                 }
                 bw_c = tc; bw_u = tq; dw_b = dt;
+                ew_w = ewn_w; ew_s = ewn_s;
             }
             // -------------------------------------------------------------- s
             {
@@ -375,23 +381,21 @@ namespace
                 TF tc, tq;
                 vface_c(k+1, g.kstart, g.kend, L.rhohp*Wn[c],
                         qs[1], qs[2], qs[3], qs[4], qs[5], qs[6], tc, tq);
-                ts += - ( tc - bs_c ) * cf + ( tq - bs_u ) * cf;
-                bs_c = tc; bs_u = tq;
-                const TF tPr_i = a.tPr_i, sv = a.svisc;
-                const TF evisce = TF(0.5)*(Ec[ce   ]+Ec[ce+1 ]) * tPr_i + sv;
-                const TF eviscw = TF(0.5)*(Ec[ce-1 ]+Ec[ce   ]) * tPr_i + sv;
-                const TF eviscn = TF(0.5)*(Ec[ce   ]+Ec[ce+EW]) * tPr_i + sv;
-                const TF eviscs = TF(0.5)*(Ec[ce-EW]+Ec[ce   ]) * tPr_i + sv;
-                const TF evisct = TF(0.5)*(Ec[ce   ]+En[ce   ]) * tPr_i + sv;
+                // 1/(2 tPr): the two-point average and the Prandtl number in one factor
+                const TF htPr = TF(0.5)*a.tPr_i, sv = a.svisc;
+                const TF evisce = (Ec[ce   ]+Ec[ce+1 ]) * htPr + sv;
+                const TF eviscw = (Ec[ce-1 ]+Ec[ce   ]) * htPr + sv;
+                const TF eviscn = (Ec[ce   ]+Ec[ce+EW]) * htPr + sv;
+                const TF eviscs = (Ec[ce-EW]+Ec[ce   ]) * htPr + sv;
+                const TF evisct = (Ec[ce   ]+En[ce   ]) * htPr + sv;
                 const TF hor =
                          + ( evisce*(Sc[c+1]-Sc[c  ])
                            - eviscw*(Sc[c  ]-Sc[c-1]) ) * a.dxidxi
                          + ( eviscn*(Sc[c+PW]-Sc[c   ])
                            - eviscs*(Sc[c   ]-Sc[c-PW]) ) * a.dyidyi;
                 const TF dt = L.rhohp * evisct*(qs[4]-qs[3])*L.dzhip;
-                const TF ver = ( dt - ds_b ) * cf;
-                ds_b = dt;
-                ts += hor + ver;
+                ts += hor + ( ( tq - bs_u ) - ( tc - bs_c ) + ( dt - ds_b ) ) * cf;
+                bs_c = tc; bs_u = tq; ds_b = dt;
                 if (active) sto(stk, oij, ts);
             }
             __syncthreads();
