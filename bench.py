@@ -65,10 +65,10 @@ KERNELS = list(PASSES) + ["cyclic", "ghost_cells", "mirror_walls", "rk3_zero_gho
 # workload and the dominant kernel match.
 MEASURED_TRAFFIC = {
     (512, 512, 256, 8): {
-        "tend_2i5_smag2": 9904.5e6, "pres_out_rk3": 9777.6e6, "pres_in_fft_x": 5205.1e6,
-        # one launch with the CFL/dnmul maxima per three: (4432.3 + 2 x 3248.7)/3
-        "evisc": 3643.2e6,
-        "tdma": 2673.2e6, "fft_x_bwd": 1091.5e6, "fft_y_fwd": 1078.4e6, "fft_y_bwd": 1078.4e6,
+        "tend_2i5_smag2": 10041.6e6, "pres_out_rk3": 9811.1e6, "pres_in_fft_x": 5221.0e6,
+        # one launch with the CFL/dnmul maxima per three: (4371.9 + 2 x 3241.7)/3
+        "evisc": 3618.4e6,
+        "ytri_fwd": 1228.0e6, "ytri_bwd": 1225.2e6, "fft_x_bwd": 1091.5e6,
     },
 }
 # Steps of the pass that times every kernel (before the timed steps).
