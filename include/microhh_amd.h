@@ -173,12 +173,28 @@ int  mhh_pres_2_exec(mhh_ctx* ctx, double dt);
 /* Pres_2::exec followed by Timeloop::exec (src/pres_2.cxx:64-94,
  * src/timeloop.cxx:251-286): the pressure gradient and the RK3 update of the
  * momentum fields in one pass; dt is the sub time step of the solve, substep
- * and rkdt those of the RK3 update.  scalars != 0: every bound scalar is
- * integrated as well (scalar 0 in the same pass), so the call is the whole
- * Pres_2::exec + Timeloop::exec; scalars == 0 leaves them to
- * mhh_timeloop_rk3_scalars.  No single counterpart in the reference, which
- * calls the two in sequence (src/model.cxx:436-504). */
-int  mhh_pres_2_exec_rk3(mhh_ctx* ctx, double dt, int substep, double rkdt, int scalars);
+ * and rkdt those of the RK3 update.  flags:
+ *   MHH_RK3_SCALARS  every bound scalar is integrated as well (scalar 0 in the
+ *                    same pass), so the call is the whole Pres_2::exec +
+ *                    Timeloop::exec; without it the scalars are left to
+ *                    mhh_timeloop_rk3_scalars.
+ *   MHH_RK3_DEFER    (with MHH_RK3_SCALARS) u, v, w and the scalars are
+ *                    updated, but ut, vt, wt and the tendency of scalar 0 keep
+ *                    their values from before the call: the reference's
+ *                    tendencies after the update, cA (at - grad p) and cA st,
+ *                    are completed by the next mhh_tendency_2i5_smag2 on the
+ *                    fly, or by mhh_tendency_settle.  Every other call that
+ *                    reads or writes tendencies or p completes them first;
+ *                    only direct reads of the tendency memory in between see
+ *                    the deferred values.  Saves the 4 tendency write passes.
+ * No single counterpart in the reference, which calls the two in sequence
+ * (src/model.cxx:436-504). */
+enum { MHH_RK3_SCALARS = 1, MHH_RK3_DEFER = 2 };
+int  mhh_pres_2_exec_rk3(mhh_ctx* ctx, double dt, int substep, double rkdt, int flags);
+/* Completes a deferred tendency update (MHH_RK3_DEFER) in place, so that ut,
+ * vt, wt and the tendency of scalar 0 hold the reference's values; no-op when
+ * none is pending.  No counterpart in the reference. */
+int  mhh_tendency_settle(mhh_ctx* ctx);
 /* MHH_MIXED only: the pressure field then holds p minus its horizontal mean,
  * and the mean (solved in FP64) is kept per level; out receives kcells values
  * (p = field + out[k]).  For FP64 and FP32 contexts out is zero.  No
@@ -193,7 +209,8 @@ int  mhh_pres_2_divergence(mhh_ctx* ctx, double* divmax);
  * include/diff_kernels.h:144-484) in one pass over u, v, w, scalar 0 and
  * evisc, for the default boundary (no surface model).  The terms are added to
  * ut, vt, wt, st[0] in the reference order; buoyancy != 0 includes the
- * buoyancy of scalar 0 on wt.  Scalars beyond the first are not touched. */
+ * buoyancy of scalar 0 on wt.  Scalars beyond the first are not touched.
+ * A pending deferred update (MHH_RK3_DEFER) is completed in the same pass. */
 int  mhh_tendency_2i5_smag2(mhh_ctx* ctx, const mhh_smag_params* par, int buoyancy);
 
 /* ---- time integration ---- */

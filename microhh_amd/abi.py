@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libmicrohh_amd.so")
 MHH_MAX_SCALARS = 8
 MHH_FP64, MHH_FP32, MHH_MIXED = 0, 1, 2
 MHH_OK, MHH_EINVAL, MHH_EHIP, MHH_ESTATE, MHH_ENOMEM = 0, 1, 2, 3, 4
+MHH_RK3_SCALARS, MHH_RK3_DEFER = 1, 2
 BC = {"dirichlet": 0, "neumann": 1, "flux": 2, "ustar": 3, "off": 4, "noslip": 0, "freeslip": 1}
 EDGE_EW, EDGE_NS, EDGE_BOTH = 0, 1, 2
 
@@ -83,6 +84,7 @@ SIGNATURES = {
     "mhh_timeloop_rk3": (C.c_int, [VP, C.c_int, C.c_double]),
     "mhh_timeloop_rk3_scalars": (C.c_int, [VP, C.c_int, C.c_double]),
     "mhh_pres_2_exec_rk3": (C.c_int, [VP, C.c_double, C.c_int, C.c_double, C.c_int]),
+    "mhh_tendency_settle": (C.c_int, [VP]),
     "mhh_device_alloc": (C.c_int, [C.c_size_t, C.POINTER(VP)]),
     "mhh_device_free": (C.c_int, [VP]),
     "mhh_copy_to_device": (C.c_int, [VP, VP, C.c_size_t]),

@@ -326,6 +326,7 @@ extern "C" {
 int mhh_advec_2i5_exec(mhh_ctx* c)
 {
     MHH_REQUIRE_BOUND(c);
+    MHH_CHECK(mhh::settle_tendencies(c));
     if (c->gd.igc < 3 || c->gd.jgc < 3)
         return mhh::fail(MHH_EINVAL, "advec_2i5 needs igc = jgc >= 3 (src/advec_2i5.cxx:42-45)");
     return c->precision == MHH_FP64 ? exec_2i5<double>(c, c->pd) : exec_2i5<float>(c, c->pf);
@@ -344,6 +345,7 @@ int mhh_advec_2i5_set_fluxlimit(mhh_ctx* c, const int* limited, int nscalars)
 int mhh_advec_2_exec(mhh_ctx* c)
 {
     MHH_REQUIRE_BOUND(c);
+    MHH_CHECK(mhh::settle_tendencies(c));
     return c->precision == MHH_FP64 ? exec_2<double>(c, c->pd) : exec_2<float>(c, c->pf);
 }
 

@@ -343,6 +343,9 @@ int mhh_boundary_cyclic(mhh_ctx* c, void* field, int edge)
 {
     if (!c || !field) return mhh::fail(MHH_EINVAL, "mhh_boundary_cyclic: null argument");
     if (edge < MHH_EDGE_EAST_WEST || edge > MHH_EDGE_BOTH) return mhh::fail(MHH_EINVAL, "bad edge");
+    if (c->defer_pending && (field == c->f.ut || field == c->f.vt || field == c->f.wt || field == c->f.st[0]
+                             || field == c->f.p))
+        MHH_CHECK(mhh::settle_tendencies(c));
     return c->precision == MHH_FP64 ? cyclic<double>(c, static_cast<double*>(field), edge)
                                     : cyclic<float>(c, static_cast<float*>(field), edge);
 }
@@ -363,6 +366,7 @@ int mhh_boundary_set_ghost_cells(mhh_ctx* c, const mhh_bc_desc* bc)
 int mhh_thermo_dry_exec(mhh_ctx* c)
 {
     MHH_REQUIRE_BOUND(c);
+    MHH_CHECK(mhh::settle_tendencies(c));
     if (c->f.nscalars < 1) return mhh::fail(MHH_ESTATE, "thermo_dry needs scalar 0 (th)");
     if (!c->have_base_state) return mhh::fail(MHH_ESTATE, "thermo_dry needs mhh_set_base_state()");
     return c->precision == MHH_FP64 ? buoyancy<double>(c, c->pd) : buoyancy<float>(c, c->pf);
@@ -371,6 +375,7 @@ int mhh_thermo_dry_exec(mhh_ctx* c)
 int mhh_timeloop_rk3(mhh_ctx* c, int substep, double dt)
 {
     MHH_REQUIRE_BOUND(c);
+    MHH_CHECK(mhh::settle_tendencies(c));
     if (substep < 0 || substep > 2) return mhh::fail(MHH_EINVAL, "rk3 substep must be 0, 1 or 2");
     return c->precision == MHH_FP64 ? rk3<double>(c, substep, dt) : rk3<float>(c, substep, dt);
 }
@@ -378,6 +383,7 @@ int mhh_timeloop_rk3(mhh_ctx* c, int substep, double dt)
 int mhh_timeloop_rk3_scalars(mhh_ctx* c, int substep, double dt)
 {
     MHH_REQUIRE_BOUND(c);
+    MHH_CHECK(mhh::settle_tendencies(c));
     if (substep < 0 || substep > 2) return mhh::fail(MHH_EINVAL, "rk3 substep must be 0, 1 or 2");
     return c->precision == MHH_FP64 ? rk3<double>(c, substep, dt, false, true)
                                     : rk3<float>(c, substep, dt, false, true);

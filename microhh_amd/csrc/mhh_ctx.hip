@@ -290,6 +290,9 @@ int mhh_fields_bind(mhh_ctx* c, const mhh_field_set* f)
         return fail(MHH_EINVAL, "mhh_fields_bind: nscalars out of range");
     for (int n = 0; n < f->nscalars; ++n)
         if (!f->s[n] || !f->st[n]) return fail(MHH_EINVAL, "mhh_fields_bind: scalar or its tendency missing");
+    // a pending deferred tendency update belongs to the previous fields, which
+    // the caller may have freed: it is dropped, not completed
+    c->defer_pending = false;
     c->f = *f;
     c->bound = true;
     return 0;

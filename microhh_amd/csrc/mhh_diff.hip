@@ -893,6 +893,7 @@ int mhh_diff_smag2_exec_viscosity_limits(mhh_ctx* c, const mhh_smag_params* par,
 int mhh_diff_smag2_exec(mhh_ctx* c, const mhh_smag_params* par)
 {
     MHH_REQUIRE_BOUND(c);
+    MHH_CHECK(mhh::settle_tendencies(c));
     if (!par) return mhh::fail(MHH_EINVAL, "null smag params");
     if (!c->f.evisc) return mhh::fail(MHH_ESTATE, "evisc field is not bound");
     return c->precision == MHH_FP64 ? diffuse<double>(c, c->pd, *par) : diffuse<float>(c, c->pf, *par);
@@ -909,6 +910,7 @@ int mhh_diff_smag2_dnmul(mhh_ctx* c, double tPr, double* out)
 int mhh_diff_2_exec(mhh_ctx* c, double visc, const double* svisc)
 {
     MHH_REQUIRE_BOUND(c);
+    MHH_CHECK(mhh::settle_tendencies(c));
     return c->precision == MHH_FP64 ? diff_2<double>(c, c->pd, visc, svisc) : diff_2<float>(c, c->pf, visc, svisc);
 }
 

@@ -102,6 +102,14 @@ struct mhh_ctx
     // y transforms and tridiagonal sweep as separate passes even where the
     // fused ones apply (a check of the fused kernels).
     bool ytri_split = false;
+    // A deferred end of the RK3 update (MHH_RK3_DEFER): ut, vt, wt and st[0]
+    // hold the tendencies as they were before the pressure gradient and the
+    // cA scaling; the reference's values are cA (at - grad p) with the p of
+    // the last solve (cA at for the scalar).  The fused tendency pass
+    // completes them on the fly; every other call that touches tendencies or
+    // p completes them first (settle_tendencies).
+    bool defer_pending = false;
+    double defer_cA = 0.;
     // Scalars advected with the Koren flux limiter ([advec] fluxlimit_list).
     int fluxlimit[MHH_MAX_SCALARS] = {};
     // Reduction scratch: per-block partial maxima and one result slot.
@@ -179,6 +187,9 @@ namespace mhh
 
     // Partial sums per level of the mixed mode's plane mean.
     constexpr int PM_CHUNKS = 16;
+
+    // Completes a deferred RK3 tendency update, if one is pending (mhh_pres.hip).
+    int settle_tendencies(mhh_ctx* ctx);
 
     // Pressure solver setup (mhh_pres.hip).
     int pres_init(mhh_ctx* ctx);

@@ -321,6 +321,55 @@ namespace
         dft<TF, R>(v, S);
     }
 
+    // wave_fft with the twiddles of each pass loaded at that pass (tw is a
+    // few KB and cache-resident) instead of held in registers for the whole
+    // transform: for kernels that transform one row per wave.
+    template<typename TF, int R, int S>
+    __device__ __forceinline__ void wave_fft_tw(cplx<TF> (&v)[R], cplx<TF>* A, const double2* __restrict__ tw, int lane)
+    {
+        constexpr int n = 64*R, P = wave_passes<R>();
+        auto load_tw = [&](cplx<TF> (&w)[R-1], int Ns)
+        {
+            const int k = lane & (Ns - 1), ts = n/(Ns*R);
+            #pragma unroll
+            for (int q = 1; q < R; ++q)
+            {
+                const double2 x = tw[q*k*ts];
+                w[q-1].x = TF(x.x); w[q-1].y = TF(S < 0 ? x.y : -x.y);
+            }
+        };
+        dft<TF, R>(v, S);
+        #pragma unroll
+        for (int q = 0; q < R; ++q) A[lpad(lane*R + q)] = v[q];
+        wave_lds_sync();
+        int Ns = R;
+        #pragma unroll
+        for (int p = 1; p < P-1; ++p)
+        {
+            cplx<TF> w[R-1];
+            load_tw(w, Ns);
+            #pragma unroll
+            for (int q = 0; q < R; ++q) v[q] = A[lpad(lane + 64*q)];
+            #pragma unroll
+            for (int q = 1; q < R; ++q) v[q] = cmul(v[q], w[q-1]);
+            dft<TF, R>(v, S);
+            wave_lds_sync();
+            const int d = (lane/Ns)*Ns*R + (lane & (Ns - 1));
+            #pragma unroll
+            for (int q = 0; q < R; ++q) A[lpad(d + q*Ns)] = v[q];
+            wave_lds_sync();
+            Ns *= R;
+        }
+        cplx<TF> w[R-1];
+        load_tw(w, Ns);
+        #pragma unroll
+        for (int q = 0; q < R; ++q) v[q] = A[lpad(lane + 64*q)];
+        wave_lds_sync();
+        #pragma unroll
+        for (int q = 1; q < R; ++q) v[q] = cmul(v[q], w[q-1]);
+        dft<TF, R>(v, S);
+    }
+
     // y transform of contiguous rows of length n = 64 R, one wave per row
     // (no block barriers).  Waves walk the rows with a stride and load the next
     // row while transforming the current one.  S = -1 forward, +1 inverse
@@ -399,7 +448,7 @@ namespace
             const TF* __restrict__ u, const TF* __restrict__ v, const TF* __restrict__ w,
             const TF* __restrict__ ut, const TF* __restrict__ vt, const TF* __restrict__ wt,
             const TF* __restrict__ dzi, const TF* __restrict__ rho, const TF* __restrict__ rhoh,
-            TF dxi, TF dyi, TF dti)
+            TF dxi, TF dyi, TF dti, int cpx)
     {
         using T = cplx<TF>;
         constexpr int XW = xwaves<TF>(), NT = 64*XW;
@@ -408,7 +457,20 @@ namespace
         __shared__ T rows[XW][wave_row_elems<R>()];
         __shared__ double wsum[XW];
         const int lane = threadIdx.x, wv = threadIdx.y, tid = lane + 64*wv;
-        const int M = g.jtot, k = blockIdx.y, bj = blockIdx.x, nbj = gridDim.x;
+        const int M = g.jtot, nbj = M/JB;
+        // cpx > 0: a 1-D grid in which XCD x (blocks b = x mod 8) owns the
+        // cpx row groups [x cpx, (x+1) cpx) and takes them level by level,
+        // so the w and wt planes of level k+1 and the v, vt rows of the next
+        // group are read again by a block that runs shortly after on the
+        // same XCD (its own L2); else blockIdx = (group, level).
+        int k, bj;
+        if (cpx > 0)
+        {
+            const int li = blockIdx.x >> 3;
+            k = li / cpx;
+            bj = (blockIdx.x & 7)*cpx + li % cpx;
+        }
+        else { k = blockIdx.y; bj = blockIdx.x; }
         const int j0 = bj*JB, ja = j0 + 2*wv, jb = ja + 1;
         T* A = rows[wv];
 
@@ -468,9 +530,7 @@ namespace
             #pragma unroll
             for (int q = 0; q < R; ++q) { z[q].x -= ma; z[q].y -= mb; }
         }
-        T wtw[P_-1][R-1];
-        wave_twiddles<TF, R, -1>(wtw, tw, lane);
-        wave_fft<TF, R, -1>(z, A, wtw, lane);
+        wave_fft_tw<TF, R, -1>(z, A, tw, lane);
         if (MODE == 3 && lane == 0) { z[0].x = TF(sa); z[0].y = TF(sb); }   // x = 0 is lane 0, q = 0
 
         // the two half-spectra: Z[kx] and Z[n-kx] come from other lanes
@@ -1044,13 +1104,13 @@ namespace
         wt[ijk] -= dpz * dzhi[k];
     }
 
-    // Pres_2::output followed by the RK3 update of u, v, w (src/pres_2.cxx:364-387,
-    // src/timeloop.cxx:251-286): the same operations in one pass.
     // Pres_2::output (src/pres_2.cxx:364-387) followed by the RK3 update of
     // u, v, w (src/timeloop.cxx:251-286) and, with NS = 1, of scalar s (whose
     // update needs no pressure), one point per thread on the plane-ordered
-    // grid: a pure streaming pass.
-    template<typename TF, int NS>
+    // grid: a pure streaming pass.  DEFER: the tendencies are left as they
+    // came in (before the gradient and the cA scaling); the next fused
+    // tendency pass, or settle_g, completes them.
+    template<typename TF, int NS, bool DEFER>
     __global__ void __launch_bounds__(BX*BY) pres_out_rk3_g(
             TF* __restrict__ u, TF* __restrict__ v, TF* __restrict__ w,
             TF* __restrict__ ut, TF* __restrict__ vt, TF* __restrict__ wt,
@@ -1070,17 +1130,57 @@ namespace
         const TF pc = p[ijk];
         TF dpz = pc - p[ijk-g.ijcells];
         if (dpbar) dpz += dpbar[k];   // mixed mode: the FP64 mean's part
-        TF tu = ut[ijk]; tu -= (pc - p[ijk-1]) * dxi;
-        TF tv = vt[ijk]; tv -= (pc - p[ijk-g.icells]) * dyi;
-        TF tw = wt[ijk]; tw -= dpz * dzhi[k];
-        u[ijk] += cBdt*tu; ut[ijk] = zero ? TF(0) : tu*cAn;
-        v[ijk] += cBdt*tv; vt[ijk] = zero ? TF(0) : tv*cAn;
-        w[ijk] += cBdt*tw; wt[ijk] = zero ? TF(0) : tw*cAn;
+        // one rounding per gradient term, written out as the deferred update
+        // of the tendency pass writes it
+        const TF tu = fma(p[ijk-1] - pc, dxi, ut[ijk]);
+        const TF tv = fma(p[ijk-g.icells] - pc, dyi, vt[ijk]);
+        const TF tw = fma(-dpz, dzhi[k], wt[ijk]);
+        u[ijk] += cBdt*tu; v[ijk] += cBdt*tv; w[ijk] += cBdt*tw;
+        if (!DEFER)
+        {
+            ut[ijk] = zero ? TF(0) : tu*cAn;
+            vt[ijk] = zero ? TF(0) : tv*cAn;
+            wt[ijk] = zero ? TF(0) : tw*cAn;
+        }
         if (NS)
         {
             const TF ts = st[ijk];
-            s[ijk] += cBdt*ts; st[ijk] = zero ? TF(0) : ts*cAn;
+            s[ijk] += cBdt*ts;
+            if (!DEFER) st[ijk] = zero ? TF(0) : ts*cAn;
         }
+    }
+
+    // Completes a deferred RK3 update of the tendencies: at = cA (at - grad p)
+    // for u, v, w (the gradient of the p the update used, as in
+    // pres_out_rk3_g) and at = cA at for scalar 0 (NS = 1); cA = 0 writes
+    // zeros, as the reference's first substep does.
+    template<typename TF, int NS>
+    __global__ void __launch_bounds__(BX*BY) settle_g(
+            TF* __restrict__ ut, TF* __restrict__ vt, TF* __restrict__ wt, TF* __restrict__ st,
+            const TF* __restrict__ p, const TF* __restrict__ dzhi, const TF* __restrict__ dpbar,
+            TF dxi, TF dyi, TF cA, int zero, Gd g, int nbx, int nby)
+    {
+        const int tile = xcd_tile(blockIdx.x, gridDim.x);
+        if (tile >= nbx*nby*g.ktot) return;
+        const int i = (tile % nbx)*BX + threadIdx.x + g.istart;
+        const int j = ((tile / nbx) % nby)*BY + threadIdx.y + g.jstart;
+        const int k = tile/(nbx*nby) + g.kstart;
+        if (i >= g.iend || j >= g.jend) return;
+        const int ijk = i + j*g.icells + k*g.ijcells;
+        if (zero)
+        {
+            ut[ijk] = TF(0); vt[ijk] = TF(0); wt[ijk] = TF(0);
+            if (NS) st[ijk] = TF(0);
+            return;
+        }
+        const TF pc = p[ijk];
+        TF dpz = pc - p[ijk-g.ijcells];
+        if (dpbar) dpz += dpbar[k];
+        const TF tu = fma(p[ijk-1] - pc, dxi, ut[ijk]);
+        const TF tv = fma(p[ijk-g.icells] - pc, dyi, vt[ijk]);
+        const TF tw = fma(-dpz, dzhi[k], wt[ijk]);
+        ut[ijk] = tu*cA; vt[ijk] = tv*cA; wt[ijk] = tw*cA;
+        if (NS) st[ijk] *= cA;
     }
 
     // Pres_2::calc_divergence (src/pres_2.cxx:389-422), per-block maxima.  TC
@@ -1297,12 +1397,14 @@ namespace
                                const Prof<TF>& p, TF dxi, TF dyi, TF dti)
     {
         const Gd& g = c->gd;
-        const dim3 grd(g.jtot/(2*xwaves<TF>()), g.ktot), blk(64, xwaves<TF>());
+        const int nbj = g.jtot/(2*xwaves<TF>());
+        const int cpx = nbj % 8 == 0 ? nbj/8 : 0;
+        const dim3 grd = cpx ? dim3(nbj*g.ktot) : dim3(nbj, g.ktot), blk(64, xwaves<TF>());
         const TF* u = static_cast<const TF*>(c->f.u); const TF* v = static_cast<const TF*>(c->f.v);
         const TF* w = static_cast<const TF*>(c->f.w); const TF* ut = static_cast<const TF*>(c->f.ut);
         const TF* vt = static_cast<const TF*>(c->f.vt); const TF* wt = static_cast<const TF*>(c->f.wt);
 #define MHH_XF(MODE) fft_x_fwd_wave_g<TF, R, MODE><<<grd, blk, 0, c->stream>>>(P, C, tw, g, c->pm_mean, c->pm_part, \
-                u, v, w, ut, vt, wt, p.dzi, p.rhoref, p.rhorefh, dxi, dyi, dti)
+                u, v, w, ut, vt, wt, p.dzi, p.rhoref, p.rhorefh, dxi, dyi, dti, cpx)
         if (mode == 3) MHH_XF(3);
         else if (mode == 2) MHH_XF(2);
         else if (mode == 1) MHH_XF(1);
@@ -1349,9 +1451,13 @@ namespace
     // rk3 (time step rkdt) instead of Pres_2::output alone.
     template<typename TF>
     int exec(mhh_ctx* c, const Prof<TF>& p, const Pres_tables<TF>& t, double dt, int rk3 = -1, double rkdt = 0.,
-             bool with_scalars = false)
+             bool with_scalars = false, bool defer = false)
     {
         using T = cplx<TF>;
+        // a deferred update needs the p it used: complete it before the solve
+        MHH_CHECK(settle_tendencies(c));
+        if (defer && (!with_scalars || c->f.nscalars < 1))
+            return fail(MHH_EINVAL, "pres_2: a deferred tendency update needs scalar 0 in the same call");
         const Gd& g = c->gd;
         if (!c->f.p || !c->f.tmp1 || !c->f.tmp2)
             return fail(MHH_ESTATE, "pres_2 needs p, tmp1 and tmp2 bound");
@@ -1487,14 +1593,21 @@ namespace
             mhh::Timer _t(c, "pres_out_rk3");
             const int nbx = (g.itot + BX - 1)/BX, nby = (g.jtot + BY - 1)/BY;
             const int nb = xcd_blocks(nbx*nby*g.ktot);
-#define MHH_PO(NS) pres_out_rk3_g<TF, NS><<<nb, dim3(BX, BY), 0, c->stream>>>( \
+#define MHH_PO(NS, D) pres_out_rk3_g<TF, NS, D><<<nb, dim3(BX, BY), 0, c->stream>>>( \
                 static_cast<TF*>(c->f.u), static_cast<TF*>(c->f.v), static_cast<TF*>(c->f.w), \
                 static_cast<TF*>(c->f.ut), static_cast<TF*>(c->f.vt), static_cast<TF*>(c->f.wt), s0, st0, \
                 P, p.dzhi, dpb, dxi, dyi, cBdt, cA[substepn], substepn == 0, g, nbx, nby)
-            if (ns > 0) MHH_PO(1); else MHH_PO(0);
+            if (defer) MHH_PO(1, true);
+            else if (ns > 0) MHH_PO(1, false);
+            else MHH_PO(0, false);
 #undef MHH_PO
         }
         MHH_CHECK(check_launch("pres_out_rk3"));
+        if (defer)
+        {
+            c->defer_pending = true;
+            c->defer_cA = double(cA[substepn]);
+        }
         for (int n = 1; n < ns; ++n)
             MHH_CHECK(rk3_field<TF>(c, static_cast<TF*>(c->f.s[n]), static_cast<TF*>(c->f.st[n]),
                                     cBdt, cA[substepn], substepn == 0));
@@ -1507,6 +1620,23 @@ namespace
             MHH_CHECK(zero_tendency_ghosts<TF>(c, static_cast<TF*>(c->f.wt)));
         }
         return 0;
+    }
+
+    template<typename TF>
+    int settle(mhh_ctx* c, const Prof<TF>& p)
+    {
+        const Gd& g = c->gd;
+        const TF dxi = TF(1.)/TF(g.dx), dyi = TF(1.)/TF(g.dy);
+        const TF* dpb = c->mixed ? reinterpret_cast<const TF*>(c->dpbar) : nullptr;
+        const int nbx = (g.itot + BX - 1)/BX, nby = (g.jtot + BY - 1)/BY;
+        const int nb = xcd_blocks(nbx*nby*g.ktot);
+        const double cA = c->defer_cA;
+        { mhh::Timer _t(c, "settle");
+          settle_g<TF, 1><<<nb, dim3(BX, BY), 0, c->stream>>>(
+                static_cast<TF*>(c->f.ut), static_cast<TF*>(c->f.vt), static_cast<TF*>(c->f.wt),
+                static_cast<TF*>(c->f.st[0]), static_cast<const TF*>(c->f.p), p.dzhi, dpb, dxi, dyi,
+                TF(cA), cA == 0., g, nbx, nby); }
+        return check_launch("settle");
     }
 
     template<typename TF>
@@ -1590,6 +1720,13 @@ namespace mhh
         return 0;
     }
 
+    int settle_tendencies(mhh_ctx* c)
+    {
+        if (!c->defer_pending) return 0;
+        c->defer_pending = false;
+        return c->precision == MHH_FP64 ? settle<double>(c, c->pd) : settle<float>(c, c->pf);
+    }
+
     void pres_free(mhh_ctx* c)
     {
         free_tables(c->td);
@@ -1609,13 +1746,21 @@ int mhh_pres_2_exec(mhh_ctx* c, double dt)
     return c->precision == MHH_FP64 ? exec<double>(c, c->pd, c->td, dt) : exec<float>(c, c->pf, c->tf, dt);
 }
 
-int mhh_pres_2_exec_rk3(mhh_ctx* c, double dt, int substep, double rkdt, int scalars)
+int mhh_pres_2_exec_rk3(mhh_ctx* c, double dt, int substep, double rkdt, int flags)
 {
     MHH_REQUIRE_BOUND(c);
     if (!(dt > 0)) return mhh::fail(MHH_EINVAL, "pres_2: dt must be positive");
     if (substep < 0 || substep > 2) return mhh::fail(MHH_EINVAL, "rk3 substep must be 0, 1 or 2");
-    return c->precision == MHH_FP64 ? exec<double>(c, c->pd, c->td, dt, substep, rkdt, scalars != 0)
-                                    : exec<float>(c, c->pf, c->tf, dt, substep, rkdt, scalars != 0);
+    if (flags & ~(MHH_RK3_SCALARS | MHH_RK3_DEFER)) return mhh::fail(MHH_EINVAL, "pres_2: unknown flags");
+    const bool scalars = flags & MHH_RK3_SCALARS, defer = flags & MHH_RK3_DEFER;
+    return c->precision == MHH_FP64 ? exec<double>(c, c->pd, c->td, dt, substep, rkdt, scalars, defer)
+                                    : exec<float>(c, c->pf, c->tf, dt, substep, rkdt, scalars, defer);
+}
+
+int mhh_tendency_settle(mhh_ctx* c)
+{
+    MHH_REQUIRE_BOUND(c);
+    return mhh::settle_tendencies(c);
 }
 
 int mhh_pres_2_mean_profile(mhh_ctx* c, double* out)

@@ -9,6 +9,11 @@ namespace mhh
 {
     template<typename TF> __device__ __forceinline__ TF i2(TF a, TF b) { return TF(0.5)*(a + b); }
 
+    // A product rounded on its own (never contracted into a following add):
+    // for a value that the reference stores before adding to it.
+    __device__ __forceinline__ double mul_rn(double a, double b) { return __dmul_rn(a, b); }
+    __device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }
+
     // 4th-order interpolation and 3rd-order dissipation (finite_difference.h:107-121).
     template<typename TF> __device__ __forceinline__ TF i4ws(TF a, TF b, TF c, TF d)
     { return TF(7./12.)*(b + c) - TF(1./12.)*(a + d); }

@@ -35,9 +35,19 @@ namespace
     struct Tend_args
     {
         TF dxi, dyi, dxidxi, dyidyi, visc, svisc, tPr_i;
+        TF cA;          // TEND_DEFER: the RK3 factor of the deferred update
         int buoyancy;
         int nti, ntj, nkc, kchunk;
     };
+
+    // How the pass starts from the stored tendencies:
+    //   TEND_ADD    adds to them (the reference's exec);
+    //   TEND_FRESH  a deferred update with cA = 0 is pending, so they are
+    //               zero: written, not read;
+    //   TEND_DEFER  a deferred update is pending: they are first completed
+    //               to cA (at - grad p) (cA st for the scalar), the end of
+    //               the previous substep's pres_out_rk3_g.
+    enum { TEND_ADD = 0, TEND_FRESH = 1, TEND_DEFER = 2 };
 
     template<typename TF>
     __device__ __forceinline__ void load_plane(TF* __restrict__ dst, const TF* __restrict__ src, int k,
@@ -158,12 +168,13 @@ namespace
         }
     }
 
-    template<typename TF>
+    template<typename TF, int MODE>
     __global__ void __launch_bounds__(TX*TY, 2) tend_2i5_smag2_g(
             TF* __restrict__ ut, TF* __restrict__ vt, TF* __restrict__ wt, TF* __restrict__ st,
             const TF* __restrict__ u, const TF* __restrict__ v, const TF* __restrict__ w,
             const TF* __restrict__ s, const TF* __restrict__ evisc,
-            const Level<TF>* __restrict__ lev, Tend_args<TF> a, Gd g)
+            const Level<TF>* __restrict__ lev, const TF* __restrict__ p, const TF* __restrict__ dpbar,
+            Tend_args<TF> a, Gd g)
     {
         __shared__ TF lds[7*PN + 3*EN];
 
@@ -259,6 +270,11 @@ namespace
             ew_w = TF(0.25)*(Ep[ce-1] + Ec[ce-1] + Ep[ce] + Ec[ce]) + visc;
             ew_s = TF(0.25)*(Ep[ce-EW] + Ec[ce-EW] + Ep[ce] + Ec[ce]) + visc;
         }
+        // TEND_DEFER: p of the own column one level down.  FP32 carries it
+        // from level to level; FP64, short of registers, loads it with the
+        // other p values of the level.
+        constexpr bool pb_carry = sizeof(TF) == 4;
+        TF pb = (MODE == TEND_DEFER && pb_carry) ? ldo(p + size_t(kb-1)*kk, oij) : TF(0);
         // Every load of the preamble has landed: the waits inside the loop then
         // only count the loads issued in the loop.
         __builtin_amdgcn_s_waitcnt(0);
@@ -274,7 +290,20 @@ namespace
             // level, which stay in flight until the end of this one.  Every
             // level issues the same loads (the last one of the chunk fetches
             // clamped, unused planes), so the waits are the same on every pass.
-            TF tu = ldo(utk, oij), tv = ldo(vtk, oij), tw = ldo(wtk, oij), ts = ldo(stk, oij);
+            TF tu = TF(0), tv = TF(0), tw = TF(0), ts = TF(0);
+            if (MODE != TEND_FRESH)
+            {
+                tu = ldo(utk, oij); tv = ldo(vtk, oij); tw = ldo(wtk, oij); ts = ldo(stk, oij);
+            }
+            TF pc = TF(0), pw = TF(0), ps = TF(0);
+            if (MODE == TEND_DEFER)
+            {
+                // p at the point and its west, south (and lower) neighbours
+                const TF* __restrict__ pk = p + kofs;
+                pc = ldo(pk, oij); pw = ldo(pk, oij - uint32_t(sizeof(TF)));
+                ps = ldo(pk, oij - uint32_t(g.icells*sizeof(TF)));
+                if (!pb_carry) pb = ldo(pk - kk, oij);
+            }
             __builtin_amdgcn_sched_barrier(0);
             const size_t kq = size_t(min(k + 4, g.kcells - 1))*kk;
             const TF nu = ldo(u + kq, oij), nv = ldo(v + kq, oij), nw = ldo(w + kq, oij), ns = ldo(s + kq, oij);
@@ -300,6 +329,26 @@ namespace
             // the south-west edge viscosity (i-1/2, j-1/2) is u's south and v's west edge
             const TF evsw = TF(0.25)*(Ec[ce-1-EW] + Ec[ce-EW] + Ec[ce-1] + Ec[ce]) + visc;
             TF ewn_w, ewn_s;   // this level's top edges of u and v: w's west and south edges one level up
+
+            // the deferred end of the previous substep, all four at once so
+            // that the p values are dead before the stencils (tw is used
+            // above the bottom wall only)
+            if (MODE == TEND_DEFER)
+            {
+                TF dpz = pc - pb;
+                if (dpbar) dpz += dpbar[k];   // mixed mode: the FP64 mean's part
+                tu = mul_rn(fma(pw - pc, dxi, tu), a.cA);
+                tv = mul_rn(fma(ps - pc, dyi, tv), a.cA);
+                tw = mul_rn(fma(-dpz, L.dzhi, tw), a.cA);
+                ts = mul_rn(ts, a.cA);
+                // FP64: keep the compiler from sinking these into the blocks
+                // that use them, which would keep the p values live through
+                // the stencils and spill (FP32 has registers to spare and runs
+                // faster with the loads waited for later)
+                if constexpr (sizeof(TF) == 8)
+                    __asm__ volatile("" : "+v"(tu), "+v"(tv), "+v"(tw), "+v"(ts));
+                if (pb_carry) pb = pc;
+            }
 
             // -------------------------------------------------------------- u
             {
@@ -439,6 +488,12 @@ namespace
         a.dxidxi = TF(1./(g.dx*g.dx)); a.dyidyi = TF(1./(g.dy*g.dy));
         a.visc = TF(par.visc); a.svisc = TF(par.svisc[0]); a.tPr_i = TF(1)/TF(par.tPr);
         a.buoyancy = buoyancy;
+        // a pending deferred RK3 update is completed in this pass
+        const int mode = !c->defer_pending ? TEND_ADD : c->defer_cA == 0. ? TEND_FRESH : TEND_DEFER;
+        a.cA = TF(c->defer_cA);
+        if (mode == TEND_DEFER && !c->f.p)
+            return fail(MHH_ESTATE, "a deferred tendency update needs p bound");
+        const TF* dpb = c->mixed ? reinterpret_cast<const TF*>(c->dpbar) : nullptr;
         a.nti = (g.itot + TX - 1)/TX;
         a.ntj = (g.jtot + TY - 1)/TY;
         a.kchunk = c->tend_kchunk > 0 ? c->tend_kchunk : KCHUNK;
@@ -446,11 +501,18 @@ namespace
         const int ntiles = a.nti*a.ntj*a.nkc;
         const int nblocks = (ntiles + 7)/8*8;
         { mhh::Timer _t(c, "tend_2i5_smag2");
-          tend_2i5_smag2_g<TF><<<nblocks, dim3(TX, TY), 0, c->stream>>>(
-                static_cast<TF*>(c->f.ut), static_cast<TF*>(c->f.vt), static_cast<TF*>(c->f.wt),
-                static_cast<TF*>(c->f.st[0]),
-                static_cast<const TF*>(c->f.u), static_cast<const TF*>(c->f.v), static_cast<const TF*>(c->f.w),
-                static_cast<const TF*>(c->f.s[0]), static_cast<const TF*>(c->f.evisc), p.lev, a, g); }
+#define MHH_TEND(M) tend_2i5_smag2_g<TF, M><<<nblocks, dim3(TX, TY), 0, c->stream>>>( \
+                static_cast<TF*>(c->f.ut), static_cast<TF*>(c->f.vt), static_cast<TF*>(c->f.wt), \
+                static_cast<TF*>(c->f.st[0]), \
+                static_cast<const TF*>(c->f.u), static_cast<const TF*>(c->f.v), static_cast<const TF*>(c->f.w), \
+                static_cast<const TF*>(c->f.s[0]), static_cast<const TF*>(c->f.evisc), p.lev, \
+                static_cast<const TF*>(c->f.p), dpb, a, g)
+          if (mode == TEND_DEFER) MHH_TEND(TEND_DEFER);
+          else if (mode == TEND_FRESH) MHH_TEND(TEND_FRESH);
+          else MHH_TEND(TEND_ADD);
+#undef MHH_TEND
+        }
+        c->defer_pending = false;
         return check_launch("tend_2i5_smag2");
     }
 }

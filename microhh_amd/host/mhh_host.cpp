@@ -272,6 +272,15 @@ namespace mhh_host
     }
 
     template<typename TF>
+    bool Fields<TF>::is_tendency(const std::string& name) const
+    {
+        if (mp.count(name) || sp.count(name) || sd.count(name) || name.size() < 2 || name.back() != 't')
+            return false;
+        const std::string b = name.substr(0, name.size()-1);
+        return mt.count(b) || st.count(b);
+    }
+
+    template<typename TF>
     Field3d<TF>& Fields<TF>::any(const std::string& name) const
     {
         if (mp.count(name)) return *mp.at(name);
@@ -757,8 +766,13 @@ namespace mhh_host
         if (fused_tendency)
         {
             // Pres_2::exec with the RK3 update (Timeloop::exec) of u, v, w
-            // and the scalars in its last kernel.
-            check(mhh_pres_2_exec_rk3(ctx_, subdt, timeloop->get_substep(), timeloop->get_dt(), 1),
+            // and the scalars in its last kernel.  The tendencies' share of
+            // the update, cA (at - grad p), is left to the next substep's
+            // tendency pass (or mhh_tendency_settle, for a caller that reads
+            // them): the pass reads them anyway, and with cA = 0 (the end of
+            // a step) it writes them without reading.
+            check(mhh_pres_2_exec_rk3(ctx_, subdt, timeloop->get_substep(), timeloop->get_dt(),
+                                      MHH_RK3_SCALARS | MHH_RK3_DEFER),
                   "mhh_pres_2_exec_rk3");
             timeloop->advance_substep();
         }
@@ -771,11 +785,17 @@ namespace mhh_host
     }
 
     template<typename TF>
-    void* Model<TF>::field_ptr(const std::string& name) { return fields->any(name).fld; }
+    void* Model<TF>::field_ptr(const std::string& name)
+    {
+        // a pointer to a tendency is handed out with the deferred update completed
+        if (fields->is_tendency(name)) check(mhh_tendency_settle(ctx_), "mhh_tendency_settle");
+        return fields->any(name).fld;
+    }
 
     template<typename TF>
     void Model<TF>::get_field(const std::string& name, double* host)
     {
+        if (fields->is_tendency(name)) check(mhh_tendency_settle(ctx_), "mhh_tendency_settle");
         check(mhh_sync(ctx_), "mhh_sync");
         fields->get_host(name, host);
         if (name == "p" && cs.precision == MHH_MIXED)
@@ -793,6 +813,7 @@ namespace mhh_host
     template<typename TF>
     void Model<TF>::set_field(const std::string& name, const double* host)
     {
+        if (fields->is_tendency(name)) check(mhh_tendency_settle(ctx_), "mhh_tendency_settle");
         check(mhh_sync(ctx_), "mhh_sync");
         fields->set_host(name, host);
     }

@@ -90,6 +90,8 @@ namespace mhh_host
             void set_host(const std::string& name, const double* data);
             void get_host(const std::string& name, double* data) const;
             Field3d<TF>& any(const std::string& name) const;
+            // name is a tendency field ("ut", "tht", ...)
+            bool is_tendency(const std::string& name) const;
 
             std::map<std::string, std::shared_ptr<Field3d<TF>>> mp, mt, sp, st, sd;
             std::vector<double> rhoref, rhorefh;

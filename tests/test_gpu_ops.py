@@ -355,6 +355,84 @@ def test_pres_rk3_matches_pres_then_rk3(setup, substep, scalars):
         assert err < 1e-14, f"{n}: {err:.3e}"
 
 
+def _fused_substeps(g, i, par, flags, nsub, dt=0.37, rkdt=0.61):
+    """nsub substeps of the fused sequence (tendency pass, projection with the
+    RK3 update) from the fixture state with zero tendencies; the prognostic
+    fields and p after every substep."""
+    _load_common(g, i)
+    g.put("evisc", i["evisc"])
+    for n in ("ut", "vt", "wt", "st0"):
+        g.put(n, np.zeros_like(i["ut"]))
+    out = []
+    for s in range(nsub):
+        for n in ("u", "v", "w", "s0"):
+            g.call("mhh_boundary_cyclic", g.ptr(n), 2)
+        g.call("mhh_tendency_2i5_smag2", C.byref(par), 1)
+        g.call("mhh_pres_2_exec_rk3", dt, s % 3, rkdt, flags)
+        out.append({n: g.get(n) for n in ("u", "v", "w", "s0", "p")})
+    return out
+
+
+def test_deferred_rk3_matches_reference_sequence(setup):
+    """MHH_RK3_DEFER: five substeps (0, 1, 2, 0, 1) with the tendencies' share
+    of the RK3 update left to the next tendency pass (completed there with
+    cA = -5/9, -153/128, 0 and -5/9) give the same u, v, w, s and p as the
+    reference sequence, and mhh_tendency_settle then yields the reference's
+    tendencies (cA = -153/128).  The kernel variants are compiled apart and
+    may contract their sums differently: u, v, w, s agree to 1e-14 (a few
+    ulp), p, whose solve amplifies such ulps (1.2e-13 measured on the 5x7x7
+    grid), and the tendencies derived from it to the 1e-12 parity bar."""
+    from microhh_amd import abi
+    p, i, d, g, *_ = setup
+    par = abi.SmagParams(0.23, 1.0 / 3.0, 1.5e-5, (C.c_double * 8)(*([2e-5] * 8)), 1, 0, 1)
+    ref = _fused_substeps(g, i, par, abi.MHH_RK3_SCALARS, 5)
+    ref_t = {n: g.get(n) for n in ("ut", "vt", "wt", "st0")}
+    out = _fused_substeps(g, i, par, abi.MHH_RK3_SCALARS | abi.MHH_RK3_DEFER, 5)
+    for s, (a, b) in enumerate(zip(out, ref)):
+        for n in a:
+            err = normwise(a[n], b[n])
+            assert err < (TOL if n == "p" else 1e-14), f"substep {s} {n}: {err:.3e}"
+    g.call("mhh_tendency_settle")
+    settled = {n: g.get(n) for n in ref_t}
+    for n, r in ref_t.items():
+        err = normwise(settled[n], r)
+        assert err < TOL, f"settled {n}: {err:.3e}"
+    # a second settle is a no-op
+    g.call("mhh_tendency_settle")
+    for n in ref_t:
+        assert np.array_equal(g.get(n), settled[n]), n
+
+
+@pytest.mark.parametrize("substep", [0, 2])
+def test_deferred_rk3_settled_by_other_calls(setup, substep):
+    """A call that reads the tendencies (here the buoyancy of
+    mhh_thermo_dry_exec) completes a pending deferred update first: the same
+    tendencies as the reference sequence (substep 2 ends the step: cA = 0)."""
+    from microhh_amd import abi
+    p, i, d, g, *_ = setup
+    dt, rkdt = 0.37, 0.61
+    res = []
+    for flags in (abi.MHH_RK3_SCALARS, abi.MHH_RK3_SCALARS | abi.MHH_RK3_DEFER):
+        _load_common(g, i)
+        for n in ("u", "v", "w"):
+            g.call("mhh_boundary_cyclic", g.ptr(n), 2)
+        g.call("mhh_pres_2_exec_rk3", dt, substep, rkdt, flags)
+        g.call("mhh_thermo_dry_exec")
+        res.append({n: g.get(n) for n in ("u", "v", "w", "s0", "ut", "vt", "wt", "st0")})
+    for n in res[0]:
+        err = normwise(res[1][n], res[0][n])
+        assert err < 1e-14, f"{n}: {err:.3e}"
+
+
+def test_deferred_rk3_needs_scalar_update(setup):
+    from microhh_amd import abi
+    p, i, d, g, *_ = setup
+    rc = abi.lib().mhh_pres_2_exec_rk3(g.ctx, 0.3, 0, 0.5, abi.MHH_RK3_DEFER)
+    assert rc == abi.MHH_EINVAL
+    rc = abi.lib().mhh_pres_2_exec_rk3(g.ctx, 0.3, 0, 0.5, 4)
+    assert rc == abi.MHH_EINVAL
+
+
 @pytest.mark.parametrize("thermo", [1, 0])
 def test_viscosity_limits_match_separate_calls(setup, thermo):
     """mhh_diff_smag2_exec_viscosity_limits against exec_viscosity followed by
