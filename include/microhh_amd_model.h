@@ -56,9 +56,14 @@ int mhh_model_destroy(mhh_model* m);
 int mhh_model_substep(mhh_model* m);
 /* n full time steps (3 substeps each). */
 int mhh_model_steps(mhh_model* m, int n);
-/* Device pointer of a named field ("u","v","w","th","ut","vt","wt","tht","p","evisc"). */
+/* Device pointer of a named field ("u","v","w","th","ut","vt","wt","tht","p","evisc").
+ * The model's substeps leave the tendencies' share of the RK3 update to the
+ * next substep (MHH_RK3_DEFER): for a tendency name the pending update is
+ * completed first, so the memory holds the reference's values until the
+ * next substep runs. */
 int mhh_model_field_ptr(mhh_model* m, const char* name, void** ptr);
-/* Copy a named 3-D field (ghost cells included) to host float64 storage of ncells values. */
+/* Copy a named 3-D field (ghost cells included) to host float64 storage of
+ * ncells values (tendencies completed as for mhh_model_field_ptr). */
 int mhh_model_get_field(mhh_model* m, const char* name, double* host);
 int mhh_model_set_field(mhh_model* m, const char* name, const double* host);
 /* Sizes: icells, jcells, kcells, istart, jstart, kstart (6 ints). */

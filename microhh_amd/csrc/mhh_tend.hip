@@ -270,11 +270,9 @@ namespace
             ew_w = TF(0.25)*(Ep[ce-1] + Ec[ce-1] + Ep[ce] + Ec[ce]) + visc;
             ew_s = TF(0.25)*(Ep[ce-EW] + Ec[ce-EW] + Ep[ce] + Ec[ce]) + visc;
         }
-        // TEND_DEFER: p of the own column one level down.  FP32 carries it
-        // from level to level; FP64, short of registers, loads it with the
-        // other p values of the level.
-        constexpr bool pb_carry = sizeof(TF) == 4;
-        TF pb = (MODE == TEND_DEFER && pb_carry) ? ldo(p + size_t(kb-1)*kk, oij) : TF(0);
+        // TEND_DEFER: p of the own column one level down, carried from level
+        // to level
+        TF pb = MODE == TEND_DEFER ? ldo(p + size_t(kb-1)*kk, oij) : TF(0);
         // Every load of the preamble has landed: the waits inside the loop then
         // only count the loads issued in the loop.
         __builtin_amdgcn_s_waitcnt(0);
@@ -298,11 +296,10 @@ namespace
             TF pc = TF(0), pw = TF(0), ps = TF(0);
             if (MODE == TEND_DEFER)
             {
-                // p at the point and its west, south (and lower) neighbours
+                // p at the point and its west and south neighbours
                 const TF* __restrict__ pk = p + kofs;
                 pc = ldo(pk, oij); pw = ldo(pk, oij - uint32_t(sizeof(TF)));
                 ps = ldo(pk, oij - uint32_t(g.icells*sizeof(TF)));
-                if (!pb_carry) pb = ldo(pk - kk, oij);
             }
             __builtin_amdgcn_sched_barrier(0);
             const size_t kq = size_t(min(k + 4, g.kcells - 1))*kk;
@@ -347,7 +344,7 @@ namespace
                 // faster with the loads waited for later)
                 if constexpr (sizeof(TF) == 8)
                     __asm__ volatile("" : "+v"(tu), "+v"(tv), "+v"(tw), "+v"(ts));
-                if (pb_carry) pb = pc;
+                pb = pc;
             }
 
             // -------------------------------------------------------------- u
