@@ -738,48 +738,95 @@ namespace
         }
     }
 
+    // Paired modes.  For an even itot the x spectrum at kx = 0 and at the
+    // last mode kxN = itot/2 is real, so block 0 of the fused y/tridiagonal
+    // kernels takes both (nx - 1 blocks instead of nx, which would leave one
+    // block to run after all the others on a 256-CU device): it transforms
+    // Z = C[.][0][j] + i C[.][kxN][j] as one complex row.  Both y spectra are
+    // Hermitian, as are the solutions (the systems depend on ky only through
+    // bmatj[ky] = bmatj[jtot-ky]), so their independent halves fill one row
+    // of jtot slots, the layout row kx = 0 then holds between the two kernels:
+    //   slot 0, jtot/2    the real pair (mode (0, ky), mode (kxN, ky)) = Z[ky]
+    //   0 < ky < jtot/2   mode (0, ky)         = (Z[ky] + conj Z[-ky])/2
+    //   jtot/2 < ky       mode (kxN, jtot-ky)  = (Z[-ky] - conj Z[ky])/(2i)
+    // Every thread solves one complex system; only the two real pairs have
+    // different pivots for their two components.  Returns the elimination
+    // table indices of the thread's x and y components.
+    __device__ __forceinline__ void pair_systems(bool pair, int kx, int kxN, int ky, int n, int& lx, int& ly)
+    {
+        const int h = n/2 + 1;
+        if (!pair) { lx = ly = kx*h + (ky <= n/2 ? ky : n - ky); return; }
+        lx = ky <= n/2 ? ky : kxN*h + (n - ky);
+        ly = (ky == 0 || 2*ky == n) ? kxN*h + ky : lx;
+    }
+
     // y transform and tridiagonal sweep fused, for jtot = 64 R.  A block of R
-    // waves owns one kx and runs through the levels in groups of R: each wave
-    // transforms one level's y row (the wave FFT) into its LDS row, then each
-    // thread, one per ky, takes the group's R levels of its mode through the
-    // forward elimination (precomputed per system, see tdma_g) and stores them.
-    // The spectrum is read and written once instead of twice.
+    // waves owns one kx (block 0: the paired kx = 0 and kxN when paired) and
+    // runs through the levels in groups of R: each wave transforms one level's
+    // y row (the wave FFT) into its LDS row, then each thread, one per ky,
+    // takes the group's R levels of its mode through the forward elimination
+    // (precomputed per system, see tdma_g) and stores them.  The spectrum is
+    // read and written once instead of twice.
     template<typename TF, typename TC, int R>
     __global__ void __launch_bounds__(64*R) ytri_fwd_g(cplx<TF>* __restrict__ C, const double2* __restrict__ tw,
                                                        const TC* __restrict__ wi, const TC* __restrict__ a,
                                                        const TC* __restrict__ dz2, int nx, int nsys, Gd g,
-                                                       bool skip_mean)
+                                                       bool skip_mean, bool paired)
     {
         using T = cplx<TF>;
         constexpr int n = 64*R, P_ = wave_passes<R>();
         __shared__ T rows[R][wave_row_elems<R>()];
         const int lane = threadIdx.x, wv = threadIdx.y, ky = lane + 64*wv;
-        const int kx = blockIdx.x, K = g.ktot;
+        const int kx = blockIdx.x, K = g.ktot, kxN = nx - 1;
+        const bool pair = paired && kx == 0;
         const size_t ks = static_cast<size_t>(nx)*n, ls = static_cast<size_t>(nsys);
-        const int l = kx*(n/2 + 1) + (ky <= n/2 ? ky : n - ky);
-        const bool zero_mode = skip_mean && kx == 0 && ky == 0;   // mixed mode: the mean is solved in FP64
+        int lx, ly;
+        pair_systems(pair, kx, kxN, ky, n, lx, ly);
+        // mixed mode: the mean is solved in FP64 (the x component of thread 0
+        // is mode (0, 0) also when paired)
+        const bool zero_mode = skip_mean && kx == 0 && ky == 0;
         T* A = rows[wv];
         T* __restrict__ Ck = C + static_cast<size_t>(kx)*n;      // C[k][kx][.] = Ck[k*ks + .]
+        const TF* __restrict__ C0r = reinterpret_cast<const TF*>(C);                           // paired: Re C[.][0][.]
+        const TF* __restrict__ CNr = reinterpret_cast<const TF*>(C + static_cast<size_t>(kxN)*n);  // and Re C[.][kxN][.]
         T wtw[P_-1][R-1];
         wave_twiddles<TF, R, -1>(wtw, tw, lane);
+        auto load_row = [&](T (&d)[R], size_t kr) {
+            if (pair)
+            {
+                #pragma unroll
+                for (int q = 0; q < R; ++q)
+                {
+                    const size_t o = 2*(kr*ks + lane + 64*q);
+                    d[q] = T{C0r[o], CNr[o]};
+                }
+            }
+            else
+            {
+                #pragma unroll
+                for (int q = 0; q < R; ++q) d[q] = Ck[kr*ks + lane + 64*q];
+            }
+        };
 
         T cur[R];
-        TC wcur[R];
+        TC wcur[R], wcury[R];
+        load_row(cur, min(wv, K-1));
         #pragma unroll
-        for (int q = 0; q < R; ++q) cur[q] = Ck[min(wv, K-1)*ks + lane + 64*q];
-        #pragma unroll
-        for (int m = 0; m < R; ++m) wcur[m] = wi[min(m, K-1)*ls + l];
+        for (int m = 0; m < R; ++m) { wcur[m] = wi[min(m, K-1)*ls + lx]; wcury[m] = wi[min(m, K-1)*ls + ly]; }
         TC px = TC(0), py = TC(0);
         for (int k0 = 0; k0 < K; k0 += R)
         {
             // the next group's row and pivots load while this group is solved
             T nxt[R];
-            TC wnxt[R];
+            TC wnxt[R], wnxty[R];
             const size_t kn = static_cast<size_t>(min(k0 + R + wv, K-1));
+            load_row(nxt, kn);
             #pragma unroll
-            for (int q = 0; q < R; ++q) nxt[q] = Ck[kn*ks + lane + 64*q];
-            #pragma unroll
-            for (int m = 0; m < R; ++m) wnxt[m] = wi[min(k0 + R + m, K-1)*ls + l];
+            for (int m = 0; m < R; ++m)
+            {
+                wnxt[m] = wi[min(k0 + R + m, K-1)*ls + lx];
+                wnxty[m] = wi[min(k0 + R + m, K-1)*ls + ly];
+            }
             // the y transform of level k0 + wv (every wave transforms; rows
             // past the top are clamped copies, never used).  FP32: the row
             // minus its FP64 mean, bin 0 set to the FP64 sum afterwards, as in
@@ -808,11 +855,18 @@ namespace
                 const int k = k0 + m;
                 if (k >= K) break;
                 const T q = rows[m][lpad(ky)];
-                const TC w = wcur[m];
-                TC qx = dz2[k]*TC(q.x), qy = dz2[k]*TC(q.y);
+                TC zx = TC(q.x), zy = TC(q.y);
+                if (pair && ky != 0 && 2*ky != n)
+                {
+                    // the thread's half-spectrum value from Z[ky] and Z[-ky]
+                    const T r = rows[m][lpad(n - ky)];
+                    if (2*ky < n) { zx = TC(0.5)*(zx + TC(r.x)); zy = TC(0.5)*(zy - TC(r.y)); }
+                    else { const TC dx = TC(r.x) - zx, dy = TC(r.y) + zy; zx = TC(0.5)*dy; zy = TC(-0.5)*dx; }
+                }
+                TC qx = dz2[k]*zx, qy = dz2[k]*zy;
                 if (k > 0) { qx -= a[k]*px; qy -= a[k]*py; }
-                qx *= w; qy *= w;
-                if (zero_mode) { qx = TC(0); qy = TC(0); }
+                qx *= wcur[m]; qy *= wcury[m];
+                if (zero_mode) { qx = TC(0); if (!pair) qy = TC(0); }
                 Ck[k*ks + ky] = T{TF(qx), TF(qy)};
                 px = qx; py = qy;
             }
@@ -820,7 +874,7 @@ namespace
             #pragma unroll
             for (int q = 0; q < R; ++q) cur[q] = nxt[q];
             #pragma unroll
-            for (int m = 0; m < R; ++m) wcur[m] = wnxt[m];
+            for (int m = 0; m < R; ++m) { wcur[m] = wnxt[m]; wcury[m] = wnxty[m]; }
         }
     }
 
@@ -831,39 +885,44 @@ namespace
     // src/fft.cxx:420 does) and stores it.
     template<typename TF, typename TC, int R>
     __global__ void __launch_bounds__(64*R) ytri_bwd_g(cplx<TF>* __restrict__ C, const double2* __restrict__ tw,
-                                                       const TC* __restrict__ gam, int nx, int nsys, Gd g)
+                                                       const TC* __restrict__ gam, int nx, int nsys, Gd g,
+                                                       bool paired)
     {
         using T = cplx<TF>;
         constexpr int n = 64*R, P_ = wave_passes<R>();
         __shared__ T rows[R][wave_row_elems<R>()];
         const int lane = threadIdx.x, wv = threadIdx.y, ky = lane + 64*wv;
-        const int kx = blockIdx.x, K = g.ktot;
+        const int kx = blockIdx.x, K = g.ktot, kxN = nx - 1;
+        const bool pair = paired && kx == 0;
         const size_t ks = static_cast<size_t>(nx)*n, ls = static_cast<size_t>(nsys);
-        const int l = kx*(n/2 + 1) + (ky <= n/2 ? ky : n - ky);
+        int lx, ly;
+        pair_systems(pair, kx, kxN, ky, n, lx, ly);
         T* A = rows[wv];
         T* __restrict__ Ck = C + static_cast<size_t>(kx)*n;
+        T* __restrict__ CN = C + static_cast<size_t>(kxN)*n;
         T wtw[P_-1][R-1];
         wave_twiddles<TF, R, +1>(wtw, tw, lane);
 
         // the forward sweep's value at the top level is the solution there
         TC px = TC(0), py = TC(0);
-        T qv[R]; TC gv[R];
-        auto load_group = [&](T (&qd)[R], TC (&gd)[R], int kt) {
+        T qv[R]; TC gv[R], gvy[R];
+        auto load_group = [&](T (&qd)[R], TC (&gd)[R], TC (&gdy)[R], int kt) {
             #pragma unroll
             for (int m = 0; m < R; ++m)
             {
                 const int k = max(kt - m, 0);
                 qd[m] = Ck[k*ks + ky];
-                gd[m] = gam[min(k + 1, K-1)*ls + l];
+                gd[m] = gam[min(k + 1, K-1)*ls + lx];
+                gdy[m] = gam[min(k + 1, K-1)*ls + ly];
             }
         };
-        load_group(qv, gv, K-1);
+        load_group(qv, gv, gvy, K-1);
         for (int k0 = K-1; k0 >= 0; k0 -= R)
         {
             // the next group's values load while this one is solved: the
             // levels below k0 - R + 1 are not written before their group
-            T qn[R]; TC gn[R];
-            load_group(qn, gn, k0 - R);
+            T qn[R]; TC gn[R], gny[R];
+            load_group(qn, gn, gny, k0 - R);
             // back substitution (src/pres_2.cxx:254-262) of levels k0, k0-1, ...
             #pragma unroll
             for (int m = 0; m < R; ++m)
@@ -871,15 +930,36 @@ namespace
                 const int k = k0 - m;
                 if (k < 0) break;
                 TC qx = TC(qv[m].x), qy = TC(qv[m].y);
-                if (k < K-1) { qx -= gv[m]*px; qy -= gv[m]*py; }
+                if (k < K-1) { qx -= gv[m]*px; qy -= gvy[m]*py; }
                 rows[m][lpad(ky)] = T{TF(qx), TF(qy)};
                 px = qx; py = qy;
             }
             __syncthreads();
             const int k = k0 - wv;
             T v[R];
-            #pragma unroll
-            for (int q = 0; q < R; ++q) v[q] = A[lpad(lane + 64*q)];
+            if (pair)
+            {
+                // W = P0 + i PN over all ky from the row's half spectra S:
+                // W[0], W[n/2] = S; 0 < ky < n/2: S[ky] + i S[-ky];
+                // n/2 < ky: conj S[-ky] + i conj S[ky]
+                #pragma unroll
+                for (int q = 0; q < R; ++q)
+                {
+                    const int x = lane + 64*q;
+                    const T sp = A[lpad(x)];
+                    if (x == 0 || 2*x == n) v[q] = sp;
+                    else
+                    {
+                        const T sm = A[lpad(n - x)];
+                        v[q] = 2*x < n ? T{sp.x - sm.y, sp.y + sm.x} : T{sm.x + sp.y, sp.x - sm.y};
+                    }
+                }
+            }
+            else
+            {
+                #pragma unroll
+                for (int q = 0; q < R; ++q) v[q] = A[lpad(lane + 64*q)];
+            }
             wave_lds_sync();
             wave_fft<TF, R, +1>(v, A, wtw, lane);
             if (k >= 0)
@@ -889,12 +969,20 @@ namespace
                 {
                     T z = v[q];
                     z.x *= TF(1)/TF(n); z.y *= TF(1)/TF(n);
-                    Ck[static_cast<size_t>(k)*ks + lane + 64*q] = z;
+                    const size_t o = static_cast<size_t>(k)*ks + lane + 64*q;
+                    if (pair)
+                    {
+                        // p0 + i pN, both real: the real rows kx = 0 and kxN
+                        Ck[o] = T{z.x, TF(0)};
+                        CN[o] = T{z.y, TF(0)};
+                    }
+                    else
+                        Ck[o] = z;
                 }
             }
             __syncthreads();
             #pragma unroll
-            for (int m = 0; m < R; ++m) { qv[m] = qn[m]; gv[m] = gn[m]; }
+            for (int m = 0; m < R; ++m) { qv[m] = qn[m]; gv[m] = gn[m]; gvy[m] = gny[m]; }
         }
     }
 
@@ -1372,10 +1460,13 @@ namespace
     void launch_ytri(mhh_ctx* c, cplx<TF>* C, const double2* twy, const Pres_tables<TC>& d, int nx, bool skip_mean)
     {
         const dim3 blk(64, R);
-        { mhh::Timer _t(c, "ytri_fwd"); ytri_fwd_g<TF, TC, R><<<nx, blk, 0, c->stream>>>(C, twy, d.tri_wi, d.a, d.dz2,
-                nx, d.nsys, c->gd, skip_mean); }
-        { mhh::Timer _t(c, "ytri_bwd"); ytri_bwd_g<TF, TC, R><<<nx, blk, 0, c->stream>>>(C, twy, d.tri_gam,
-                nx, d.nsys, c->gd); }
+        // even itot: kx = 0 and the last mode share block 0 (both are real)
+        const bool paired = c->gd.itot % 2 == 0 && nx >= 2;
+        const int nb = paired ? nx - 1 : nx;
+        { mhh::Timer _t(c, "ytri_fwd"); ytri_fwd_g<TF, TC, R><<<nb, blk, 0, c->stream>>>(C, twy, d.tri_wi, d.a, d.dz2,
+                nx, d.nsys, c->gd, skip_mean, paired); }
+        { mhh::Timer _t(c, "ytri_bwd"); ytri_bwd_g<TF, TC, R><<<nb, blk, 0, c->stream>>>(C, twy, d.tri_gam,
+                nx, d.nsys, c->gd, paired); }
     }
 
     template<typename TF, typename TC>

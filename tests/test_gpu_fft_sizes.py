@@ -100,15 +100,18 @@ def test_specialised_matches_general_engine(n):
         assert err < 1e-13, f"n={n} {name}: {err:.3e}"
 
 
-def test_mixed_specialised_matches_general_engine():
+@pytest.mark.parametrize("n", [256, 512])
+@pytest.mark.parametrize("prec", ["mixed", "fp32"])
+def test_float_specialised_matches_general_engine(prec, n):
     """The mixed mode (FP32 fields and transforms, FP64 plane means and
-    tridiagonal) through both transform engines: the same projection to FP32
-    rounding."""
+    tridiagonal) and plain FP32 through both transform engines: the same
+    projection to FP32 rounding."""
     from microhh_amd import abi
-    p, i = _inputs(256, seed=99)
+    precision = abi.MHH_MIXED if prec == "mixed" else abi.MHH_FP32
+    p, i = _inputs(n, seed=99)
     dt = 0.33
-    fast, _ = _gpu_pres(p, i, dt, generic=False, precision=abi.MHH_MIXED)
-    gen, _ = _gpu_pres(p, i, dt, generic=True, precision=abi.MHH_MIXED)
+    fast, _ = _gpu_pres(p, i, dt, generic=False, precision=precision)
+    gen, _ = _gpu_pres(p, i, dt, generic=True, precision=precision)
     for name in ("ut", "vt", "wt"):
         err = normwise(fast[name], gen[name])
         assert err < 1e-5, f"{name}: {err:.3e}"
