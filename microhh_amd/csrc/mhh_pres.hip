@@ -448,7 +448,7 @@ namespace
             const TF* __restrict__ u, const TF* __restrict__ v, const TF* __restrict__ w,
             const TF* __restrict__ ut, const TF* __restrict__ vt, const TF* __restrict__ wt,
             const TF* __restrict__ dzi, const TF* __restrict__ rho, const TF* __restrict__ rhoh,
-            TF dxi, TF dyi, TF dti, int cpx)
+            TF dxi, TF dyi, TF dti, int cpx, int packed)
     {
         using T = cplx<TF>;
         constexpr int XW = xwaves<TF>(), NT = 64*XW;
@@ -559,15 +559,24 @@ namespace
             for (int q = 0; q < XW; ++q) sm += wsum[q];
             part[static_cast<size_t>(k)*nbj + bj] = sm;
         }
-        // C[k][kx][j0 .. j0+JB): consecutive threads on consecutive j
+        // C[k][kx][j0 .. j0+JB): consecutive threads on consecutive j.  The
+        // real bins kx = 0 and n/2 go to C[k][0][j] as one complex value when
+        // packed (the layout of the paired y/tridiagonal kernels), else each
+        // to its own row with a zero imaginary part.
         T* Ck = C + static_cast<size_t>(k)*nx*M;
-        for (int idx = tid; idx < nx*JB; idx += NT)
+        const int nkx = packed ? nx - 1 : nx;
+        for (int idx = tid; idx < nkx*JB; idx += NT)
         {
             const int kx = idx / JB, jl = idx % JB;
             const int r = jl >> 1, part = jl & 1;
             const T* Sr = rows[r];
             T val;
-            if (kx == 0 || 2*kx == n)
+            if (kx == 0 && packed)
+            {
+                const T d0 = Sr[0], dn = Sr[1];   // lpad(0) = 0, lpad(1) = 1
+                val.x = part ? d0.y : d0.x; val.y = part ? dn.y : dn.x;
+            }
+            else if (kx == 0 || 2*kx == n)
             {
                 const T d = Sr[kx == 0 ? 0 : 1];
                 val.x = part ? d.y : d.x; val.y = TF(0);
@@ -584,7 +593,8 @@ namespace
     template<typename TF, int R>
     __global__ void __launch_bounds__(64*xwaves<TF>()) fft_x_bwd_wave_g(const cplx<TF>* __restrict__ C,
                                                                          TF* __restrict__ p,
-                                                                         const double2* __restrict__ tw, Gd g)
+                                                                         const double2* __restrict__ tw, Gd g,
+                                                                         int packed)
     {
         using T = cplx<TF>;
         constexpr int XW = xwaves<TF>(), NT = 64*XW;
@@ -597,15 +607,22 @@ namespace
         T wtw[P_-1][R-1];
         wave_twiddles<TF, R, +1>(wtw, tw, lane);
 
-        // stage the block's spectra packed as in the forward kernel
+        // stage the block's spectra packed as in the forward kernel (packed:
+        // bins 0 and n/2 both in C[k][0][j])
         const T* Ck = C + static_cast<size_t>(k)*nx*M;
-        for (int idx = tid; idx < nx*JB; idx += NT)
+        const int nkx = packed ? nx - 1 : nx;
+        for (int idx = tid; idx < nkx*JB; idx += NT)
         {
             const int kx = idx / JB, jl = idx % JB;
             const int r = jl >> 1, part = jl & 1;
             const T val = Ck[static_cast<size_t>(kx)*M + j0 + jl];
             T* Sr = rows[r];
-            if (kx == 0 || 2*kx == n)
+            if (kx == 0 && packed)
+            {
+                reinterpret_cast<TF*>(&Sr[0])[part] = val.x;
+                reinterpret_cast<TF*>(&Sr[1])[part] = val.y;
+            }
+            else if (kx == 0 || 2*kx == n)
             {
                 TF* d = reinterpret_cast<TF*>(&Sr[kx == 0 ? 0 : 1]);
                 d[part] = val.x;
@@ -738,14 +755,15 @@ namespace
         }
     }
 
-    // Paired modes.  For an even itot the x spectrum at kx = 0 and at the
-    // last mode kxN = itot/2 is real, so block 0 of the fused y/tridiagonal
-    // kernels takes both (nx - 1 blocks instead of nx, which would leave one
-    // block to run after all the others on a 256-CU device): it transforms
-    // Z = C[.][0][j] + i C[.][kxN][j] as one complex row.  Both y spectra are
-    // Hermitian, as are the solutions (the systems depend on ky only through
-    // bmatj[ky] = bmatj[jtot-ky]), so their independent halves fill one row
-    // of jtot slots, the layout row kx = 0 then holds between the two kernels:
+    // Paired modes.  The x spectrum at kx = 0 and at the last mode kxN =
+    // itot/2 is real, and the wave x transforms can store the two packed as
+    // Z = X0 + i XN in row kx = 0; block 0 of the fused y/tridiagonal kernels
+    // then takes both (nx - 1 blocks instead of nx, which would leave one
+    // block to run after all the others on a 256-CU device), transforming Z
+    // as one complex row.  Both y spectra are Hermitian, as are the solutions
+    // (the systems depend on ky only through bmatj[ky] = bmatj[jtot-ky]), so
+    // their independent halves fill one row of jtot slots, the layout row
+    // kx = 0 holds between the two kernels:
     //   slot 0, jtot/2    the real pair (mode (0, ky), mode (kxN, ky)) = Z[ky]
     //   0 < ky < jtot/2   mode (0, ky)         = (Z[ky] + conj Z[-ky])/2
     //   jtot/2 < ky       mode (kxN, jtot-ky)  = (Z[-ky] - conj Z[ky])/(2i)
@@ -787,25 +805,11 @@ namespace
         const bool zero_mode = skip_mean && kx == 0 && ky == 0;
         T* A = rows[wv];
         T* __restrict__ Ck = C + static_cast<size_t>(kx)*n;      // C[k][kx][.] = Ck[k*ks + .]
-        const TF* __restrict__ C0r = reinterpret_cast<const TF*>(C);                           // paired: Re C[.][0][.]
-        const TF* __restrict__ CNr = reinterpret_cast<const TF*>(C + static_cast<size_t>(kxN)*n);  // and Re C[.][kxN][.]
         T wtw[P_-1][R-1];
         wave_twiddles<TF, R, -1>(wtw, tw, lane);
         auto load_row = [&](T (&d)[R], size_t kr) {
-            if (pair)
-            {
-                #pragma unroll
-                for (int q = 0; q < R; ++q)
-                {
-                    const size_t o = 2*(kr*ks + lane + 64*q);
-                    d[q] = T{C0r[o], CNr[o]};
-                }
-            }
-            else
-            {
-                #pragma unroll
-                for (int q = 0; q < R; ++q) d[q] = Ck[kr*ks + lane + 64*q];
-            }
+            #pragma unroll
+            for (int q = 0; q < R; ++q) d[q] = Ck[kr*ks + lane + 64*q];
         };
 
         T cur[R];
@@ -899,7 +903,6 @@ namespace
         pair_systems(pair, kx, kxN, ky, n, lx, ly);
         T* A = rows[wv];
         T* __restrict__ Ck = C + static_cast<size_t>(kx)*n;
-        T* __restrict__ CN = C + static_cast<size_t>(kxN)*n;
         T wtw[P_-1][R-1];
         wave_twiddles<TF, R, +1>(wtw, tw, lane);
 
@@ -967,17 +970,10 @@ namespace
                 #pragma unroll
                 for (int q = 0; q < R; ++q)
                 {
+                    // paired: p0 + i pN, the two real x bins packed in row 0
                     T z = v[q];
                     z.x *= TF(1)/TF(n); z.y *= TF(1)/TF(n);
-                    const size_t o = static_cast<size_t>(k)*ks + lane + 64*q;
-                    if (pair)
-                    {
-                        // p0 + i pN, both real: the real rows kx = 0 and kxN
-                        Ck[o] = T{z.x, TF(0)};
-                        CN[o] = T{z.y, TF(0)};
-                    }
-                    else
-                        Ck[o] = z;
+                    Ck[static_cast<size_t>(k)*ks + lane + 64*q] = z;
                 }
             }
             __syncthreads();
@@ -1457,11 +1453,12 @@ namespace
 
     // Fused y transforms and tridiagonal sweep (jtot = 64 R, R = 2, 4, 8).
     template<typename TF, typename TC, int R>
-    void launch_ytri(mhh_ctx* c, cplx<TF>* C, const double2* twy, const Pres_tables<TC>& d, int nx, bool skip_mean)
+    void launch_ytri(mhh_ctx* c, cplx<TF>* C, const double2* twy, const Pres_tables<TC>& d, int nx, bool skip_mean,
+                     bool paired)
     {
         const dim3 blk(64, R);
-        // even itot: kx = 0 and the last mode share block 0 (both are real)
-        const bool paired = c->gd.itot % 2 == 0 && nx >= 2;
+        // paired (the x transform stored bins 0 and itot/2 packed in row 0):
+        // block 0 takes both
         const int nb = paired ? nx - 1 : nx;
         { mhh::Timer _t(c, "ytri_fwd"); ytri_fwd_g<TF, TC, R><<<nb, blk, 0, c->stream>>>(C, twy, d.tri_wi, d.a, d.dz2,
                 nx, d.nsys, c->gd, skip_mean, paired); }
@@ -1470,13 +1467,14 @@ namespace
     }
 
     template<typename TF, typename TC>
-    int ytri(mhh_ctx* c, cplx<TF>* C, const double2* twy, const Pres_tables<TC>& d, int nx, bool skip_mean)
+    int ytri(mhh_ctx* c, cplx<TF>* C, const double2* twy, const Pres_tables<TC>& d, int nx, bool skip_mean,
+             bool paired)
     {
         switch (c->gd.jtot)
         {
-            case 512: launch_ytri<TF, TC, 8>(c, C, twy, d, nx, skip_mean); break;
-            case 256: launch_ytri<TF, TC, 4>(c, C, twy, d, nx, skip_mean); break;
-            default:  launch_ytri<TF, TC, 2>(c, C, twy, d, nx, skip_mean); break;
+            case 512: launch_ytri<TF, TC, 8>(c, C, twy, d, nx, skip_mean, paired); break;
+            case 256: launch_ytri<TF, TC, 4>(c, C, twy, d, nx, skip_mean, paired); break;
+            default:  launch_ytri<TF, TC, 2>(c, C, twy, d, nx, skip_mean, paired); break;
         }
         return check_launch("ytri");
     }
@@ -1485,7 +1483,7 @@ namespace
     // R = 2, 4, 8; jtot a multiple of 8).
     template<typename TF, int R>
     void launch_fft_x_fwd_wave(mhh_ctx* c, const TF* P, cplx<TF>* C, const double2* tw, int mode,
-                               const Prof<TF>& p, TF dxi, TF dyi, TF dti)
+                               const Prof<TF>& p, TF dxi, TF dyi, TF dti, int packed)
     {
         const Gd& g = c->gd;
         const int nbj = g.jtot/(2*xwaves<TF>());
@@ -1495,7 +1493,7 @@ namespace
         const TF* w = static_cast<const TF*>(c->f.w); const TF* ut = static_cast<const TF*>(c->f.ut);
         const TF* vt = static_cast<const TF*>(c->f.vt); const TF* wt = static_cast<const TF*>(c->f.wt);
 #define MHH_XF(MODE) fft_x_fwd_wave_g<TF, R, MODE><<<grd, blk, 0, c->stream>>>(P, C, tw, g, c->pm_mean, c->pm_part, \
-                u, v, w, ut, vt, wt, p.dzi, p.rhoref, p.rhorefh, dxi, dyi, dti, cpx)
+                u, v, w, ut, vt, wt, p.dzi, p.rhoref, p.rhorefh, dxi, dyi, dti, cpx, packed)
         if (mode == 3) MHH_XF(3);
         else if (mode == 2) MHH_XF(2);
         else if (mode == 1) MHH_XF(1);
@@ -1505,23 +1503,23 @@ namespace
 
     template<typename TF>
     int fft_x_fwd_wave(mhh_ctx* c, const TF* P, cplx<TF>* C, const double* tw, int mode,
-                       const Prof<TF>& p, TF dxi, TF dyi, TF dti, const char* name)
+                       const Prof<TF>& p, TF dxi, TF dyi, TF dti, const char* name, int packed)
     {
         const double2* t2 = reinterpret_cast<const double2*>(tw);
         {
             mhh::Timer _t(c, name);
             switch (c->gd.itot)
             {
-                case 512: launch_fft_x_fwd_wave<TF, 8>(c, P, C, t2, mode, p, dxi, dyi, dti); break;
-                case 256: launch_fft_x_fwd_wave<TF, 4>(c, P, C, t2, mode, p, dxi, dyi, dti); break;
-                default:  launch_fft_x_fwd_wave<TF, 2>(c, P, C, t2, mode, p, dxi, dyi, dti); break;
+                case 512: launch_fft_x_fwd_wave<TF, 8>(c, P, C, t2, mode, p, dxi, dyi, dti, packed); break;
+                case 256: launch_fft_x_fwd_wave<TF, 4>(c, P, C, t2, mode, p, dxi, dyi, dti, packed); break;
+                default:  launch_fft_x_fwd_wave<TF, 2>(c, P, C, t2, mode, p, dxi, dyi, dti, packed); break;
             }
         }
         return check_launch(name);
     }
 
     template<typename TF>
-    int fft_x_bwd_wave(mhh_ctx* c, const cplx<TF>* C, TF* P, const double* tw)
+    int fft_x_bwd_wave(mhh_ctx* c, const cplx<TF>* C, TF* P, const double* tw, int packed)
     {
         const Gd& g = c->gd;
         const double2* t2 = reinterpret_cast<const double2*>(tw);
@@ -1530,9 +1528,9 @@ namespace
             mhh::Timer _t(c, "fft_x_bwd");
             switch (g.itot)
             {
-                case 512: fft_x_bwd_wave_g<TF, 8><<<grd, blk, 0, c->stream>>>(C, P, t2, g); break;
-                case 256: fft_x_bwd_wave_g<TF, 4><<<grd, blk, 0, c->stream>>>(C, P, t2, g); break;
-                default:  fft_x_bwd_wave_g<TF, 2><<<grd, blk, 0, c->stream>>>(C, P, t2, g); break;
+                case 512: fft_x_bwd_wave_g<TF, 8><<<grd, blk, 0, c->stream>>>(C, P, t2, g, packed); break;
+                case 256: fft_x_bwd_wave_g<TF, 4><<<grd, blk, 0, c->stream>>>(C, P, t2, g, packed); break;
+                default:  fft_x_bwd_wave_g<TF, 2><<<grd, blk, 0, c->stream>>>(C, P, t2, g, packed); break;
             }
         }
         return check_launch("fft_x_bwd");
@@ -1588,11 +1586,16 @@ namespace
                     c->pbar, c->dpbar, d.a, d.dz2, c->pm_coef, 1./(double(N)*double(M)), g); }
             MHH_CHECK(check_launch("mean_mode"));
         }
+        // y transforms and tridiagonal sweep fused for jtot = 128, 256, 512;
+        // with the wave x transforms as well, the real x bins 0 and itot/2
+        // travel packed in row kx = 0 and share a block there
+        const bool ywave = !c->fft_generic && !c->ytri_split && (M == 128 || M == 256 || M == 512);
+        const int packed = xwave && ywave ? 1 : 0;
         // 2. forward x transform
         if (xwave)
         {
             const int mode = c->mixed ? 3 : 2;
-            MHH_CHECK(fft_x_fwd_wave<TF>(c, P, C, t.twx, mode, p, dxi, dyi, dti, "pres_in_fft_x"));
+            MHH_CHECK(fft_x_fwd_wave<TF>(c, P, C, t.twx, mode, p, dxi, dyi, dti, "pres_in_fft_x", packed));
             if (c->mixed)
             {
                 { mhh::Timer _t(c, "pres_mean"); mean_mode_g<<<1, 256, mm_lds, c->stream>>>(c->pm_part, M/(2*xwaves<TF>()), c->pm_mean,
@@ -1610,7 +1613,6 @@ namespace
             MHH_CHECK(check_launch("fft_x_fwd"));
         }
         const int nrows = nx*K;
-        const bool ywave = !c->fft_generic && !c->ytri_split && (M == 128 || M == 256 || M == 512);
         if (ywave)
         {
             // 3-5. y transform, tridiagonal sweep, inverse y transform: two
@@ -1619,10 +1621,10 @@ namespace
             if (c->mixed)
             {
                 const Pres_tables<double>& d = c->td;
-                MHH_CHECK((ytri<TF, double>(c, C, twy, d, nx, true)));
+                MHH_CHECK((ytri<TF, double>(c, C, twy, d, nx, true, packed)));
             }
             else
-                MHH_CHECK((ytri<TF, TF>(c, C, twy, t, nx, false)));
+                MHH_CHECK((ytri<TF, TF>(c, C, twy, t, nx, false, packed)));
         }
         else
         {
@@ -1651,7 +1653,7 @@ namespace
         }
         // 6. inverse x transform (divides by itot) into the ghosted p with halos
         if (xwave)
-            MHH_CHECK(fft_x_bwd_wave<TF>(c, C, P, t.twx));
+            MHH_CHECK(fft_x_bwd_wave<TF>(c, C, P, t.twx, packed));
         else
         {
             const int RB = rows_per_block(px, sizeof(T));
