@@ -70,12 +70,12 @@ KERNELS = list(PASSES) + ["cyclic", "ghost_cells", "mirror_walls", "rk3_zero_gho
 # workload and the dominant kernel match.
 MEASURED_TRAFFIC = {
     (512, 512, 256, 8): {
-        # mean over a step's three launches: (7485.4 + 2 x 10861.9)/3 (the
+        # mean over a step's three launches: (7538.0 + 2 x 10804.5)/3 (the
         # first substep writes the tendencies without reading them)
-        "tend_2i5_smag2": 9736.4e6, "pres_out_rk3": 7606.9e6, "pres_in_fft_x": 5137.3e6,
-        # one launch with the CFL/dnmul maxima per three: (4358.1 + 2 x 3241.2)/3
-        "evisc": 3613.5e6,
-        "ytri_fwd": 1227.2e6, "ytri_bwd": 1225.0e6, "fft_x_bwd": 1091.5e6,
+        "tend_2i5_smag2": 9715.7e6, "pres_out_rk3": 7606.8e6, "pres_in_fft_x": 5133.3e6,
+        # one launch with the CFL/dnmul maxima per three: (4362.1 + 2 x 3230.9)/3
+        "evisc": 3608.0e6,
+        "ytri_fwd": 1222.3e6, "ytri_bwd": 1219.2e6, "fft_x_bwd": 1089.4e6,
     },
 }
 # Steps of the pass that times every kernel (before the timed steps).
