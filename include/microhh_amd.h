@@ -212,6 +212,16 @@ int  mhh_pres_2_divergence(mhh_ctx* ctx, double* divmax);
  * buoyancy of scalar 0 on wt.  Scalars beyond the first are not touched.
  * A pending deferred update (MHH_RK3_DEFER) is completed in the same pass. */
 int  mhh_tendency_2i5_smag2(mhh_ctx* ctx, const mhh_smag_params* par, int buoyancy);
+/* The same pass, also returning the two time-step inputs of
+ * Model::set_time_step (src/model.cxx:764-786) from the fields it reads: the
+ * 2i5 CFL per unit time step (Advec_2i5::get_cfl, src/advec_2i5.cxx:954-971)
+ * and dnmul (Diff_smag2::get_time_limit, include/diff_kernels.h:486-511),
+ * the values mhh_diff_smag2_exec_viscosity_limits returns.  u, v, w and evisc
+ * are the same when the tendencies are computed, and the reference uses dt
+ * first in the projection, so a first substep may set its time step from
+ * here. */
+int  mhh_tendency_2i5_smag2_limits(mhh_ctx* ctx, const mhh_smag_params* par, int buoyancy,
+                                   double* cfl_rate, double* dnmul);
 
 /* ---- time integration ---- */
 /* Timeloop::exec, rkorder 3 (src/timeloop.cxx:358-412, 251-286) on u, v, w

@@ -13,6 +13,8 @@
 // the +-3 vertical neighbours of the thread's own column come from register
 // queues.  Block ids are swizzled so that the tiles one XCD works on are
 // neighbours: their halos are then served by that XCD's L2.
+#include <algorithm>
+
 #include "mhh_internal.h"
 #include "mhh_stencil.h"
 
@@ -35,6 +37,7 @@ namespace
     struct Tend_args
     {
         TF dxi, dyi, dxidxi, dyidyi, visc, svisc, tPr_i;
+        TF tPrfac_i;    // LIM: 1/min(1, tPr) of dnmul
         TF cA;          // TEND_DEFER: the RK3 factor of the deferred update
         int buoyancy;
         int nti, ntj, nkc, kchunk;
@@ -168,22 +171,31 @@ namespace
         }
     }
 
-    template<typename TF, int MODE>
+    // LIM: the pass also takes the two time-step inputs of the substep from
+    // the fields it holds anyway, as block maxima in red_cfl / red_dn: the
+    // 2i5 CFL rate (Advec_2i5::get_cfl without dt, src/advec_2i5.cxx:954-971;
+    // u, v from the halo-3 planes, w from the column queue) and dnmul
+    // (include/diff_kernels.h:486-511, from evisc at the point).
+    template<typename TF, int MODE, bool LIM>
     __global__ void __launch_bounds__(TX*TY, 2) tend_2i5_smag2_g(
             TF* __restrict__ ut, TF* __restrict__ vt, TF* __restrict__ wt, TF* __restrict__ st,
             const TF* __restrict__ u, const TF* __restrict__ v, const TF* __restrict__ w,
             const TF* __restrict__ s, const TF* __restrict__ evisc,
             const Level<TF>* __restrict__ lev, const TF* __restrict__ p, const TF* __restrict__ dpbar,
-            Tend_args<TF> a, Gd g)
+            Tend_args<TF> a, Gd g, double* __restrict__ red_cfl, double* __restrict__ red_dn)
     {
-        __shared__ TF lds[7*PN + 3*EN];
+        __shared__ __align__(16) TF lds[7*PN + 3*EN];
 
         // XCD-aware tile order: block b runs on XCD b % 8; give each XCD a
         // contiguous range of tiles (i fastest, then j, then the k chunk).
         const int nb = gridDim.x;
         const int ntiles = a.nti*a.ntj*a.nkc;
         const int tile = (blockIdx.x % 8)*(nb/8) + blockIdx.x/8;
-        if (tile >= ntiles) return;
+        if (tile >= ntiles)
+        {
+            if (LIM && threadIdx.x == 0 && threadIdx.y == 0) { red_cfl[1 + blockIdx.x] = 0.; red_dn[1 + blockIdx.x] = 0.; }
+            return;
+        }
         const int ti = tile % a.nti;
         const int tj = (tile / a.nti) % a.ntj;
         const int kc = tile / (a.nti*a.ntj);
@@ -273,6 +285,7 @@ namespace
         // TEND_DEFER: p of the own column one level down, carried from level
         // to level
         TF pb = MODE == TEND_DEFER ? ldo(p + size_t(kb-1)*kk, oij) : TF(0);
+        double cm = 0., dm = 0.;   // LIM: running maxima of the CFL rate and dnmul
         // Every load of the preamble has landed: the waits inside the loop then
         // only count the loads issued in the loop.
         __builtin_amdgcn_s_waitcnt(0);
@@ -444,6 +457,22 @@ namespace
                 bs_c = tc; bs_u = tq; ds_b = dt;
                 if (active) sto(stk, oij, ts);
             }
+            if (LIM && active)
+            {
+                // the expressions of visc_tile_g's limits, w at k-2 .. k+3 = qw[1 .. 6]
+                dm = fmax(dm, static_cast<double>(fabs(Ec[ce]*a.tPrfac_i*(a.dxidxi + a.dyidyi + L.dzi*L.dzi))));
+                TF wpart;
+                if (k == g.kstart || k == g.kend-1)
+                    wpart = fabs(i2(qw[3], qw[4]))*L.dzi;
+                else if (k == g.kstart+1 || k == g.kend-2)
+                    wpart = fabs(i4ws(qw[2], qw[3], qw[4], qw[5]))*L.dzi;
+                else
+                    wpart = fabs(i6ws(qw[1], qw[2], qw[3], qw[4], qw[5], qw[6]))*L.dzi;
+                const TF val = fabs(i6ws(Uc[c-2], Uc[c-1], Uc[c], Uc[c+1], Uc[c+2], Uc[c+3]))*dxi
+                             + fabs(i6ws(Vc[c-2*PW], Vc[c-PW], Vc[c], Vc[c+PW], Vc[c+2*PW], Vc[c+3*PW]))*dyi
+                             + wpart;
+                cm = fmax(cm, static_cast<double>(val));
+            }
             __syncthreads();
 
             // rotate the plane ring, store the prefetched planes, advance the queues
@@ -464,10 +493,26 @@ namespace
             }
             qu[6] = nu; qv[6] = nv; qw[6] = nw; qs[6] = ns;
         }
+        if (LIM)
+        {
+            // block maxima, in the plane ring's LDS (no longer read)
+            __syncthreads();
+            double* smc = reinterpret_cast<double*>(lds);
+            double* smd = smc + TX*TY;
+            smc[tid] = cm; smd[tid] = dm;
+            __syncthreads();
+            for (int st2 = TX*TY/2; st2 > 0; st2 >>= 1)
+            {
+                if (tid < st2) { smc[tid] = fmax(smc[tid], smc[tid + st2]); smd[tid] = fmax(smd[tid], smd[tid + st2]); }
+                __syncthreads();
+            }
+            if (tid == 0) { red_cfl[1 + blockIdx.x] = smc[0]; red_dn[1 + blockIdx.x] = smd[0]; }
+        }
     }
 
     template<typename TF>
-    int tend(mhh_ctx* c, const Prof<TF>& p, const mhh_smag_params& par, int buoyancy)
+    int tend(mhh_ctx* c, const Prof<TF>& p, const mhh_smag_params& par, int buoyancy,
+             double* cfl_rate = nullptr, double* dnmul = nullptr)
     {
         const Gd& g = c->gd;
         if (c->f.nscalars < 1)
@@ -484,6 +529,7 @@ namespace
         a.dxi = TF(1.)/TF(g.dx); a.dyi = TF(1.)/TF(g.dy);
         a.dxidxi = TF(1./(g.dx*g.dx)); a.dyidyi = TF(1./(g.dy*g.dy));
         a.visc = TF(par.visc); a.svisc = TF(par.svisc[0]); a.tPr_i = TF(1)/TF(par.tPr);
+        a.tPrfac_i = TF(1)/std::min(TF(1.), TF(par.tPr));
         a.buoyancy = buoyancy;
         // a pending deferred RK3 update is completed in this pass
         const int mode = !c->defer_pending ? TEND_ADD : c->defer_cA == 0. ? TEND_FRESH : TEND_DEFER;
@@ -497,24 +543,56 @@ namespace
         a.nkc = (g.ktot + a.kchunk - 1)/a.kchunk;
         const int ntiles = a.nti*a.ntj*a.nkc;
         const int nblocks = (ntiles + 7)/8*8;
+        const bool lim = cfl_rate && dnmul;
+        double* rc = nullptr; double* rd = nullptr;
+        if (lim)
+        {
+            MHH_CHECK(ensure_reduction(c, 2*nblocks + 1));
+            rc = c->red; rd = c->red + nblocks + 1;
+        }
         { mhh::Timer _t(c, "tend_2i5_smag2");
-#define MHH_TEND(M) tend_2i5_smag2_g<TF, M><<<nblocks, dim3(TX, TY), 0, c->stream>>>( \
+#define MHH_TEND(M, L) tend_2i5_smag2_g<TF, M, L><<<nblocks, dim3(TX, TY), 0, c->stream>>>( \
                 static_cast<TF*>(c->f.ut), static_cast<TF*>(c->f.vt), static_cast<TF*>(c->f.wt), \
                 static_cast<TF*>(c->f.st[0]), \
                 static_cast<const TF*>(c->f.u), static_cast<const TF*>(c->f.v), static_cast<const TF*>(c->f.w), \
                 static_cast<const TF*>(c->f.s[0]), static_cast<const TF*>(c->f.evisc), p.lev, \
-                static_cast<const TF*>(c->f.p), dpb, a, g)
-          if (mode == TEND_DEFER) MHH_TEND(TEND_DEFER);
-          else if (mode == TEND_FRESH) MHH_TEND(TEND_FRESH);
-          else MHH_TEND(TEND_ADD);
+                static_cast<const TF*>(c->f.p), dpb, a, g, rc, rd)
+          if (lim)
+          {
+              if (mode == TEND_DEFER) MHH_TEND(TEND_DEFER, true);
+              else if (mode == TEND_FRESH) MHH_TEND(TEND_FRESH, true);
+              else MHH_TEND(TEND_ADD, true);
+          }
+          else
+          {
+              if (mode == TEND_DEFER) MHH_TEND(TEND_DEFER, false);
+              else if (mode == TEND_FRESH) MHH_TEND(TEND_FRESH, false);
+              else MHH_TEND(TEND_ADD, false);
+          }
 #undef MHH_TEND
         }
         c->defer_pending = false;
-        return check_launch("tend_2i5_smag2");
+        MHH_CHECK(check_launch("tend_2i5_smag2"));
+        if (!lim) return 0;
+        MHH_CHECK(reduce_max_partials_at(c, rc, nblocks, cfl_rate));
+        return reduce_max_partials_at(c, rd, nblocks, dnmul);
     }
 }
 
 extern "C" {
+
+int mhh_tendency_2i5_smag2_limits(mhh_ctx* c, const mhh_smag_params* par, int buoyancy,
+                                  double* cfl_rate, double* dnmul)
+{
+    MHH_REQUIRE_BOUND(c);
+    if (!par || !cfl_rate || !dnmul) return mhh::fail(MHH_EINVAL, "null argument");
+    if (!c->f.evisc) return mhh::fail(MHH_ESTATE, "evisc field is not bound");
+    if (buoyancy && !c->have_base_state) return mhh::fail(MHH_ESTATE, "buoyancy needs mhh_set_base_state()");
+    if (c->fluxlimit[0])
+        return mhh::fail(MHH_ESTATE, "the fused tendency has no flux limiter: use mhh_advec_2i5_exec for limited scalars");
+    return c->precision == MHH_FP64 ? tend<double>(c, c->pd, *par, buoyancy, cfl_rate, dnmul)
+                                    : tend<float>(c, c->pf, *par, buoyancy, cfl_rate, dnmul);
+}
 
 int mhh_tendency_2i5_smag2(mhh_ctx* c, const mhh_smag_params* par, int buoyancy)
 {

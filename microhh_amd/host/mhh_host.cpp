@@ -729,48 +729,35 @@ namespace mhh_host
         // The in-scope part of Model::exec (src/model.cxx:368-507).
         boundary->set_prognostic_cyclic_bcs();
         boundary->set_ghost_cells();
-        if (fused_tendency && !timeloop->in_substep())
-        {
-            // Diff_smag2::exec_viscosity with the two time-step inputs of the
-            // first substep in the same pass
-            const auto& par = static_cast<Diff_smag2<TF>&>(*diff).params();
-            double cfl_rate, dnmul;
-            check(mhh_diff_smag2_exec_viscosity_limits(ctx_, &par, &cfl_rate, &dnmul),
-                  "mhh_diff_smag2_exec_viscosity_limits");
-            set_time_step_from(cfl_rate, dnmul);
-        }
-        else
-        {
-            diff->exec_viscosity();
-            set_time_step();
-        }
-        const double subdt = timeloop->get_sub_time_step();
+        diff->exec_viscosity();
         if (fused_tendency)
         {
             // Thermo_dry::exec, Advec_2i5::exec and Diff_smag2::exec in one
             // kernel, terms added in the same order.  The ghost-cell update
             // that the reference runs between buoyancy and advection only
-            // rewrites the ghost levels, which buoyancy does not read.
+            // rewrites the ghost levels, which buoyancy does not read.  On
+            // the first substep of a step the same pass returns the CFL and
+            // dnmul maxima of Model::set_time_step: u, v, w and evisc are
+            // those set_time_step would read, and dt is first used by the
+            // projection.
             boundary->set_ghost_cells();
             const auto& par = static_cast<Diff_smag2<TF>&>(*diff).params();
-            check(mhh_tendency_2i5_smag2(ctx_, &par, 1), "mhh_tendency_2i5_smag2");
-        }
-        else
-        {
-            if (thermo)
-                thermo->exec(subdt);
-            boundary->set_ghost_cells();
-            advec->exec();
-            diff->exec();
-        }
-        if (fused_tendency)
-        {
+            if (!timeloop->in_substep())
+            {
+                double cfl_rate, dnmul;
+                check(mhh_tendency_2i5_smag2_limits(ctx_, &par, 1, &cfl_rate, &dnmul),
+                      "mhh_tendency_2i5_smag2_limits");
+                set_time_step_from(cfl_rate, dnmul);
+            }
+            else
+                check(mhh_tendency_2i5_smag2(ctx_, &par, 1), "mhh_tendency_2i5_smag2");
             // Pres_2::exec with the RK3 update (Timeloop::exec) of u, v, w
             // and the scalars in its last kernel.  The tendencies' share of
             // the update, cA (at - grad p), is left to the next substep's
             // tendency pass (or mhh_tendency_settle, for a caller that reads
             // them): the pass reads them anyway, and with cA = 0 (the end of
             // a step) it writes them without reading.
+            const double subdt = timeloop->get_sub_time_step();
             check(mhh_pres_2_exec_rk3(ctx_, subdt, timeloop->get_substep(), timeloop->get_dt(),
                                       MHH_RK3_SCALARS | MHH_RK3_DEFER),
                   "mhh_pres_2_exec_rk3");
@@ -778,6 +765,13 @@ namespace mhh_host
         }
         else
         {
+            set_time_step();
+            const double subdt = timeloop->get_sub_time_step();
+            if (thermo)
+                thermo->exec(subdt);
+            boundary->set_ghost_cells();
+            advec->exec();
+            diff->exec();
             pres->exec(subdt);
             timeloop->exec();
         }

@@ -454,6 +454,31 @@ def test_viscosity_limits_match_separate_calls(setup, thermo):
     assert abs(dn.value - dn1.value) <= 1e-15 * dn1.value
 
 
+def test_tendency_limits_match_viscosity_limits(setup):
+    """mhh_tendency_2i5_smag2_limits: the same tendencies as the plain fused
+    pass, and the CFL rate and dnmul that the viscosity pass with limits
+    returns for the same u, v, w and evisc."""
+    from microhh_amd import abi
+    p, i, d, g, *_ = setup
+    par = abi.SmagParams(0.23, 1.0 / 3.0, 1.5e-5, (C.c_double * 8)(*([2e-5] * 8)), 1, 0, 1)
+    names = ("ut", "vt", "wt", "st0")
+    _load_common(g, i)
+    cfl_v, dn_v = C.c_double(), C.c_double()
+    g.call("mhh_diff_smag2_exec_viscosity_limits", C.byref(par), C.byref(cfl_v), C.byref(dn_v))
+    ev = g.get("evisc")
+    cfl_t, dn_t = C.c_double(), C.c_double()
+    g.call("mhh_tendency_2i5_smag2_limits", C.byref(par), 1, C.byref(cfl_t), C.byref(dn_t))
+    lim = {n: g.get(n) for n in names}
+    _load_common(g, i)
+    g.put("evisc", ev)
+    g.call("mhh_tendency_2i5_smag2", C.byref(par), 1)
+    for n in names:
+        err = normwise(lim[n], g.get(n))
+        assert err < 1e-14, f"{n}: {err:.3e}"
+    assert abs(cfl_t.value - cfl_v.value) <= 1e-14 * cfl_v.value, (cfl_t.value, cfl_v.value)
+    assert abs(dn_t.value - dn_v.value) <= 1e-14 * dn_v.value, (dn_t.value, dn_v.value)
+
+
 def test_fused_tendency_matches_operator_sequence(setup):
     """mhh_tendency_2i5_smag2 against thermo_dry + advec_2i5 + diff_smag2 run
     one after another through the ABI, then against the oracle chain."""
