@@ -23,6 +23,7 @@
 
 #include "mhh_internal.h"
 #include "mhh_fft.h"
+#include "mhh_stencil.h"
 
 namespace
 {
@@ -785,10 +786,39 @@ namespace
     // takes the group's R levels of its mode through the forward elimination
     // (precomputed per system, see tdma_g) and stores them.  The spectrum is
     // read and written once instead of twice.
+    // Reciprocal pivots of the elimination of one system, level by level, as
+    // setup_tables computes them (the same roundings: the products are not
+    // fused into the subtractions, the division is IEEE), so they equal its
+    // tri_wi entries: the forward sweep needs no table.
+    template<typename TC>
+    struct Pivots
+    {
+        TC lam, w;
+        bool mean;
+        __device__ __forceinline__ void first(const TC* __restrict__ av, const TC* __restrict__ acv,
+                                              const TC* __restrict__ dz2rho, const TC* __restrict__ cv, int K)
+        {
+            TC b = mul_rn(dz2rho[0], lam) - acv[0];
+            b += av[0];
+            if (K == 1) { if (mean) b -= cv[0]; else b += cv[0]; }
+            w = TC(1)/b;
+        }
+        __device__ __forceinline__ void next(const TC* __restrict__ av, const TC* __restrict__ acv,
+                                             const TC* __restrict__ dz2rho, const TC* __restrict__ cv, int k, int K)
+        {
+            const TC gk = cv[k-1]*w;
+            TC b = mul_rn(dz2rho[k], lam) - acv[k];
+            if (k == K-1) { if (mean) b -= cv[K-1]; else b += cv[K-1]; }
+            w = TC(1)/(b - mul_rn(av[k], gk));
+        }
+    };
+
     template<typename TF, typename TC, int R>
     __global__ void __launch_bounds__(64*R) ytri_fwd_g(cplx<TF>* __restrict__ C, const double2* __restrict__ tw,
-                                                       const TC* __restrict__ wi, const TC* __restrict__ a,
-                                                       const TC* __restrict__ dz2, int nx, int nsys, Gd g,
+                                                       const TC* __restrict__ bmati, const TC* __restrict__ bmatj,
+                                                       const TC* __restrict__ a, const TC* __restrict__ cv,
+                                                       const TC* __restrict__ acv, const TC* __restrict__ dz2rho,
+                                                       const TC* __restrict__ dz2, int nx, Gd g,
                                                        bool skip_mean, bool paired)
     {
         using T = cplx<TF>;
@@ -797,12 +827,20 @@ namespace
         const int lane = threadIdx.x, wv = threadIdx.y, ky = lane + 64*wv;
         const int kx = blockIdx.x, K = g.ktot, kxN = nx - 1;
         const bool pair = paired && kx == 0;
-        const size_t ks = static_cast<size_t>(nx)*n, ls = static_cast<size_t>(nsys);
+        const size_t ks = static_cast<size_t>(nx)*n;
         int lx, ly;
         pair_systems(pair, kx, kxN, ky, n, lx, ly);
         // mixed mode: the mean is solved in FP64 (the x component of thread 0
         // is mode (0, 0) also when paired)
         const bool zero_mode = skip_mean && kx == 0 && ky == 0;
+        // the thread's systems: lambda = bmati[kx] + bmatj[ky] of the mode
+        // behind table index l = kx (n/2 + 1) + ky, the mean mode l = 0 with
+        // its Dirichlet top row; only waves 0 and R/2 of a paired block hold
+        // a second (real) system in the y component
+        const bool two = pair && (wv == 0 || 2*wv == R);
+        Pivots<TC> px_, py_;
+        px_.lam = bmati[lx / (n/2 + 1)] + bmatj[lx % (n/2 + 1)]; px_.mean = lx == 0;
+        py_.lam = bmati[ly / (n/2 + 1)] + bmatj[ly % (n/2 + 1)]; py_.mean = ly == 0;
         T* A = rows[wv];
         T* __restrict__ Ck = C + static_cast<size_t>(kx)*n;      // C[k][kx][.] = Ck[k*ks + .]
         T wtw[P_-1][R-1];
@@ -813,24 +851,14 @@ namespace
         };
 
         T cur[R];
-        TC wcur[R], wcury[R];
         load_row(cur, min(wv, K-1));
-        #pragma unroll
-        for (int m = 0; m < R; ++m) { wcur[m] = wi[min(m, K-1)*ls + lx]; wcury[m] = wi[min(m, K-1)*ls + ly]; }
         TC px = TC(0), py = TC(0);
         for (int k0 = 0; k0 < K; k0 += R)
         {
-            // the next group's row and pivots load while this group is solved
+            // the next group's row loads while this group is solved
             T nxt[R];
-            TC wnxt[R], wnxty[R];
             const size_t kn = static_cast<size_t>(min(k0 + R + wv, K-1));
             load_row(nxt, kn);
-            #pragma unroll
-            for (int m = 0; m < R; ++m)
-            {
-                wnxt[m] = wi[min(k0 + R + m, K-1)*ls + lx];
-                wnxty[m] = wi[min(k0 + R + m, K-1)*ls + ly];
-            }
             // the y transform of level k0 + wv (every wave transforms; rows
             // past the top are clamped copies, never used).  FP32: the row
             // minus its FP64 mean, bin 0 set to the FP64 sum afterwards, as in
@@ -867,9 +895,12 @@ namespace
                     if (2*ky < n) { zx = TC(0.5)*(zx + TC(r.x)); zy = TC(0.5)*(zy - TC(r.y)); }
                     else { const TC dx = TC(r.x) - zx, dy = TC(r.y) + zy; zx = TC(0.5)*dy; zy = TC(-0.5)*dx; }
                 }
+                if (k == 0) { px_.first(a, acv, dz2rho, cv, K); if (two) py_.first(a, acv, dz2rho, cv, K); }
+                else { px_.next(a, acv, dz2rho, cv, k, K); if (two) py_.next(a, acv, dz2rho, cv, k, K); }
+                const TC wx = px_.w, wy = two ? py_.w : px_.w;
                 TC qx = dz2[k]*zx, qy = dz2[k]*zy;
                 if (k > 0) { qx -= a[k]*px; qy -= a[k]*py; }
-                qx *= wcur[m]; qy *= wcury[m];
+                qx *= wx; qy *= wy;
                 if (zero_mode) { qx = TC(0); if (!pair) qy = TC(0); }
                 Ck[k*ks + ky] = T{TF(qx), TF(qy)};
                 px = qx; py = qy;
@@ -877,8 +908,6 @@ namespace
             __syncthreads();
             #pragma unroll
             for (int q = 0; q < R; ++q) cur[q] = nxt[q];
-            #pragma unroll
-            for (int m = 0; m < R; ++m) { wcur[m] = wnxt[m]; wcury[m] = wnxty[m]; }
         }
     }
 
@@ -1460,8 +1489,8 @@ namespace
         // paired (the x transform stored bins 0 and itot/2 packed in row 0):
         // block 0 takes both
         const int nb = paired ? nx - 1 : nx;
-        { mhh::Timer _t(c, "ytri_fwd"); ytri_fwd_g<TF, TC, R><<<nb, blk, 0, c->stream>>>(C, twy, d.tri_wi, d.a, d.dz2,
-                nx, d.nsys, c->gd, skip_mean, paired); }
+        { mhh::Timer _t(c, "ytri_fwd"); ytri_fwd_g<TF, TC, R><<<nb, blk, 0, c->stream>>>(C, twy, d.bmati, d.bmatj,
+                d.a, d.c, d.ac, d.dz2rho, d.dz2, nx, c->gd, skip_mean, paired); }
         { mhh::Timer _t(c, "ytri_bwd"); ytri_bwd_g<TF, TC, R><<<nb, blk, 0, c->stream>>>(C, twy, d.tri_gam,
                 nx, d.nsys, c->gd, paired); }
     }
