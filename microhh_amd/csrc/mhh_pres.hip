@@ -259,6 +259,26 @@ namespace
     template<int R> constexpr int wave_passes() { return R == 8 ? 3 : R == 4 ? 4 : 7; }
     template<int R> constexpr int wave_row_elems() { return ((64*R - 1) + ((64*R - 1) >> 3) + 1) | 1; }
 
+    // LDS rows of the wave x transforms.  The FFT passes use the lpad slots
+    // 0 .. lpad(n-1); after the transform the staged half-spectra of the
+    // wave's two real rows sit as Xa(kx) in slot kx and Xb(kx) in slot
+    // XH + kx, the DC pair (a, b) in slot 0 and the Nyquist pair in slot XH.
+    // XH = 32 R + 2 and a row stride of 4 mod 32 slots spread the transposed
+    // copies between C and the rows (lanes over kx, the rows and the two
+    // parts) over distinct LDS banks for 8- and 16-byte slots.
+    template<int R> constexpr int xhalf() { return 32*R + 2; }
+    template<int R> constexpr int xrow_base() { return wave_row_elems<R>() > 2*xhalf<R>() ? wave_row_elems<R>() : 2*xhalf<R>(); }
+    template<int R> constexpr int xrow_elems() { return xrow_base<R>() + ((4 - xrow_base<R>() % 32) + 32) % 32; }
+
+    template<typename TF, int R>
+    __device__ __forceinline__ void xstage_store(cplx<TF>* S, int kx, cplx<TF> xa, cplx<TF> xb)
+    {
+        constexpr int n = 64*R, XH = xhalf<R>();
+        if (kx == 0) { cplx<TF> d; d.x = xa.x; d.y = xb.x; S[0] = d; }
+        else if (2*kx == n) { cplx<TF> d; d.x = xa.x; d.y = xb.x; S[XH] = d; }
+        else { S[kx] = xa; S[XH + kx] = xb; }
+    }
+
     // Per-lane twiddles of passes 1 .. P-1 of the wave FFT of length n = 64 R:
     // pass p (Ns = R^p) multiplies input q of butterfly `lane` by
     // tw[q (lane mod Ns) n/(Ns R)], conjugated for the inverse (S = +1).
@@ -453,9 +473,9 @@ namespace
     {
         using T = cplx<TF>;
         constexpr int XW = xwaves<TF>(), NT = 64*XW;
-        constexpr int n = 64*R, nx = n/2 + 1, JB = 2*XW, P_ = wave_passes<R>();
+        constexpr int n = 64*R, nx = n/2 + 1, JB = 2*XW, P_ = wave_passes<R>(), XH = xhalf<R>();
         constexpr int NQ = (nx + 63)/64;
-        __shared__ T rows[XW][wave_row_elems<R>()];
+        __shared__ T rows[XW][xrow_elems<R>()];
         __shared__ double wsum[XW];
         const int lane = threadIdx.x, wv = threadIdx.y, tid = lane + 64*wv;
         const int M = g.jtot, nbj = M/JB;
@@ -550,7 +570,7 @@ namespace
         for (int q = 0; q < NQ; ++q)
         {
             const int kx = lane + 64*q;
-            if (kx < nx) stage_store<TF>(A, kx, n, xs[q][0], xs[q][1]);
+            if (kx < nx) xstage_store<TF, R>(A, kx, xs[q][0], xs[q][1]);
         }
         __syncthreads();
         if (MODE == 3 && tid == 0)
@@ -574,16 +594,16 @@ namespace
             T val;
             if (kx == 0 && packed)
             {
-                const T d0 = Sr[0], dn = Sr[1];   // lpad(0) = 0, lpad(1) = 1
+                const T d0 = Sr[0], dn = Sr[XH];
                 val.x = part ? d0.y : d0.x; val.y = part ? dn.y : dn.x;
             }
             else if (kx == 0 || 2*kx == n)
             {
-                const T d = Sr[kx == 0 ? 0 : 1];
+                const T d = Sr[kx == 0 ? 0 : XH];
                 val.x = part ? d.y : d.x; val.y = TF(0);
             }
             else
-                val = Sr[lpad(stage_base(kx, n) + part)];
+                val = Sr[part*XH + kx];
             Ck[static_cast<size_t>(kx)*M + j0 + jl] = val;
         }
     }
@@ -599,8 +619,8 @@ namespace
     {
         using T = cplx<TF>;
         constexpr int XW = xwaves<TF>(), NT = 64*XW;
-        constexpr int n = 64*R, nx = n/2 + 1, JB = 2*XW, P_ = wave_passes<R>();
-        __shared__ T rows[XW][wave_row_elems<R>()];
+        constexpr int n = 64*R, nx = n/2 + 1, JB = 2*XW, P_ = wave_passes<R>(), XH = xhalf<R>();
+        __shared__ T rows[XW][xrow_elems<R>()];
         const int lane = threadIdx.x, wv = threadIdx.y, tid = lane + 64*wv;
         const int M = g.jtot, k = blockIdx.y;
         const int j0 = blockIdx.x*JB;
@@ -621,15 +641,15 @@ namespace
             if (kx == 0 && packed)
             {
                 reinterpret_cast<TF*>(&Sr[0])[part] = val.x;
-                reinterpret_cast<TF*>(&Sr[1])[part] = val.y;
+                reinterpret_cast<TF*>(&Sr[XH])[part] = val.y;
             }
             else if (kx == 0 || 2*kx == n)
             {
-                TF* d = reinterpret_cast<TF*>(&Sr[kx == 0 ? 0 : 1]);
+                TF* d = reinterpret_cast<TF*>(&Sr[kx == 0 ? 0 : XH]);
                 d[part] = val.x;
             }
             else
-                Sr[lpad(stage_base(kx, n) + part)] = val;
+                Sr[part*XH + kx] = val;
         }
         __syncthreads();
         // Hermitian extension of both spectra and packing Z = Xa + i Xb
@@ -642,13 +662,12 @@ namespace
             T xa, xb;
             if (kx == 0 || 2*kx == n)
             {
-                const T d = A[kx == 0 ? 0 : 1];
+                const T d = A[kx == 0 ? 0 : XH];
                 xa.x = d.x; xa.y = TF(0); xb.x = d.y; xb.y = TF(0);
             }
             else
             {
-                const int b = stage_base(kx, n);
-                xa = A[lpad(b)]; xb = A[lpad(b+1)];
+                xa = A[kx]; xb = A[XH + kx];
                 if (x > n/2) { xa.y = -xa.y; xb.y = -xb.y; }
             }
             z[q].x = xa.x - xb.y; z[q].y = xa.y + xb.x;
