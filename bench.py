@@ -70,13 +70,13 @@ KERNELS = list(PASSES) + ["cyclic", "ghost_cells", "mirror_walls", "rk3_zero_gho
 # workload and the dominant kernel match.
 MEASURED_TRAFFIC = {
     (512, 512, 256, 8): {
-        # mean over a step's three launches: (7538.0 + 2 x 10804.5)/3 (the
+        # mean over a step's three launches: (7602.4 + 2 x 10893.7)/3 (the
         # first substep writes the tendencies without reading them)
-        "tend_2i5_smag2": 9715.7e6, "pres_out_rk3": 7606.8e6, "pres_in_fft_x": 5133.3e6,
+        "tend_2i5_smag2": 9796.7e6, "pres_out_rk3": 7606.9e6, "pres_in_fft_x": 5124.4e6,
         # the model takes the CFL/dnmul maxima from the tendency pass, so every
         # viscosity launch is the plain one
-        "evisc": 3230.9e6,
-        "ytri_fwd": 1222.3e6, "ytri_bwd": 1219.2e6, "fft_x_bwd": 1089.4e6,
+        "evisc": 3238.1e6,
+        "ytri_fwd": 1076.2e6, "ytri_bwd": 1220.8e6, "fft_x_bwd": 1089.4e6,
     },
 }
 # Steps of the pass that times every kernel (before the timed steps).
