@@ -230,6 +230,42 @@ int  mhh_timeloop_rk3(mhh_ctx* ctx, int substep, double dt);
 /* The same update for the bound scalars only. */
 int  mhh_timeloop_rk3_scalars(mhh_ctx* ctx, int substep, double dt);
 
+/* ---- graphs of the fused substep ----
+ * The reference launches every kernel from the host (src/model.cxx:356-545);
+ * no counterpart there.  A graph records the kernels of one fused substep --
+ * Boundary::set_prognostic_cyclic_bcs, set_ghost_cells, Diff_smag2::
+ * exec_viscosity, set_ghost_cells, the fused tendency pass and the
+ * projection with the RK3 update, as the model driver issues them for the
+ * dry CBL path -- for one context, or for an ensemble of contexts (models of
+ * any sizes, each with its own non-null stream) as independent branches of
+ * one graph, so one launch advances every member.  The kernel arguments are
+ * those of the capture (the time step among them): mhh_graph_update records
+ * the sequence again for new values and updates the graph in place.  A
+ * pending deferred tendency update (MHH_RK3_DEFER) is host state: the graph
+ * records the state each context must be in when launched and sets the one
+ * it leaves; mhh_graph_launch fails with MHH_ESTATE on a mismatch.  Capture
+ * queues nothing: the contexts are unchanged until the launch.  The first
+ * substep of a step, whose time step comes from the CFL and dnmul maxima, is
+ * launched directly (mhh_tendency_2i5_smag2_limits). */
+typedef struct
+{
+    mhh_smag_params smag;   /* viscosity and diffusion parameters */
+    mhh_bc_desc bc;         /* boundary types of the ghost-cell update */
+    int buoyancy;           /* buoyancy of scalar 0 in the tendency pass */
+    double subdt;           /* sub time step of the projection */
+    int substep;            /* RK3 substep (0, 1, 2) */
+    double rkdt;            /* time step of the RK3 update */
+    int flags;              /* MHH_RK3_SCALARS, MHH_RK3_DEFER */
+} mhh_substep_desc;
+typedef struct mhh_graph mhh_graph;
+int  mhh_graph_capture_substep(mhh_ctx* ctx, const mhh_substep_desc* desc, mhh_graph** out);
+/* Ensemble: n contexts, desc[m] for ctx[m]; the graph is launched on ctx[0]'s
+ * stream and ordered with work queued on the other members' streams. */
+int  mhh_graph_capture_substeps(mhh_ctx* const* ctx, const mhh_substep_desc* desc, int n, mhh_graph** out);
+int  mhh_graph_update(mhh_graph* graph, const mhh_substep_desc* desc);
+int  mhh_graph_launch(mhh_graph* graph);
+int  mhh_graph_destroy(mhh_graph* graph);
+
 /* ---- device buffers ----
  * RAII device vectors of the reference (cuda_vector, include/cuda_buffer.h)
  * and the host<->device field copies of Fields::prepare_device /
@@ -243,6 +279,10 @@ int  mhh_copy_to_host(void* dst, const void* src, size_t bytes);
  * known-size reference kernel for HBM traffic counters. */
 int  mhh_copy_device(void* dst, const void* src, size_t bytes);
 int  mhh_device_synchronize(void);
+/* A HIP stream of the library's runtime (default priority), for callers that
+ * hold no HIP runtime of their own (e.g. the contexts of a captured graph). */
+int  mhh_stream_create(void** stream);
+int  mhh_stream_destroy(void* stream);
 
 /* ---- kernel timing ----
  * Per-kernel HIP-event timing on the context's stream, the measurement the

@@ -53,6 +53,11 @@ class SmagParams(C.Structure):
                 ("surface_model", C.c_int), ("thermo", C.c_int)]
 
 
+class SubstepDesc(C.Structure):
+    _fields_ = [("smag", SmagParams), ("bc", BcDesc), ("buoyancy", C.c_int), ("subdt", C.c_double),
+                ("substep", C.c_int), ("rkdt", C.c_double), ("flags", C.c_int)]
+
+
 # name: (restype, argtypes) -- every symbol declared in include/microhh_amd.h
 SIGNATURES = {
     "mhh_ctx_create": (C.c_int, [C.POINTER(GridDesc), C.c_int, VP, C.POINTER(VP)]),
@@ -92,6 +97,13 @@ SIGNATURES = {
     "mhh_copy_to_host": (C.c_int, [VP, VP, C.c_size_t]),
     "mhh_copy_device": (C.c_int, [VP, VP, C.c_size_t]),
     "mhh_device_synchronize": (C.c_int, []),
+    "mhh_stream_create": (C.c_int, [C.POINTER(VP)]),
+    "mhh_stream_destroy": (C.c_int, [VP]),
+    "mhh_graph_capture_substep": (C.c_int, [VP, C.POINTER(SubstepDesc), C.POINTER(VP)]),
+    "mhh_graph_capture_substeps": (C.c_int, [C.POINTER(VP), C.POINTER(SubstepDesc), C.c_int, C.POINTER(VP)]),
+    "mhh_graph_update": (C.c_int, [VP, C.POINTER(SubstepDesc)]),
+    "mhh_graph_launch": (C.c_int, [VP]),
+    "mhh_graph_destroy": (C.c_int, [VP]),
     "mhh_timing_enable": (C.c_int, [VP, C.c_int]),
     "mhh_timing_select": (C.c_int, [VP, C.c_char_p]),
     "mhh_timing_reset": (C.c_int, [VP]),

@@ -373,6 +373,22 @@ int mhh_copy_device(void* dst, const void* src, size_t bytes)
     return mhh::check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
 }
 
+int mhh_stream_create(void** stream)
+{
+    if (!stream) return mhh::fail(MHH_EINVAL, "mhh_stream_create: null argument");
+    hipStream_t s = nullptr;
+    MHH_CHECK(mhh::check_hip(hipStreamCreate(&s), "hipStreamCreate"));
+    *stream = s;
+    return 0;
+}
+
+int mhh_stream_destroy(void* stream)
+{
+    if (!stream) return 0;
+    (void)hipStreamSynchronize(static_cast<hipStream_t>(stream));
+    return mhh::check_hip(hipStreamDestroy(static_cast<hipStream_t>(stream)), "hipStreamDestroy");
+}
+
 int mhh_device_synchronize(void)
 {
     return mhh::check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");

@@ -866,6 +866,14 @@ namespace
     }
 }
 
+namespace mhh
+{
+    int smag_prepare(mhh_ctx* c, const mhh_smag_params& par)
+    {
+        return c->precision == MHH_FP64 ? ensure_mlen<double>(c, par.cs) : ensure_mlen<float>(c, par.cs);
+    }
+}
+
 extern "C" {
 
 int mhh_diff_smag2_exec_viscosity(mhh_ctx* c, const mhh_smag_params* par)

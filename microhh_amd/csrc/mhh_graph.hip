@@ -1,0 +1,213 @@
+// mhh_graph.hip -- HIP graphs of the fused substep, for one model or an
+// ensemble of models (one context each) advanced by a single launch.
+//
+// A graph records the kernels of the fused substep sequence of the model
+// driver (src/model.cxx:368-504): cyclic boundaries and ghost levels, the
+// eddy viscosity, the fused tendency pass, and the projection with the RK3
+// update.  Each context records on its own stream; the streams fork from and
+// join into the first one during capture, so the members are independent
+// branches of the graph and run concurrently.  The kernel arguments (the sub
+// time step among them) are those of the capture; mhh_graph_update records
+// the sequence again for new values and updates the instantiated graph in
+// place.
+//
+// The library keeps one piece of host state across the sequence, a pending
+// deferred RK3 tendency update (MHH_RK3_DEFER); capture runs the host logic
+// once, so the graph records the state each context must be in when it is
+// launched and the state it leaves.
+#include <vector>
+
+#include "mhh_internal.h"
+
+struct mhh_graph
+{
+    std::vector<mhh_ctx*> ctx;
+    std::vector<mhh_substep_desc> desc;
+    std::vector<char> pend_in, pend_out;      // deferred update pending before / after
+    std::vector<double> cA_in, cA_out;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+};
+
+namespace
+{
+    using mhh::fail;
+    using mhh::check_hip;
+
+    // The fused substep on one context, as the model driver issues it.
+    int substep_sequence(mhh_ctx* c, const mhh_substep_desc& d)
+    {
+        MHH_CHECK(mhh_boundary_set_prognostic_cyclic_bcs(c));
+        MHH_CHECK(mhh_boundary_set_ghost_cells(c, &d.bc));
+        MHH_CHECK(mhh_diff_smag2_exec_viscosity(c, &d.smag));
+        MHH_CHECK(mhh_boundary_set_ghost_cells(c, &d.bc));
+        MHH_CHECK(mhh_tendency_2i5_smag2(c, &d.smag, d.buoyancy));
+        return mhh_pres_2_exec_rk3(c, d.subdt, d.substep, d.rkdt, d.flags);
+    }
+
+    // Record the sequences of every context into a new graph.  The host
+    // state of each context is restored afterwards (nothing ran); its state
+    // after the sequence is returned in g.
+    int record(mhh_graph* g, hipGraph_t* out)
+    {
+        const int n = static_cast<int>(g->ctx.size());
+        hipStream_t s0 = g->ctx[0]->stream;
+        std::vector<char> timing(n);
+        for (int m = 0; m < n; ++m)
+        {
+            mhh_ctx* c = g->ctx[m];
+            // the length-scale table is uploaded on first use (a host copy,
+            // which a capture cannot hold): make sure it is there
+            MHH_CHECK(mhh::smag_prepare(c, g->desc[m].smag));
+            timing[m] = c->timing;
+            c->timing = false;
+            c->defer_pending = g->pend_in[m];
+            c->defer_cA = g->cA_in[m];
+        }
+        int rc = check_hip(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        for (int m = 0; m < n && rc == 0; ++m)
+        {
+            mhh_ctx* c = g->ctx[m];
+            if (c->stream != s0)
+            {
+                rc = check_hip(hipEventRecord(g->fork, s0), "hipEventRecord");
+                if (!rc) rc = check_hip(hipStreamWaitEvent(c->stream, g->fork, 0), "hipStreamWaitEvent");
+            }
+            if (!rc) rc = substep_sequence(c, g->desc[m]);
+            if (!rc && c->stream != s0)
+            {
+                rc = check_hip(hipEventRecord(g->join[m], c->stream), "hipEventRecord");
+                if (!rc) rc = check_hip(hipStreamWaitEvent(s0, g->join[m], 0), "hipStreamWaitEvent");
+            }
+            g->pend_out[m] = c->defer_pending;
+            g->cA_out[m] = c->defer_cA;
+        }
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(s0, &graph);
+        for (int m = 0; m < n; ++m)
+        {
+            mhh_ctx* c = g->ctx[m];
+            c->timing = timing[m];
+            c->defer_pending = g->pend_in[m];
+            c->defer_cA = g->cA_in[m];
+        }
+        if (rc)
+        {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        MHH_CHECK(check_hip(e, "hipStreamEndCapture"));
+        *out = graph;
+        return 0;
+    }
+
+    void release(mhh_graph* g)
+    {
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        if (g->fork) (void)hipEventDestroy(g->fork);
+        for (hipEvent_t e : g->join) if (e) (void)hipEventDestroy(e);
+        delete g;
+    }
+}
+
+extern "C" {
+
+int mhh_graph_capture_substeps(mhh_ctx* const* ctx, const mhh_substep_desc* desc, int n, mhh_graph** out)
+{
+    if (!ctx || !desc || !out || n < 1) return fail(MHH_EINVAL, "mhh_graph_capture_substeps: bad argument");
+    *out = nullptr;
+    for (int m = 0; m < n; ++m)
+    {
+        if (!ctx[m]) return fail(MHH_EINVAL, "mhh_graph_capture_substeps: null context");
+        if (!ctx[m]->bound) return fail(MHH_ESTATE, "mhh_graph_capture_substeps: fields are not bound");
+        if (!ctx[m]->stream)
+            return fail(MHH_EINVAL, "mhh_graph_capture_substeps: a captured context needs its own (non-null) stream");
+        for (int q = 0; q < m; ++q)
+            if (ctx[q] == ctx[m]) return fail(MHH_EINVAL, "mhh_graph_capture_substeps: a context appears twice");
+    }
+    auto* g = new mhh_graph();
+    g->ctx.assign(ctx, ctx + n);
+    g->desc.assign(desc, desc + n);
+    g->pend_in.resize(n); g->pend_out.resize(n); g->cA_in.resize(n); g->cA_out.resize(n);
+    g->join.assign(n, nullptr);
+    for (int m = 0; m < n; ++m) { g->pend_in[m] = ctx[m]->defer_pending; g->cA_in[m] = ctx[m]->defer_cA; }
+    int rc = check_hip(hipEventCreateWithFlags(&g->fork, hipEventDisableTiming), "hipEventCreate");
+    for (int m = 0; m < n && !rc; ++m)
+        rc = check_hip(hipEventCreateWithFlags(&g->join[m], hipEventDisableTiming), "hipEventCreate");
+    if (!rc) rc = record(g, &g->graph);
+    if (!rc) rc = check_hip(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    if (rc) { release(g); return rc; }
+    *out = g;
+    return 0;
+}
+
+int mhh_graph_capture_substep(mhh_ctx* ctx, const mhh_substep_desc* desc, mhh_graph** out)
+{
+    return mhh_graph_capture_substeps(&ctx, desc, 1, out);
+}
+
+int mhh_graph_update(mhh_graph* g, const mhh_substep_desc* desc)
+{
+    if (!g || !desc) return fail(MHH_EINVAL, "mhh_graph_update: null argument");
+    const int n = static_cast<int>(g->ctx.size());
+    g->desc.assign(desc, desc + n);
+    // the entry state is the contexts' current one
+    for (int m = 0; m < n; ++m) { g->pend_in[m] = g->ctx[m]->defer_pending; g->cA_in[m] = g->ctx[m]->defer_cA; }
+    hipGraph_t graph = nullptr;
+    MHH_CHECK(record(g, &graph));
+    hipGraphExecUpdateResult res;
+    hipGraphNode_t bad = nullptr;
+    if (hipGraphExecUpdate(g->exec, graph, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess)
+    {
+        // the topology changed (another kernel variant): instantiate anew
+        (void)hipGetLastError();
+        hipGraphExec_t exec = nullptr;
+        const int rc = check_hip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+        if (rc) { (void)hipGraphDestroy(graph); return rc; }
+        (void)hipGraphExecDestroy(g->exec);
+        g->exec = exec;
+    }
+    (void)hipGraphDestroy(g->graph);
+    g->graph = graph;
+    return 0;
+}
+
+int mhh_graph_launch(mhh_graph* g)
+{
+    if (!g) return fail(MHH_EINVAL, "mhh_graph_launch: null graph");
+    const int n = static_cast<int>(g->ctx.size());
+    for (int m = 0; m < n; ++m)
+        if (g->ctx[m]->defer_pending != bool(g->pend_in[m]) || (g->pend_in[m] && g->ctx[m]->defer_cA != g->cA_in[m]))
+            return fail(MHH_ESTATE, "mhh_graph_launch: a context is not in the state the graph was recorded in "
+                                    "(deferred tendency update); update the graph first");
+    hipStream_t s0 = g->ctx[0]->stream;
+    // work queued on the members' streams comes first, work queued after
+    // the launch comes after the graph
+    for (int m = 1; m < n; ++m)
+    {
+        MHH_CHECK(check_hip(hipEventRecord(g->join[m], g->ctx[m]->stream), "hipEventRecord"));
+        MHH_CHECK(check_hip(hipStreamWaitEvent(s0, g->join[m], 0), "hipStreamWaitEvent"));
+    }
+    MHH_CHECK(check_hip(hipGraphLaunch(g->exec, s0), "hipGraphLaunch"));
+    if (n > 1)
+    {
+        MHH_CHECK(check_hip(hipEventRecord(g->fork, s0), "hipEventRecord"));
+        for (int m = 1; m < n; ++m)
+            MHH_CHECK(check_hip(hipStreamWaitEvent(g->ctx[m]->stream, g->fork, 0), "hipStreamWaitEvent"));
+    }
+    for (int m = 0; m < n; ++m) { g->ctx[m]->defer_pending = g->pend_out[m]; g->ctx[m]->defer_cA = g->cA_out[m]; }
+    return 0;
+}
+
+int mhh_graph_destroy(mhh_graph* g)
+{
+    if (!g) return 0;
+    for (mhh_ctx* c : g->ctx) (void)hipStreamSynchronize(c->stream);
+    release(g);
+    return 0;
+}
+
+} // extern "C"

@@ -191,6 +191,11 @@ namespace mhh
     // Completes a deferred RK3 tendency update, if one is pending (mhh_pres.hip).
     int settle_tendencies(mhh_ctx* ctx);
 
+    // Uploads what the Smagorinsky viscosity of these parameters needs once
+    // (the length scale per level), so that later calls queue kernels only
+    // (mhh_diff.hip).
+    int smag_prepare(mhh_ctx* ctx, const mhh_smag_params& par);
+
     // Pressure solver setup (mhh_pres.hip).
     int pres_init(mhh_ctx* ctx);
     void pres_free(mhh_ctx* ctx);

@@ -25,7 +25,7 @@ class GpuCase:
     """One context with all fields allocated (zero) as float64/float32 tensors."""
 
     def __init__(self, itot, jtot, ktot, igc, jgc, kgc, dx, dy, grid, nscalars=1,
-                 precision=abi.MHH_FP64, thref=None, threfh=None):
+                 precision=abi.MHH_FP64, thref=None, threfh=None, stream=None):
         self.dtype = "float64" if precision == abi.MHH_FP64 else "float32"
         self.icells, self.jcells, self.kcells = itot + 2 * igc, jtot + 2 * jgc, ktot + 2 * kgc
         self.shape3 = (self.kcells, self.jcells, self.icells)
@@ -36,7 +36,7 @@ class GpuCase:
         gd = abi.GridDesc(itot, jtot, ktot, igc, jgc, kgc, dx, dy, float(grid["zh"][kgc + ktot]),
                           *[dptr(self._grid[k]) for k in ("z", "zh", "dz", "dzh", "dzi", "dzhi", "rhoref", "rhorefh")])
         h = C.c_void_p()
-        abi.check(L.mhh_ctx_create(C.byref(gd), precision, None, C.byref(h)), "mhh_ctx_create")
+        abi.check(L.mhh_ctx_create(C.byref(gd), precision, stream, C.byref(h)), "mhh_ctx_create")
         self.ctx = h
         self.t = {}
         for n in FIELDS3:

@@ -1,0 +1,168 @@
+"""HIP graphs of the fused substep (mhh_graph_*): one model, and an ensemble
+of models of different sizes advanced by one launch per substep, against the
+same substeps issued call by call.  The graphs hold the same kernels with the
+same arguments, so the fields agree bit for bit.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(32, 24, 16, 5), (48, 40, 24, 6), (64, 32, 20, 7)]
+# the member grids of the batched dry-CBL ensemble (FP32), one of each
+ENSEMBLE = [(32, 32, 32, 11), (64, 64, 48, 12), (96, 96, 64, 13), (128, 128, 96, 14)]
+STEPS = [(0, 0.41), (1, 0.41), (2, 0.41), (0, 0.37), (1, 0.37)]   # (substep, dt)
+CB = (1. / 3., 15. / 16., 8. / 15.)
+
+
+def _desc(substep, dt):
+    from microhh_amd import abi
+    smag = abi.SmagParams(0.23, 1.0 / 3.0, 1.5e-5, (C.c_double * 8)(*([2e-5] * 8)), 1, 0, 1)
+    bc = abi.BcDesc(abi.BC["noslip"], abi.BC["freeslip"], (C.c_int * 8)(*([abi.BC["dirichlet"]] * 8)),
+                    (C.c_int * 8)(*([abi.BC["neumann"]] * 8)))
+    return abi.SubstepDesc(smag, bc, 1, CB[substep] * dt, substep, dt,
+                           abi.MHH_RK3_SCALARS | abi.MHH_RK3_DEFER)
+
+
+def _case(itot, jtot, ktot, seed, stream, fp32=False):
+    from microhh_amd import abi
+    from tests.golden.make_golden import make_inputs
+    from tests._gpu import GpuCase
+    p, i = make_inputs(dict(itot=itot, jtot=jtot, ktot=ktot, stretch=ktot <= 24, anelastic=True, dx=50.0,
+                            dy=40.0, seed=seed))
+    kc = ktot + 2 * p["kgc"]
+    thref = np.full(kc, 300.0) + 0.01 * np.arange(kc)
+    g = GpuCase(itot, jtot, ktot, p["igc"], p["jgc"], p["kgc"], p["dx"], p["dy"], i, nscalars=1,
+                precision=abi.MHH_FP32 if fp32 else abi.MHH_FP64,
+                thref=thref, threfh=thref + 0.005, stream=stream)
+    for n, src in (("u", "u"), ("v", "v"), ("w", "w"), ("s0", "s")):
+        g.put(n, i[src])
+    for n in ("ut", "vt", "wt", "st0"):
+        g.put(n, np.zeros_like(i["u"]))
+    return g
+
+
+def _direct(g, d):
+    g.call("mhh_boundary_set_prognostic_cyclic_bcs")
+    g.call("mhh_boundary_set_ghost_cells", C.byref(d.bc))
+    g.call("mhh_diff_smag2_exec_viscosity", C.byref(d.smag))
+    g.call("mhh_boundary_set_ghost_cells", C.byref(d.bc))
+    g.call("mhh_tendency_2i5_smag2", C.byref(d.smag), d.buoyancy)
+    g.call("mhh_pres_2_exec_rk3", d.subdt, d.substep, d.rkdt, d.flags)
+
+
+def _state(g):
+    return {n: g.get(n) for n in ("u", "v", "w", "s0", "p")}
+
+
+@pytest.fixture
+def streams():
+    from microhh_amd import abi
+    L = abi.lib()
+    made = []
+
+    def make():
+        s = C.c_void_p()
+        abi.check(L.mhh_stream_create(C.byref(s)), "mhh_stream_create")
+        made.append(s)
+        return s
+    yield make
+    for s in made:
+        L.mhh_stream_destroy(s)
+
+
+def test_graph_substeps_match_direct_calls(streams):
+    from microhh_amd import abi
+    L = abi.lib()
+    it, jt, kt, seed = SIZES[0]
+    a = _case(it, jt, kt, seed, streams())
+    b = _case(it, jt, kt, seed, streams())
+    graph = C.c_void_p()
+    try:
+        for n, (sub, dt) in enumerate(STEPS):
+            d = _desc(sub, dt)
+            _direct(a, d)
+            if n == 0:
+                abi.check(L.mhh_graph_capture_substep(b.ctx, C.byref(d), C.byref(graph)), "capture")
+            else:
+                abi.check(L.mhh_graph_update(graph, C.byref(d)), "update")
+            abi.check(L.mhh_graph_launch(graph), "launch")
+            sa, sb = _state(a), _state(b)
+            for name in sa:
+                assert np.array_equal(sa[name], sb[name]), f"step {n} {name}"
+        # settled tendencies agree too
+        a.call("mhh_tendency_settle"); b.call("mhh_tendency_settle")
+        for name in ("ut", "vt", "wt", "st0"):
+            assert np.array_equal(a.get(name), b.get(name)), name
+    finally:
+        L.mhh_graph_destroy(graph)
+        a.close(); b.close()
+
+
+@pytest.mark.parametrize("members", ["fp64", "fp32-ensemble-grids"])
+def test_graph_ensemble_one_launch_per_substep(streams, members):
+    """Models of different sizes, each on its own stream, in one graph: every
+    member as its call-by-call run."""
+    from microhh_amd import abi
+    L = abi.lib()
+    sizes, fp32 = (SIZES, False) if members == "fp64" else (ENSEMBLE, True)
+    direct = [_case(*sz, streams(), fp32) for sz in sizes]
+    batch = [_case(*sz, streams(), fp32) for sz in sizes]
+    ctxs = (C.c_void_p * len(batch))(*[g.ctx for g in batch])
+    graph = C.c_void_p()
+    try:
+        for n, (sub, dt) in enumerate(STEPS):
+            # each member its own time step
+            descs = (abi.SubstepDesc * len(batch))(*[_desc(sub, dt * (1. + 0.1 * m)) for m in range(len(batch))])
+            for m, g in enumerate(direct):
+                _direct(g, descs[m])
+            if n == 0:
+                abi.check(L.mhh_graph_capture_substeps(ctxs, descs, len(batch), C.byref(graph)), "capture")
+            else:
+                abi.check(L.mhh_graph_update(graph, descs), "update")
+            abi.check(L.mhh_graph_launch(graph), "launch")
+            for m in range(len(batch)):
+                sa, sb = _state(direct[m]), _state(batch[m])
+                for name in sa:
+                    assert np.array_equal(sa[name], sb[name]), f"step {n} member {m} {name}"
+    finally:
+        L.mhh_graph_destroy(graph)
+        for g in direct + batch:
+            g.close()
+
+
+def test_graph_launch_checks_deferred_state(streams):
+    """A graph recorded with a deferred update pending cannot be launched once
+    the update has been completed (another call settled it)."""
+    from microhh_amd import abi
+    L = abi.lib()
+    it, jt, kt, seed = SIZES[0]
+    g = _case(it, jt, kt, seed, streams())
+    graph = C.c_void_p()
+    try:
+        d0 = _desc(0, 0.4)
+        _direct(g, d0)                      # leaves a deferred update pending
+        d1 = _desc(1, 0.4)
+        abi.check(L.mhh_graph_capture_substep(g.ctx, C.byref(d1), C.byref(graph)), "capture")
+        g.call("mhh_tendency_settle")       # state changes
+        assert L.mhh_graph_launch(graph) == abi.MHH_ESTATE
+        abi.check(L.mhh_graph_update(graph, C.byref(d1)), "update")
+        abi.check(L.mhh_graph_launch(graph), "launch")
+    finally:
+        L.mhh_graph_destroy(graph)
+        g.close()
+
+
+def test_graph_needs_own_stream():
+    from microhh_amd import abi
+    L = abi.lib()
+    it, jt, kt, seed = SIZES[0]
+    g = _case(it, jt, kt, seed, None)
+    graph = C.c_void_p()
+    try:
+        d = _desc(0, 0.4)
+        assert L.mhh_graph_capture_substep(g.ctx, C.byref(d), C.byref(graph)) == abi.MHH_EINVAL
+    finally:
+        g.close()
