@@ -234,6 +234,79 @@ def mixed_precision(mm, abi, args, fp64_model, fp64_ms_per_step):
     return out
 
 
+ENSEMBLE_GRIDS = [(32, 32, 32), (64, 64, 48), (96, 96, 64), (128, 128, 96)]
+
+
+def ensemble_bench(args):
+    """The batched dry-CBL ensemble: members_per_size FP32 members of each of
+    32^3, 64x64x48, 96x96x64 and 128x128x96 (dx = dy = 100 m, dz = 50 m),
+    each with its own noise seed and surface forcing and a fixed time step,
+    advanced together (mhh_ensemble_*: every member's substep issued on its
+    own stream without host synchronisation, or one graph launch of all
+    members per substep).  The same members stepped one model at a time
+    (mhh_model_substep, member after member) are timed beside it.  The surface forcing is a
+    per-member surface theta (Dirichlet, 300.2-302 K): a prescribed heat flux
+    needs the surface layer, which this build does not have."""
+    import numpy as np
+    from microhh_amd import abi, model as mm
+    rng = np.random.default_rng(20240607)
+    cases = []
+    for (i, j, k) in ENSEMBLE_GRIDS:
+        for _ in range(args.members_per_size):
+            c = mm.dry_cbl_case(itot=i, jtot=j, ktot=k, xsize=100.0 * i, ysize=100.0 * j, zsize=50.0 * k,
+                                precision=mm.MHH_FP32, adaptive=False, dt=2.0,
+                                rndseed=int(rng.integers(1, 2 ** 30)))
+            c["sbot"] = 300.0 + float(rng.uniform(0.2, 2.0))
+            cases.append(c)
+    npts = sum(c["itot"] * c["jtot"] * c["ktot"] for c in cases)
+    log(f"bench: ensemble of {len(cases)} FP32 members, {npts / 1e6:.1f} M points")
+    streams = mm.Ensemble.streams(len(cases) if args.ensemble_streams == "own" else 1)
+    members = [mm.Model(c, stream=streams[n % len(streams)]) for n, c in enumerate(cases)]
+    ens = mm.Ensemble(members, mode=args.ensemble_mode)
+    L = abi.lib()
+
+    def timed(run, nsteps):
+        abi.check(L.mhh_device_synchronize(), "sync")
+        t0 = time.perf_counter()
+        run(nsteps)
+        abi.check(L.mhh_device_synchronize(), "sync")
+        return time.perf_counter() - t0
+
+    def one_by_one(nsteps):
+        for _ in range(3 * nsteps):
+            for m in members:
+                m.substep()
+
+    log(f"bench: {args.warmup} warmup steps")
+    ens.steps(args.warmup)
+    log(f"bench: {args.steps} timed steps, ensemble ({args.ensemble_mode})")
+    t_graph = timed(ens.steps, args.steps)
+    st = ens.stats()
+    log(f"bench: {args.steps} timed steps, member by member")
+    t_direct = timed(one_by_one, args.steps)
+    nsub = 3 * args.steps
+    finite = all(np.isfinite(m.field("th")).all() for m in members[:: args.members_per_size])
+    ens.close()
+    for m in members:
+        m.close()
+    mm.Ensemble.destroy_streams(streams)
+    value = npts * nsub / t_graph / 1e6
+    print(json.dumps({
+        "metric": "ensemble Mpoint-substeps/s", "value": round(value, 2), "unit": "Mpoint-substeps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(t_graph / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded noise per member)",
+        "config": {"workload": "dry CBL ensemble", "mode": args.ensemble_mode,
+                   "members": len(cases), "grids": [list(g) for g in ENSEMBLE_GRIDS],
+                   "members_per_grid": args.members_per_size, "points": npts, "dt": 2.0,
+                   "surface": "per-member Dirichlet theta 300.2-302 K", "streams": args.ensemble_streams},
+        "member_by_member": {"value": round(npts * nsub / t_direct / 1e6, 2),
+                             "ms_per_step": round(t_direct / args.steps * 1e3, 3)},
+        "speedup_vs_member_by_member": round(t_direct / t_graph, 3),
+        "ensemble_stats": st, "fields_finite": bool(finite),
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -248,9 +321,24 @@ def main():
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel events in the timed steps")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the known-size copy kernel (HBM counter calibration)")
+    ap.add_argument("--workload", default="dycore", choices=["dycore", "ensemble"],
+                    help="dycore: the 512x512x256 dry CBL (default); ensemble: 64 FP32 dry-CBL members "
+                         "of four sizes advanced by one graph launch per substep")
+    ap.add_argument("--members-per-size", type=int, default=16)
+    ap.add_argument("--ensemble-streams", default="own", choices=["own", "shared"],
+                    help="own: one stream per member; shared: one stream for all members")
+    ap.add_argument("--ensemble-mode", default="streams", choices=["streams", "graph"],
+                    help="streams: every member's substep issued on its stream without host synchronisation; "
+                         "graph: one graph launch of all members per substep")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.workload == "ensemble":
+        if args.gpus != 1:
+            print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+            sys.exit(2)
+        ensemble_bench(args)
         return
 
     if args.gpus != 1:

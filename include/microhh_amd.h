@@ -257,6 +257,8 @@ typedef struct
     double rkdt;            /* time step of the RK3 update */
     int flags;              /* MHH_RK3_SCALARS, MHH_RK3_DEFER */
 } mhh_substep_desc;
+/* The sequence a graph records, issued directly on the context's stream. */
+int  mhh_substep_exec(mhh_ctx* ctx, const mhh_substep_desc* desc);
 typedef struct mhh_graph mhh_graph;
 int  mhh_graph_capture_substep(mhh_ctx* ctx, const mhh_substep_desc* desc, mhh_graph** out);
 /* Ensemble: n contexts, desc[m] for ctx[m]; the graph is launched on ctx[0]'s

@@ -75,6 +75,35 @@ int mhh_model_div(mhh_model* m, double* div);
 /* The operator context the model runs on (for callers that add operators). */
 int mhh_model_ctx(mhh_model* m, mhh_ctx** ctx);
 
+/* ---- ensembles ----
+ * No counterpart in the reference, which runs one model per process.  n
+ * models (any sizes, any precision; each created with its own non-null
+ * stream, for instance from mhh_stream_create) advanced together.  A
+ * substep that a member can run as a graph substep -- the fused
+ * configuration, and not the first substep of a step with an adaptive time
+ * step, whose dt comes from the CFL and dn maxima -- is
+ *   MHH_ENSEMBLE_STREAMS: issued on the member's stream with no host
+ *     synchronisation (mhh_substep_exec), member after member, so the
+ *     members' kernels overlap on the GPU;
+ *   MHH_ENSEMBLE_GRAPH: one launch of a graph of all members
+ *     (mhh_graph_capture_substeps), when every member can; one graph is
+ *     kept per substep index and recorded again only when a member's time
+ *     step or deferred-update state differs from the recording.
+ * Any other substep runs member by member, as mhh_model_substep.  The
+ * members' fields, time and substep counters are those of the same
+ * substeps run one model at a time.  The models must outlive the ensemble;
+ * calls on a member between ensemble substeps are allowed. */
+typedef struct mhh_ensemble mhh_ensemble;
+enum { MHH_ENSEMBLE_STREAMS = 0, MHH_ENSEMBLE_GRAPH = 1 };
+int mhh_ensemble_create(mhh_model* const* models, int n, int mode, mhh_ensemble** out);
+int mhh_ensemble_substep(mhh_ensemble* e);
+/* n full time steps (3 substeps each). */
+int mhh_ensemble_steps(mhh_ensemble* e, int n);
+/* Counts so far: graph launches, graph recordings (first captures and
+ * updates), and ensemble substeps issued member by member. */
+int mhh_ensemble_stats(const mhh_ensemble* e, long long* launches, long long* updates, long long* direct);
+int mhh_ensemble_destroy(mhh_ensemble* e);
+
 #ifdef __cplusplus
 }
 #endif

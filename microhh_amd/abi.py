@@ -99,6 +99,7 @@ SIGNATURES = {
     "mhh_device_synchronize": (C.c_int, []),
     "mhh_stream_create": (C.c_int, [C.POINTER(VP)]),
     "mhh_stream_destroy": (C.c_int, [VP]),
+    "mhh_substep_exec": (C.c_int, [VP, C.POINTER(SubstepDesc)]),
     "mhh_graph_capture_substep": (C.c_int, [VP, C.POINTER(SubstepDesc), C.POINTER(VP)]),
     "mhh_graph_capture_substeps": (C.c_int, [C.POINTER(VP), C.POINTER(SubstepDesc), C.c_int, C.POINTER(VP)]),
     "mhh_graph_update": (C.c_int, [VP, C.POINTER(SubstepDesc)]),

@@ -144,6 +144,12 @@ int mhh_graph_capture_substeps(mhh_ctx* const* ctx, const mhh_substep_desc* desc
     return 0;
 }
 
+int mhh_substep_exec(mhh_ctx* ctx, const mhh_substep_desc* desc)
+{
+    if (!ctx || !desc) return fail(MHH_EINVAL, "mhh_substep_exec: null argument");
+    return substep_sequence(ctx, *desc);
+}
+
 int mhh_graph_capture_substep(mhh_ctx* ctx, const mhh_substep_desc* desc, mhh_graph** out)
 {
     return mhh_graph_capture_substeps(&ctx, desc, 1, out);

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 #include "mhh_host.h"
 
@@ -779,6 +780,24 @@ namespace mhh_host
     }
 
     template<typename TF>
+    bool Model<TF>::graph_desc(mhh_substep_desc* d)
+    {
+        // The fused branch of substep() with the time step already fixed:
+        // the same calls, with the plain tendency pass in place of the one
+        // that returns the time-step maxima.
+        if (!fused_tendency || (!timeloop->in_substep() && timeloop->is_adaptive()))
+            return false;
+        d->smag = static_cast<Diff_smag2<TF>&>(*diff).params();
+        d->bc = boundary->bc_desc();
+        d->buoyancy = 1;
+        d->subdt = timeloop->get_sub_time_step();
+        d->substep = timeloop->get_substep();
+        d->rkdt = timeloop->get_dt();
+        d->flags = MHH_RK3_SCALARS | MHH_RK3_DEFER;
+        return true;
+    }
+
+    template<typename TF>
     void* Model<TF>::field_ptr(const std::string& name)
     {
         // a pointer to a tendency is handed out with the deferred update completed
@@ -963,6 +982,149 @@ int mhh_model_ctx(mhh_model* m, mhh_ctx** ctx)
 {
     if (!m || !ctx) { mhh_host_set_error("null argument"); return MHH_EINVAL; }
     *ctx = m->m->ctx();
+    return 0;
+}
+
+} // extern "C"
+
+// An ensemble of models advanced together.  MHH_ENSEMBLE_STREAMS: every
+// member's substep is issued in turn on the member's own stream, without a
+// host synchronisation where the time step is fixed, so the members' kernels
+// overlap on the GPU.  MHH_ENSEMBLE_GRAPH: every substep that all members can
+// run as a graph substep is one launch of a graph holding all of them
+// (mhh_graph_capture_substeps); one graph is kept per substep index, recorded
+// on first use and recorded again only when a member's descriptor or
+// deferred-update state differs from the recording, so with a fixed time
+// step the graphs are recorded once.  In either mode a substep that some
+// member cannot run that way (the first substep of a step with an adaptive
+// time step) runs member by member through the model's own substep.
+struct mhh_ensemble
+{
+    int mode = MHH_ENSEMBLE_STREAMS;
+    std::vector<mhh_model*> members;
+    std::vector<mhh_ctx*> ctx;
+    mhh_graph* graph[3] = {nullptr, nullptr, nullptr};
+    std::vector<mhh_substep_desc> recorded[3];
+    long long launches = 0, updates = 0, direct = 0;
+};
+
+namespace
+{
+    bool same_desc(const std::vector<mhh_substep_desc>& a, const std::vector<mhh_substep_desc>& b)
+    {
+        return a.size() == b.size() && std::memcmp(a.data(), b.data(), a.size()*sizeof(mhh_substep_desc)) == 0;
+    }
+
+    void ensemble_substep(mhh_ensemble* e)
+    {
+        const int n = static_cast<int>(e->members.size());
+        std::vector<mhh_substep_desc> d(n);
+        std::memset(d.data(), 0, n*sizeof(mhh_substep_desc));   // padding too: descriptors compare bytewise
+        if (e->mode == MHH_ENSEMBLE_STREAMS)
+        {
+            for (int m = 0; m < n; ++m)
+            {
+                mhh_model* mm = e->members[m];
+                if (mm->m->graph_desc(&d[m]))
+                {
+                    mhh_host::check(mhh_substep_exec(e->ctx[m], &d[m]), "mhh_substep_exec");
+                    mm->m->graph_done();
+                }
+                else
+                    mm->m->substep();
+            }
+            ++e->direct;
+            return;
+        }
+        bool graphable = true;
+        for (int m = 0; m < n && graphable; ++m)
+            graphable = e->members[m]->m->graph_desc(&d[m]);
+        if (graphable)
+            for (int m = 1; m < n; ++m)
+                graphable = graphable && d[m].substep == d[0].substep;
+        if (!graphable)
+        {
+            for (mhh_model* mm : e->members) mm->m->substep();
+            ++e->direct;
+            return;
+        }
+        const int s = d[0].substep;
+        if (!e->graph[s])
+        {
+            mhh_host::check(mhh_graph_capture_substeps(e->ctx.data(), d.data(), n, &e->graph[s]), "mhh_graph_capture_substeps");
+            e->recorded[s] = d;
+            ++e->updates;
+        }
+        else if (!same_desc(d, e->recorded[s]))
+        {
+            mhh_host::check(mhh_graph_update(e->graph[s], d.data()), "mhh_graph_update");
+            e->recorded[s] = d;
+            ++e->updates;
+        }
+        int rc = mhh_graph_launch(e->graph[s]);
+        if (rc == MHH_ESTATE)
+        {
+            // a member's deferred-update state changed since the recording
+            mhh_host::check(mhh_graph_update(e->graph[s], d.data()), "mhh_graph_update");
+            ++e->updates;
+            rc = mhh_graph_launch(e->graph[s]);
+        }
+        mhh_host::check(rc, "mhh_graph_launch");
+        ++e->launches;
+        for (mhh_model* mm : e->members) mm->m->graph_done();
+    }
+}
+
+extern "C" {
+
+int mhh_ensemble_create(mhh_model* const* models, int n, int mode, mhh_ensemble** out)
+{
+    if (!models || !out || n < 1 || (mode != MHH_ENSEMBLE_STREAMS && mode != MHH_ENSEMBLE_GRAPH))
+    {
+        mhh_host_set_error("mhh_ensemble_create: bad argument");
+        return MHH_EINVAL;
+    }
+    *out = nullptr;
+    for (int m = 0; m < n; ++m)
+    {
+        if (!models[m]) { mhh_host_set_error("mhh_ensemble_create: null model"); return MHH_EINVAL; }
+        for (int q = 0; q < m; ++q)
+            if (models[q] == models[m]) { mhh_host_set_error("mhh_ensemble_create: a model appears twice"); return MHH_EINVAL; }
+    }
+    auto* e = new mhh_ensemble();
+    e->mode = mode;
+    e->members.assign(models, models + n);
+    for (int m = 0; m < n; ++m) e->ctx.push_back(models[m]->m->ctx());
+    *out = e;
+    return 0;
+}
+
+int mhh_ensemble_substep(mhh_ensemble* e)
+{
+    if (!e) { mhh_host_set_error("null ensemble"); return MHH_EINVAL; }
+    return guarded([&]() { ensemble_substep(e); return 0; });
+}
+
+int mhh_ensemble_steps(mhh_ensemble* e, int n)
+{
+    if (!e || n < 0) { mhh_host_set_error("null ensemble or negative step count"); return MHH_EINVAL; }
+    return guarded([&]() { for (int s = 0; s < 3*n; ++s) ensemble_substep(e); return 0; });
+}
+
+int mhh_ensemble_stats(const mhh_ensemble* e, long long* launches, long long* updates, long long* direct)
+{
+    if (!e) { mhh_host_set_error("null ensemble"); return MHH_EINVAL; }
+    if (launches) *launches = e->launches;
+    if (updates) *updates = e->updates;
+    if (direct) *direct = e->direct;
+    return 0;
+}
+
+int mhh_ensemble_destroy(mhh_ensemble* e)
+{
+    if (!e) return 0;
+    for (mhh_graph*& g : e->graph) { if (g) mhh_graph_destroy(g); g = nullptr; }
+    delete e;
     return 0;
 }
 

@@ -249,6 +249,7 @@ namespace mhh_host
             void advance_substep() { substep = (substep + 1) % 3; }
             double get_sub_time_step() const;
             bool in_substep() const { return substep > 0; }
+            bool is_adaptive() const { return adaptivestep; }
             void set_time_step_limit();
             void set_time_step_limit(unsigned long idtlimin) { idtlim = std::min(idtlim, idtlimin); }
             void set_time_step();
@@ -283,6 +284,13 @@ namespace mhh_host
             virtual double dn() = 0;
             virtual double div() = 0;
             virtual mhh_ctx* ctx() = 0;
+            // The next substep as a graph descriptor (mhh_graph_*); false
+            // when it cannot run as one: an unfused configuration, or the
+            // first substep of a step whose time step is set from the CFL and
+            // dn maxima (adaptive time step).
+            virtual bool graph_desc(mhh_substep_desc* d) = 0;
+            // The host side of a substep that a graph launch ran.
+            virtual void graph_done() = 0;
     };
 
     template<typename TF>
@@ -304,6 +312,8 @@ namespace mhh_host
             double dn() override { return diff->get_dn(timeloop->get_dt()); }
             double div() override { boundary->set_prognostic_cyclic_bcs(); return pres->check_divergence(); }
             mhh_ctx* ctx() override { return ctx_; }
+            bool graph_desc(mhh_substep_desc* d) override;
+            void graph_done() override { timeloop->advance_substep(); timeloop->step_time(); }
 
         private:
             void set_time_step();

@@ -49,6 +49,12 @@ MODEL_SIGNATURES = {
     "mhh_model_dn": (C.c_int, [C.c_void_p, abi.DP]),
     "mhh_model_div": (C.c_int, [C.c_void_p, abi.DP]),
     "mhh_model_ctx": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "mhh_ensemble_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "mhh_ensemble_substep": (C.c_int, [C.c_void_p]),
+    "mhh_ensemble_steps": (C.c_int, [C.c_void_p, C.c_int]),
+    "mhh_ensemble_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                     C.POINTER(C.c_longlong)]),
+    "mhh_ensemble_destroy": (C.c_int, [C.c_void_p]),
 }
 
 _declared = False
@@ -182,6 +188,61 @@ class Model:
     def close(self):
         if getattr(self, "h", None):
             lib().mhh_model_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+MHH_ENSEMBLE_STREAMS, MHH_ENSEMBLE_GRAPH = 0, 1
+
+
+class Ensemble:
+    """Models advanced together (mhh_ensemble_*): mode "streams" issues every
+    member's substep on its own stream without host synchronisation, mode
+    "graph" launches one graph of all members per substep.  Each member
+    needs its own stream: create them with Ensemble.streams(n) and pass one
+    to each Model."""
+
+    def __init__(self, models, mode="streams"):
+        self.models = list(models)
+        arr = (C.c_void_p * len(self.models))(*[m.h for m in self.models])
+        h = C.c_void_p()
+        code = {"streams": MHH_ENSEMBLE_STREAMS, "graph": MHH_ENSEMBLE_GRAPH}[mode]
+        abi.check(lib().mhh_ensemble_create(arr, len(self.models), code, C.byref(h)), "mhh_ensemble_create")
+        self.h = h
+
+    @staticmethod
+    def streams(n):
+        out = []
+        for _ in range(n):
+            s = C.c_void_p()
+            abi.check(abi.lib().mhh_stream_create(C.byref(s)), "mhh_stream_create")
+            out.append(s)
+        return out
+
+    @staticmethod
+    def destroy_streams(streams):
+        for s in streams:
+            abi.lib().mhh_stream_destroy(s)
+
+    def substep(self):
+        abi.check(lib().mhh_ensemble_substep(self.h), "mhh_ensemble_substep")
+
+    def steps(self, n):
+        abi.check(lib().mhh_ensemble_steps(self.h, n), "mhh_ensemble_steps")
+
+    def stats(self):
+        v = [C.c_longlong() for _ in range(3)]
+        abi.check(lib().mhh_ensemble_stats(self.h, *[C.byref(x) for x in v]), "mhh_ensemble_stats")
+        return dict(launches=v[0].value, recordings=v[1].value, direct=v[2].value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mhh_ensemble_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -166,3 +166,52 @@ def test_graph_needs_own_stream():
         assert L.mhh_graph_capture_substep(g.ctx, C.byref(d), C.byref(graph)) == abi.MHH_EINVAL
     finally:
         g.close()
+
+
+@pytest.mark.parametrize("mode", ["streams", "graph"])
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_ensemble_matches_models_run_alone(adaptive, mode):
+    """mhh_ensemble_*: three driver models of different sizes advanced
+    together against the same models stepped one at a time.  With an
+    adaptive time step the first substep of every step runs through the
+    model's own substep and the other two as graph launches (graph mode) or
+    as mhh_substep_exec on the member's stream (streams mode): bitwise the
+    same.  With a fixed time step every substep takes the second route; the
+    first substep of a step then runs the tendency pass without the
+    time-step maxima (a kernel variant compiled apart, whose sums may
+    contract differently): within the 1e-12 parity bar."""
+    from microhh_amd import model as mm
+    sizes = [(32, 24, 16), (48, 40, 24), (64, 32, 20)]
+    cases = [mm.dry_cbl_case(itot=i, jtot=j, ktot=k, xsize=100.0 * i, ysize=100.0 * j, zsize=50.0 * k,
+                             dt=2.0, adaptive=adaptive, rndseed=3 + n) for n, (i, j, k) in enumerate(sizes)]
+    streams = mm.Ensemble.streams(len(cases))
+    alone = [mm.Model(c) for c in cases]
+    members = [mm.Model(c, stream=s) for c, s in zip(cases, streams)]
+    ens = mm.Ensemble(members, mode=mode)
+    try:
+        nsteps = 3
+        for m in alone:
+            m.steps(nsteps)
+        ens.steps(nsteps)
+        st = ens.stats()
+        if mode == "streams":
+            assert st == dict(launches=0, recordings=0, direct=3 * nsteps), st
+        elif adaptive:
+            assert st == dict(launches=2 * nsteps, recordings=st["recordings"], direct=nsteps), st
+        else:
+            assert st["launches"] == 3 * nsteps and st["direct"] == 0, st
+            assert st["recordings"] <= 4, st      # one per substep index, plus substep 0's state after step 1
+        for n, (a, b) in enumerate(zip(alone, members)):
+            assert a.time() == b.time(), n
+            for name in ("u", "v", "w", "th", "p", "ut", "tht"):
+                ra, rb = a.field(name), b.field(name)
+                if adaptive:
+                    assert np.array_equal(ra, rb), f"member {n} {name}"
+                else:
+                    err = np.max(np.abs(ra - rb)) / max(np.max(np.abs(ra)), 1e-300)
+                    assert err < 1e-12, f"member {n} {name}: {err:.3e}"
+    finally:
+        ens.close()
+        for m in alone + members:
+            m.close()
+        mm.Ensemble.destroy_streams(streams)
