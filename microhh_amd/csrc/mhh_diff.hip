@@ -179,7 +179,6 @@ namespace
     // (include/diff_kernels.h:486-511) for the time step.
     constexpr int VTX = 64, VTY = 4;
     constexpr int VPW = VTX + 2, VPH = VTY + 2, VPN = VPW*VPH;
-    constexpr int VKCHUNK = 64;
     constexpr int VNQ = (VPN + VTX*VTY - 1)/(VTX*VTY);
 
     template<typename TF>
@@ -268,7 +267,7 @@ namespace
             TF* __restrict__ evisc, const TF* __restrict__ u, const TF* __restrict__ v,
             const TF* __restrict__ w, const TF* __restrict__ th, const Level<TF>* __restrict__ lev,
             const TF* __restrict__ mlen0, const TF* __restrict__ zc, Visc_scalars<TF> a, Gd g,
-            int nti, int ntj, int nkc, double* __restrict__ red_cfl, double* __restrict__ red_dn)
+            int nti, int ntj, int nkc, int kchunk, double* __restrict__ red_cfl, double* __restrict__ red_dn)
     {
         __shared__ TF lds[8*VPN];
         __shared__ double smc[limits ? VTX*VTY : 1], smd[limits ? VTX*VTY : 1];
@@ -285,7 +284,7 @@ namespace
         const int i0 = g.istart + ti*VTX, j0 = g.jstart + tj*VTY;
         const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
         const bool active = i < g.iend && j < g.jend;
-        const int kb = g.kstart + kc*VKCHUNK, ke = min(kb + VKCHUNK, g.kend);
+        const int kb = g.kstart + kc*kchunk, ke = min(kb + kchunk, g.kend);
         const int kk = g.ijcells;
         // threads outside the domain compute on the clamped column and store nothing
         const int ij = min(i, g.iend-1) + min(j, g.jend-1)*g.icells;
@@ -747,7 +746,8 @@ namespace
         if (!sfc)
         {
             // plane-ring kernel; with `limits` also the CFL and dnmul maxima
-            const int nti = (g.itot + VTX - 1)/VTX, ntj = (g.jtot + VTY - 1)/VTY, nkc = (g.ktot + VKCHUNK - 1)/VKCHUNK;
+            const int nti = (g.itot + VTX - 1)/VTX, ntj = (g.jtot + VTY - 1)/VTY;
+            const int kchunk = march_kchunk(nti*ntj, g.ktot), nkc = (g.ktot + kchunk - 1)/kchunk;
             const int nblocks = (nti*ntj*nkc + 7)/8*8;
             double* rc = nullptr; double* rd = nullptr;
             if (lim)
@@ -765,7 +765,7 @@ namespace
             const TF* ml = mlen_ptr<TF>(c);
             mhh::Timer _t(c, "evisc");
 #define MHH_VISC_LAUNCH(T, L) visc_tile_g<TF, T, L><<<nblocks, vb, 0, c->stream>>>( \
-                a.evisc, a.u, a.v, a.w, a.th, p.lev, ml, p.z, vs, g, nti, ntj, nkc, rc, rd)
+                a.evisc, a.u, a.v, a.w, a.th, p.lev, ml, p.z, vs, g, nti, ntj, nkc, kchunk, rc, rd)
             if (th && lim)       MHH_VISC_LAUNCH(true, true);
             else if (th)         MHH_VISC_LAUNCH(true, false);
             else if (lim)        MHH_VISC_LAUNCH(false, true);

@@ -154,6 +154,17 @@ namespace mhh
     __host__ __device__ inline int xcd_tile(int b, int nb) { return (b % 8)*(nb/8) + b/8; }
     inline int xcd_blocks(int ntiles) { return (ntiles + 7)/8*8; }
 
+    // Levels per block of the k-marching plane-ring kernels: 64, or fewer
+    // on small grids, so that a launch still has about 4 blocks per CU (each
+    // chunk reads a few levels past its ends again; 8 levels is the floor).
+    // The results do not depend on it.
+    inline int march_kchunk(int ntiles_xy, int ktot)
+    {
+        int kc = 64;
+        while (kc > 8 && static_cast<long>(ntiles_xy)*((ktot + kc - 1)/kc) < 1024) kc /= 2;
+        return kc;
+    }
+
     // Max-reductions: each thread folds a chunk of RED_KCHUNK levels of its
     // column, each block writes one partial, one block folds the partials.
     constexpr int RED_KCHUNK = 32;

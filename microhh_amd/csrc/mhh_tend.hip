@@ -26,7 +26,6 @@ namespace
     constexpr int H = 3;                  // horizontal halo of the 2i5 stencil
     constexpr int PW = TX + 2*H, PH = TY + 2*H, PN = PW*PH;
     constexpr int EW = TX + 2, EH = TY + 2, EN = EW*EH;
-    constexpr int KCHUNK = 64;
     constexpr double grav = 9.81;        // include/constants.h:29
 
     // Scalars of the fused tendency; the field and profile pointers are kernel
@@ -539,7 +538,7 @@ namespace
         const TF* dpb = c->mixed ? reinterpret_cast<const TF*>(c->dpbar) : nullptr;
         a.nti = (g.itot + TX - 1)/TX;
         a.ntj = (g.jtot + TY - 1)/TY;
-        a.kchunk = c->tend_kchunk > 0 ? c->tend_kchunk : KCHUNK;
+        a.kchunk = c->tend_kchunk > 0 ? c->tend_kchunk : march_kchunk(a.nti*a.ntj, g.ktot);
         a.nkc = (g.ktot + a.kchunk - 1)/a.kchunk;
         const int ntiles = a.nti*a.ntj*a.nkc;
         const int nblocks = (ntiles + 7)/8*8;
