@@ -125,28 +125,43 @@ namespace
         return check_launch("set_prognostic_cyclic_bcs");
     }
 
-    // calc_ghost_cells_bot_2nd / top_2nd (src/boundary.cxx:700-775) over the full plane.
+    // calc_ghost_cells_bot_2nd / top_2nd (src/boundary.cxx:700-775) over the
+    // full plane, for every field of Boundary::set_ghost_cells (u, v and the
+    // scalars) in one launch: blockIdx.y selects the field.
     template<typename TF>
-    __global__ void ghost_2nd_g(TF* __restrict__ a, const TF* __restrict__ abot, const TF* __restrict__ agradbot,
-                                TF* __restrict__ atop, const TF* __restrict__ agradtop,
-                                TF dzh_bot, TF dzh_top, int botype, int toptype, Gd g)
+    struct Ghost_list
+    {
+        TF* a[2 + MHH_MAX_SCALARS];
+        const TF* bot[2 + MHH_MAX_SCALARS];
+        const TF* gradbot[2 + MHH_MAX_SCALARS];
+        TF* top[2 + MHH_MAX_SCALARS];
+        const TF* gradtop[2 + MHH_MAX_SCALARS];
+        int botype[2 + MHH_MAX_SCALARS], toptype[2 + MHH_MAX_SCALARS];
+    };
+
+    template<typename TF>
+    __global__ void ghost_2nd_g(Ghost_list<TF> gl, TF dzh_bot, TF dzh_top, Gd g)
     {
         const int ij = blockIdx.x*blockDim.x + threadIdx.x;
         if (ij >= g.ijcells) return;
+        const int f = blockIdx.y;
+        TF* __restrict__ a = gl.a[f];
+        const int botype = gl.botype[f], toptype = gl.toptype[f];
         const int kk = g.ijcells, ks = g.kstart, ke = g.kend;
         if (botype == MHH_BC_DIRICHLET)
-            a[ij + (ks-1)*kk] = TF(2.)*abot[ij] - a[ij + ks*kk];
+            a[ij + (ks-1)*kk] = TF(2.)*gl.bot[f][ij] - a[ij + ks*kk];
         else if (botype == MHH_BC_NEUMANN || botype == MHH_BC_FLUX)
-            a[ij + (ks-1)*kk] = -agradbot[ij]*dzh_bot + a[ij + ks*kk];
+            a[ij + (ks-1)*kk] = -gl.gradbot[f][ij]*dzh_bot + a[ij + ks*kk];
 
         if (toptype == MHH_BC_DIRICHLET || toptype == MHH_BC_OFF)
         {
+            TF* __restrict__ atop = gl.top[f];
             if (toptype == MHH_BC_OFF)
                 atop[ij] = TF(3./2.)*a[ij + (ke-1)*kk] - TF(1./2.)*a[ij + (ke-2)*kk];
             a[ij + ke*kk] = TF(2.)*atop[ij] - a[ij + (ke-1)*kk];
         }
         else if (toptype == MHH_BC_NEUMANN || toptype == MHH_BC_FLUX)
-            a[ij + ke*kk] = agradtop[ij]*dzh_top + a[ij + (ke-1)*kk];
+            a[ij + ke*kk] = gl.gradtop[f][ij]*dzh_top + a[ij + (ke-1)*kk];
     }
 
     // calc_buoyancy_tend_2nd (src/thermo_dry.cxx:163-180); threfh_rel is
@@ -266,35 +281,43 @@ namespace
         return check_launch("boundary_cyclic");
     }
 
+    // Check one field's boundary planes and add it to the list.
     template<typename TF>
-    int ghost(mhh_ctx* c, const Prof<TF>& p, void* a, void* bot, void* gbot, void* top, void* gtop,
-              int botype, int toptype, const char* name)
+    int ghost_add(Ghost_list<TF>& gl, int& n, void* a, void* bot, void* gbot, void* top, void* gtop,
+                  int botype, int toptype, const char* name)
     {
-        const Gd& g = c->gd;
         const bool need_bot = botype == MHH_BC_DIRICHLET || botype == MHH_BC_NEUMANN || botype == MHH_BC_FLUX;
         const bool need_top = toptype == MHH_BC_DIRICHLET || toptype == MHH_BC_OFF ||
                               toptype == MHH_BC_NEUMANN || toptype == MHH_BC_FLUX;
+        if (!a)
+            return fail(MHH_ESTATE, std::string("ghost cells: ") + name + " not bound");
         if (need_bot && !(botype == MHH_BC_DIRICHLET ? bot : gbot))
             return fail(MHH_ESTATE, std::string("ghost cells: bottom boundary plane of ") + name + " not bound");
         if (need_top && !((toptype == MHH_BC_DIRICHLET || toptype == MHH_BC_OFF) ? top : gtop))
             return fail(MHH_ESTATE, std::string("ghost cells: top boundary plane of ") + name + " not bound");
-        { mhh::Timer _t(c, "ghost_cells"); ghost_2nd_g<TF><<<(g.ijcells + 255)/256, 256, 0, c->stream>>>(
-                static_cast<TF*>(a), static_cast<const TF*>(bot), static_cast<const TF*>(gbot),
-                static_cast<TF*>(top), static_cast<const TF*>(gtop),
-                TF(c->h_dzh[g.kstart]), TF(c->h_dzh[g.kend]), botype, toptype, g); }
-        return check_launch("ghost_cells");
+        gl.a[n] = static_cast<TF*>(a);
+        gl.bot[n] = static_cast<const TF*>(bot); gl.gradbot[n] = static_cast<const TF*>(gbot);
+        gl.top[n] = static_cast<TF*>(top);       gl.gradtop[n] = static_cast<const TF*>(gtop);
+        gl.botype[n] = botype; gl.toptype[n] = toptype;
+        ++n;
+        return 0;
     }
 
     template<typename TF>
     int set_ghost_cells(mhh_ctx* c, const Prof<TF>& p, const mhh_bc_desc& bc)
     {
         const mhh_field_set& f = c->f;
-        MHH_CHECK(ghost<TF>(c, p, f.u, f.u_bot, f.u_gradbot, f.u_top, f.u_gradtop, bc.mbcbot, bc.mbctop, "u"));
-        MHH_CHECK(ghost<TF>(c, p, f.v, f.v_bot, f.v_gradbot, f.v_top, f.v_gradtop, bc.mbcbot, bc.mbctop, "v"));
-        for (int n = 0; n < f.nscalars; ++n)
-            MHH_CHECK(ghost<TF>(c, p, f.s[n], f.s_bot[n], f.s_gradbot[n], f.s_top[n], f.s_gradtop[n],
-                                bc.sbcbot[n], bc.sbctop[n], "scalar"));
-        return 0;
+        const Gd& g = c->gd;
+        Ghost_list<TF> gl = {};
+        int n = 0;
+        MHH_CHECK(ghost_add<TF>(gl, n, f.u, f.u_bot, f.u_gradbot, f.u_top, f.u_gradtop, bc.mbcbot, bc.mbctop, "u"));
+        MHH_CHECK(ghost_add<TF>(gl, n, f.v, f.v_bot, f.v_gradbot, f.v_top, f.v_gradtop, bc.mbcbot, bc.mbctop, "v"));
+        for (int q = 0; q < f.nscalars; ++q)
+            MHH_CHECK(ghost_add<TF>(gl, n, f.s[q], f.s_bot[q], f.s_gradbot[q], f.s_top[q], f.s_gradtop[q],
+                                    bc.sbcbot[q], bc.sbctop[q], "scalar"));
+        { mhh::Timer _t(c, "ghost_cells"); ghost_2nd_g<TF><<<dim3((g.ijcells + 255)/256, n), 256, 0, c->stream>>>(
+                gl, TF(c->h_dzh[g.kstart]), TF(c->h_dzh[g.kend]), g); }
+        return check_launch("ghost_cells");
     }
 
     template<typename TF>
