@@ -262,12 +262,31 @@ namespace
     // LDS plane ring (halo 1).  Pointers are __restrict__ kernel parameters,
     // so the per-level factors come in through scalar loads.  With `limits`
     // also the block maxima of the 2i5 CFL rate and of dnmul.
+    // A value of an interior column at level plane a also written to the
+    // column's periodic images in the horizontal ghost columns, east-west,
+    // north-south and in the corners: the result of the cyclic update
+    // (boundary_cyclic east-west, then north-south) for grids with itot >=
+    // 2 igc and jtot >= 2 jgc, where every ghost column is the image of
+    // exactly one interior column.
+    template<typename TF>
+    __device__ __forceinline__ void put_images(TF* __restrict__ a, int i, int j, TF v, const Gd& g)
+    {
+        const int ie = (i - g.istart < g.igc) ? i + g.itot : (g.iend - i <= g.igc) ? i - g.itot : -1;
+        const int je = (j - g.jstart < g.jgc) ? j + g.jtot : (g.jend - j <= g.jgc) ? j - g.jtot : -1;
+        if (ie >= 0) a[ie + j*g.icells] = v;
+        if (je >= 0)
+        {
+            a[i + je*g.icells] = v;
+            if (ie >= 0) a[ie + je*g.icells] = v;
+        }
+    }
+
     template<typename TF, bool thermo, bool limits>
     __global__ void __launch_bounds__(VTX*VTY) visc_tile_g(
             TF* __restrict__ evisc, const TF* __restrict__ u, const TF* __restrict__ v,
             const TF* __restrict__ w, const TF* __restrict__ th, const Level<TF>* __restrict__ lev,
             const TF* __restrict__ mlen0, const TF* __restrict__ zc, Visc_scalars<TF> a, Gd g,
-            int nti, int ntj, int nkc, int kchunk, double* __restrict__ red_cfl, double* __restrict__ red_dn)
+            int nti, int ntj, int nkc, int kchunk, int halo, double* __restrict__ red_cfl, double* __restrict__ red_dn)
     {
         __shared__ TF lds[8*VPN];
         __shared__ double smc[limits ? VTX*VTY : 1], smd[limits ? VTX*VTY : 1];
@@ -395,6 +414,17 @@ namespace
                 ev = sq(fac * mlen0[k]) * sqrt(s2);
             }
             if (active) *reinterpret_cast<TF*>(reinterpret_cast<char*>(evisc + size_t(k)*kk) + oij) = ev;
+            if (halo && active)
+            {
+                // the wall mirror and the cyclic update of the pass, done here
+                put_images(evisc + size_t(k)*kk, i, j, ev, g);
+                if (k == g.kstart || k == g.kend-1)
+                {
+                    TF* __restrict__ wl = evisc + size_t(k == g.kstart ? k-1 : k+1)*kk;
+                    wl[i + j*g.icells] = ev;
+                    put_images(wl, i, j, ev, g);
+                }
+            }
             if (limits && active)
             {
                 dm = fmax(dm, static_cast<double>(fabs(ev*a.tPrfac_i*(a.dxidxi + a.dyidyi + L.dzi*L.dzi))));
@@ -748,6 +778,9 @@ namespace
             // plane-ring kernel; with `limits` also the CFL and dnmul maxima
             const int nti = (g.itot + VTX - 1)/VTX, ntj = (g.jtot + VTY - 1)/VTY;
             const int kchunk = march_kchunk(nti*ntj, g.ktot), nkc = (g.ktot + kchunk - 1)/kchunk;
+            // the wall mirror and the cyclic update inside the pass where
+            // every ghost column is one periodic image and kgc = 1
+            const int halo = g.itot >= 2*g.igc && g.jtot >= 2*g.jgc && g.kgc == 1;
             const int nblocks = (nti*ntj*nkc + 7)/8*8;
             double* rc = nullptr; double* rd = nullptr;
             if (lim)
@@ -765,13 +798,14 @@ namespace
             const TF* ml = mlen_ptr<TF>(c);
             mhh::Timer _t(c, "evisc");
 #define MHH_VISC_LAUNCH(T, L) visc_tile_g<TF, T, L><<<nblocks, vb, 0, c->stream>>>( \
-                a.evisc, a.u, a.v, a.w, a.th, p.lev, ml, p.z, vs, g, nti, ntj, nkc, kchunk, rc, rd)
+                a.evisc, a.u, a.v, a.w, a.th, p.lev, ml, p.z, vs, g, nti, ntj, nkc, kchunk, halo, rc, rd)
             if (th && lim)       MHH_VISC_LAUNCH(true, true);
             else if (th)         MHH_VISC_LAUNCH(true, false);
             else if (lim)        MHH_VISC_LAUNCH(false, true);
             else                 MHH_VISC_LAUNCH(false, false);
 #undef MHH_VISC_LAUNCH
             if (lim) *nred = nblocks;
+            if (halo) return check_launch("evisc");
         }
         else if (th && ms)     { mhh::Timer _t(c, "evisc"); evisc_g<TF, true,  true,  true ><<<grd, blk, 0, c->stream>>>(a, g); }
         else if (th && !ms)    { mhh::Timer _t(c, "evisc"); evisc_g<TF, true,  true,  false><<<grd, blk, 0, c->stream>>>(a, g); }
