@@ -109,6 +109,42 @@ namespace
         }
     }
 
+    // Both copies in one launch, for jtot > 1, itot >= igc and jtot >= jgc:
+    // every ghost point of the horizontal frame takes the value of its
+    // periodic source, the interior point (i, j) wrapped by itot and jtot.
+    // That is what the east-west copy followed by the north-south copy
+    // leaves (the corners through the east-west ghost columns), and the
+    // launch reads interior points only and writes ghost points only.
+    // Threads: the east-west ghost columns over all jcells rows, then the
+    // north-south ghost rows over the interior columns.
+    template<typename TF>
+    __global__ void cyclic_frame_multi_g(Field_list<TF> fl, Gd g)
+    {
+        TF* __restrict__ a = fl.f[blockIdx.z];
+        const int t = blockIdx.x*blockDim.x + threadIdx.x;
+        const size_t kk = size_t(blockIdx.y)*g.ijcells;
+        const int new_ = 2*g.igc*g.jcells;
+        int ig, jg, is, js;
+        if (t < new_)
+        {
+            jg = t / (2*g.igc);
+            const int cc = t - jg*2*g.igc;
+            ig = cc < g.igc ? cc : g.iend + cc - g.igc;
+            is = cc < g.igc ? g.iend - g.igc + cc : g.istart + cc - g.igc;
+            js = jg < g.jstart ? jg + g.jtot : jg >= g.jend ? jg - g.jtot : jg;
+        }
+        else
+        {
+            const int u = t - new_;
+            if (u >= 2*g.jgc*g.itot) return;
+            const int r = u / g.itot;
+            ig = is = g.istart + u - r*g.itot;
+            jg = r < g.jgc ? r : g.jend + r - g.jgc;
+            js = r < g.jgc ? g.jend - g.jgc + r : g.jstart + r - g.jgc;
+        }
+        a[kk + ig + size_t(jg)*g.icells] = a[kk + is + size_t(js)*g.icells];
+    }
+
     template<typename TF>
     int cyclic_prognostic(mhh_ctx* c)
     {
@@ -120,6 +156,12 @@ namespace
         fl.f[nf++] = static_cast<TF*>(c->f.w);
         for (int n = 0; n < c->f.nscalars; ++n) fl.f[nf++] = static_cast<TF*>(c->f.s[n]);
         const int new_ = 2*g.igc*g.jcells, nns = 2*g.jgc*g.icells;
+        if (g.jtot > 1 && g.itot >= g.igc && g.jtot >= g.jgc)
+        {
+            const int nt = new_ + 2*g.jgc*g.itot;
+            { mhh::Timer _t(c, "cyclic"); cyclic_frame_multi_g<TF><<<dim3((nt + 255)/256, g.kcells, nf), 256, 0, c->stream>>>(fl, g); }
+            return check_launch("set_prognostic_cyclic_bcs");
+        }
         { mhh::Timer _t(c, "cyclic"); cyclic_ew_multi_g<TF><<<dim3((new_ + 255)/256, g.kcells, nf), 256, 0, c->stream>>>(fl, g); }
         { mhh::Timer _t(c, "cyclic"); cyclic_ns_multi_g<TF><<<dim3((nns + 255)/256, g.kcells, nf), 256, 0, c->stream>>>(fl, g); }
         return check_launch("set_prognostic_cyclic_bcs");
