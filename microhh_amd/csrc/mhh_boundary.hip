@@ -243,8 +243,9 @@ namespace
     // above the interior, then per interior level the jgc rows at each end and
     // the igc columns at each end of the interior rows.
     template<typename TF>
-    __global__ void zero_ghosts_g(TF* __restrict__ at, Gd g)
+    __global__ void zero_ghosts_g(Field_list<TF> fl, Gd g)
     {
+        TF* __restrict__ at = fl.f[blockIdx.y];
         const long long n = static_cast<long long>(blockIdx.x)*blockDim.x + threadIdx.x;
         const long long nfull = 2LL*g.kgc*g.ijcells;
         const int nring = g.ijcells - g.itot*g.jtot;
@@ -278,17 +279,23 @@ namespace
 
 namespace mhh
 {
+    // Zero the ghost cells of n tendency fields (n <= 3 + MHH_MAX_SCALARS)
+    // in one launch.
     template<typename TF>
-    int zero_tendency_ghosts(mhh_ctx* c, TF* at)
+    int zero_tendency_ghosts(mhh_ctx* c, TF* const* at, int n)
     {
         const Gd& g = c->gd;
+        if (n < 1) return 0;
+        if (n > 3 + MHH_MAX_SCALARS) return fail(MHH_EINVAL, "zero_tendency_ghosts: too many fields");
+        Field_list<TF> fl{};
+        for (int q = 0; q < n; ++q) fl.f[q] = at[q];
         const long long total = 2LL*g.kgc*g.ijcells + static_cast<long long>(g.ijcells - g.itot*g.jtot)*g.ktot;
         const unsigned int nb = static_cast<unsigned int>((total + 255)/256);
-        { mhh::Timer _t(c, "rk3_zero_ghosts"); zero_ghosts_g<TF><<<nb, 256, 0, c->stream>>>(at, g); }
+        { mhh::Timer _t(c, "rk3_zero_ghosts"); zero_ghosts_g<TF><<<dim3(nb, n), 256, 0, c->stream>>>(fl, g); }
         return check_launch("zero_ghosts");
     }
-    template int zero_tendency_ghosts<double>(mhh_ctx*, double*);
-    template int zero_tendency_ghosts<float>(mhh_ctx*, float*);
+    template int zero_tendency_ghosts<double>(mhh_ctx*, double* const*, int);
+    template int zero_tendency_ghosts<float>(mhh_ctx*, float* const*, int);
 
     template<typename TF>
     int rk3_field(mhh_ctx* c, TF* a, TF* at, TF cBdt, TF cAn, int zero)
@@ -297,7 +304,7 @@ namespace mhh
         { mhh::Timer _t(c, "rk3"); rk3_g<TF><<<grid3(g, g.kstart, g.kend), dim3(BX, BY), 0, c->stream>>>(
                 a, at, cBdt, cAn, zero, g); }
         MHH_CHECK(check_launch("rk3"));
-        return zero ? zero_tendency_ghosts<TF>(c, at) : 0;
+        return zero ? zero_tendency_ghosts<TF>(c, &at, 1) : 0;
     }
     template int rk3_field<double>(mhh_ctx*, double*, double*, double, double, int);
     template int rk3_field<float>(mhh_ctx*, float*, float*, float, float, int);
@@ -391,14 +398,15 @@ namespace
             fld[nf] = c->f.v; ten[nf++] = c->f.vt;
             fld[nf] = c->f.w; ten[nf++] = c->f.wt;
         }
+        TF* tl[3 + MHH_MAX_SCALARS];
         for (int q = 0; q < nf; ++q)
         {
             { mhh::Timer _t(c, "rk3"); rk3_g<TF><<<grid3(g, g.kstart, g.kend), dim3(BX, BY), 0, c->stream>>>(
                     static_cast<TF*>(fld[q]), static_cast<TF*>(ten[q]), cBdt, cA[substepn], zero, g); }
-            if (zero)
-                MHH_CHECK(zero_tendency_ghosts<TF>(c, static_cast<TF*>(ten[q])));
+            tl[q] = static_cast<TF*>(ten[q]);
         }
-        return check_launch("rk3");
+        MHH_CHECK(check_launch("rk3"));
+        return zero ? zero_tendency_ghosts<TF>(c, tl, nf) : 0;
     }
 }
 

@@ -191,7 +191,7 @@ namespace mhh
     };
 
     // Zero the ghost cells of a tendency field (mhh_boundary.hip).
-    template<typename TF> int zero_tendency_ghosts(mhh_ctx* c, TF* at);
+    template<typename TF> int zero_tendency_ghosts(mhh_ctx* c, TF* const* at, int n);
     // RK3 update of one field and its tendency (a += cBdt*at; at = zero ? 0 :
     // cAn*at), the tendency's ghost cells cleared when zero.
     template<typename TF> int rk3_field(mhh_ctx* c, TF* a, TF* at, TF cBdt, TF cAn, int zero);

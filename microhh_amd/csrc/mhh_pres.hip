@@ -1752,13 +1752,11 @@ namespace
         for (int n = 1; n < ns; ++n)
             MHH_CHECK(rk3_field<TF>(c, static_cast<TF*>(c->f.s[n]), static_cast<TF*>(c->f.st[n]),
                                     cBdt, cA[substepn], substepn == 0));
-        if (substepn == 0 && ns > 0)
-            MHH_CHECK(zero_tendency_ghosts<TF>(c, static_cast<TF*>(c->f.st[0])));
         if (substepn == 0)
         {
-            MHH_CHECK(zero_tendency_ghosts<TF>(c, static_cast<TF*>(c->f.ut)));
-            MHH_CHECK(zero_tendency_ghosts<TF>(c, static_cast<TF*>(c->f.vt)));
-            MHH_CHECK(zero_tendency_ghosts<TF>(c, static_cast<TF*>(c->f.wt)));
+            TF* tl[4] = {static_cast<TF*>(c->f.ut), static_cast<TF*>(c->f.vt), static_cast<TF*>(c->f.wt),
+                         ns > 0 ? static_cast<TF*>(c->f.st[0]) : nullptr};
+            MHH_CHECK(zero_tendency_ghosts<TF>(c, tl, ns > 0 ? 4 : 3));
         }
         return 0;
     }
