@@ -632,11 +632,24 @@ namespace
         // bins 0 and n/2 both in C[k][0][j])
         const T* Ck = C + static_cast<size_t>(k)*nx*M;
         const int nkx = packed ? nx - 1 : nx;
-        for (int idx = tid; idx < nkx*JB; idx += NT)
+        // every load of the thread in flight before the first LDS store
+        constexpr int NLD = (nx*JB + NT - 1)/NT;
+        T vals[NLD];
+        #pragma unroll
+        for (int u = 0; u < NLD; ++u)
         {
+            const int idx = tid + u*NT;
+            if (idx < nkx*JB)
+                vals[u] = Ck[static_cast<size_t>(idx / JB)*M + j0 + idx % JB];
+        }
+        #pragma unroll
+        for (int u = 0; u < NLD; ++u)
+        {
+            const int idx = tid + u*NT;
+            if (idx >= nkx*JB) break;
             const int kx = idx / JB, jl = idx % JB;
             const int r = jl >> 1, part = jl & 1;
-            const T val = Ck[static_cast<size_t>(kx)*M + j0 + jl];
+            const T val = vals[u];
             T* Sr = rows[r];
             if (kx == 0 && packed)
             {
