@@ -44,3 +44,32 @@ def test_error_paths_without_gpu():
     h = C.c_void_p()
     assert L.mhh_ctx_create(C.byref(gd), 0, None, C.byref(h)) == 1   # ktot < 6
     assert b"ktot" in L.mhh_last_error()
+
+
+def test_graph_and_ensemble_argument_checks_without_gpu():
+    """The graph, direct-substep and ensemble entry points reject bad
+    arguments before any HIP call."""
+    from microhh_amd import model
+    L = abi.lib()
+    ML = model.lib()
+    out = C.c_void_p()
+    d = abi.SubstepDesc()
+    assert L.mhh_substep_exec(None, C.byref(d)) == abi.MHH_EINVAL
+    assert L.mhh_graph_capture_substep(None, C.byref(d), C.byref(out)) == abi.MHH_EINVAL
+    assert L.mhh_graph_capture_substeps(None, C.byref(d), 1, C.byref(out)) == abi.MHH_EINVAL
+    ctxs = (C.c_void_p * 2)(None, None)
+    assert L.mhh_graph_capture_substeps(ctxs, C.byref(d), 0, C.byref(out)) == abi.MHH_EINVAL
+    assert L.mhh_graph_capture_substeps(ctxs, C.byref(d), 2, C.byref(out)) == abi.MHH_EINVAL
+    assert b"null context" in L.mhh_last_error()
+    assert L.mhh_graph_update(None, C.byref(d)) == abi.MHH_EINVAL
+    assert L.mhh_graph_launch(None) == abi.MHH_EINVAL
+    assert L.mhh_graph_destroy(None) == 0
+    models = (C.c_void_p * 2)(None, None)
+    assert ML.mhh_ensemble_create(models, 0, model.MHH_ENSEMBLE_STREAMS, C.byref(out)) == abi.MHH_EINVAL
+    assert ML.mhh_ensemble_create(models, 2, 7, C.byref(out)) == abi.MHH_EINVAL
+    assert ML.mhh_ensemble_create(models, 2, model.MHH_ENSEMBLE_GRAPH, C.byref(out)) == abi.MHH_EINVAL
+    assert b"null model" in L.mhh_last_error()
+    assert ML.mhh_ensemble_substep(None) == abi.MHH_EINVAL
+    assert ML.mhh_ensemble_steps(None, 1) == abi.MHH_EINVAL
+    assert ML.mhh_ensemble_stats(None, None, None, None) == abi.MHH_EINVAL
+    assert ML.mhh_ensemble_destroy(None) == 0

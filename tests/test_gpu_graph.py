@@ -171,19 +171,24 @@ def test_graph_needs_own_stream():
 @pytest.mark.parametrize("mode", ["streams", "graph"])
 @pytest.mark.parametrize("adaptive", [False, True])
 def test_ensemble_matches_models_run_alone(adaptive, mode):
-    """mhh_ensemble_*: three driver models of different sizes advanced
-    together against the same models stepped one at a time.  With an
+    """mhh_ensemble_*: four driver models of different sizes (one of them
+    in the mixed mode) advanced together against the same models stepped
+    one at a time.  With an
     adaptive time step the first substep of every step runs through the
     model's own substep and the other two as graph launches (graph mode) or
     as mhh_substep_exec on the member's stream (streams mode): bitwise the
     same.  With a fixed time step every substep takes the second route; the
     first substep of a step then runs the tendency pass without the
     time-step maxima (a kernel variant compiled apart, whose sums may
-    contract differently): within the 1e-12 parity bar."""
+    contract differently): within the 1e-12 parity bar in FP64, and within
+    a few FP32 roundings (1e-5) for the mixed member."""
     from microhh_amd import model as mm
-    sizes = [(32, 24, 16), (48, 40, 24), (64, 32, 20)]
+    # three FP64 members and one in the mixed mode (FP32 fields, FP64 mean pressure)
+    sizes = [(32, 24, 16, mm.MHH_FP64), (48, 40, 24, mm.MHH_FP64), (64, 32, 20, mm.MHH_FP64),
+             (40, 32, 16, mm.MHH_MIXED)]
     cases = [mm.dry_cbl_case(itot=i, jtot=j, ktot=k, xsize=100.0 * i, ysize=100.0 * j, zsize=50.0 * k,
-                             dt=2.0, adaptive=adaptive, rndseed=3 + n) for n, (i, j, k) in enumerate(sizes)]
+                             dt=2.0, adaptive=adaptive, rndseed=3 + n, precision=prec)
+             for n, (i, j, k, prec) in enumerate(sizes)]
     streams = mm.Ensemble.streams(len(cases))
     alone = [mm.Model(c) for c in cases]
     members = [mm.Model(c, stream=s) for c, s in zip(cases, streams)]
@@ -208,8 +213,10 @@ def test_ensemble_matches_models_run_alone(adaptive, mode):
                 if adaptive:
                     assert np.array_equal(ra, rb), f"member {n} {name}"
                 else:
+                    # the mixed member's variants differ by FP32 roundings
+                    tol = 1e-12 if sizes[n][3] == mm.MHH_FP64 else 1e-5
                     err = np.max(np.abs(ra - rb)) / max(np.max(np.abs(ra)), 1e-300)
-                    assert err < 1e-12, f"member {n} {name}: {err:.3e}"
+                    assert err < tol, f"member {n} {name}: {err:.3e}"
     finally:
         ens.close()
         for m in alone + members:
