@@ -266,6 +266,8 @@ int  mhh_graph_capture_substep(mhh_ctx* ctx, const mhh_substep_desc* desc, mhh_g
 int  mhh_graph_capture_substeps(mhh_ctx* const* ctx, const mhh_substep_desc* desc, int n, mhh_graph** out);
 int  mhh_graph_update(mhh_graph* graph, const mhh_substep_desc* desc);
 int  mhh_graph_launch(mhh_graph* graph);
+/* Waits for the members' streams, then frees the graph.  The contexts (and
+ * their bound fields) must outlive the graph: destroy it first. */
 int  mhh_graph_destroy(mhh_graph* graph);
 
 /* ---- device buffers ----
