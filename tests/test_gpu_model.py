@@ -164,3 +164,28 @@ def test_fp32_runs_and_projects():
         assert np.all(np.isfinite(m.field(n))), n
     assert m.div() < 1e-3
     m.close()
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_time_step_closed_forms(fused):
+    """The GPU driver's adaptive time step against the closed forms the
+    oracle is pinned by (tests/test_oracle_pins.py): a CFL-limited step
+    under uniform flow, dt = cflmax dx / U, through the CFL of the fused
+    tendency pass or of the separate reduction, and a dn-limited step of
+    constant viscosity, dt = dnmax / (visc (1/dx^2 + 1/dy^2 + 1/dz^2)); both
+    to the 1 ns truncation of the integer time step."""
+    from tests.test_oracle_pins import _uniform_case, _cfl_dt, _dn_dt
+    _, case = _uniform_case()
+    case["fused"] = 1 if fused else 0
+    m = mm.Model(case)
+    u = m.field("u")
+    u[...] = 10.0
+    m.set_field("u", u)
+    m.substep()
+    assert abs(m.time()[1] - _cfl_dt(10.0)) <= 2e-9, m.time()
+    m.close()
+    _, case = _uniform_case(diff=1, visc=100.0)
+    m = mm.Model(case)
+    m.substep()
+    assert abs(m.time()[1] - _dn_dt(100.0)) <= 2e-9, m.time()
+    m.close()

@@ -534,3 +534,49 @@ def test_pins_catch_planted_mistakes():
     wrong = out.copy()
     wrong[n // 2 + 1:] *= -1
     assert np.max(np.abs(wrong - hc_numpy(x))) > 1e-3
+
+
+# ------------------------------------------------- time-step integerisation
+def _uniform_case(diff=0, visc=1e-5):
+    """A neutral column at rest with free-slip walls (no shear) and zero heat
+    flux: dx = dy = 100 m, dz = 50 m, first dt 10 s, dtmax 60 s."""
+    from microhh_amd import model as mm
+    case = mm.dry_cbl_case(itot=16, jtot=12, ktot=8, xsize=1600.0, ysize=1200.0, zsize=400.0,
+                           adaptive=True, dt=10.0, dtmax=60.0, rndamp_uvw=0.0, rndamp_th=0.0, diff=diff)
+    case.update(mbcbot=mm.abi.BC["freeslip"], sbcbot=mm.abi.BC["neumann"], sbot=0.0, stop=0.0,
+                visc=visc, svisc=visc)
+    return mm, case
+
+
+def _cfl_dt(U):
+    # src/timeloop.cxx:138-246 with src/advec_2i5.cxx:973-992: the 2i5 CFL of
+    # a uniform flow U along x is U dt / dx, so one adaptive step sets
+    # dt = cflmax dx / U (idt truncated to whole nanoseconds)
+    return 1.2 * 100.0 / U
+
+
+def _dn_dt(visc):
+    # src/diff_2.cxx:114-130: dnmul = visc (1/dx^2 + 1/dy^2 + 1/dz^2), so the
+    # dn limit sets dt = dnmax / dnmul
+    return 0.4 / (visc * (1 / 100.0 ** 2 + 1 / 100.0 ** 2 + 1 / 50.0 ** 2))
+
+
+def test_time_step_closed_forms():
+    """The adaptive time step of the oracle's driver against two closed
+    forms: a CFL-limited step under uniform flow (U = 10 m/s: 12 s) and a
+    dn-limited step of constant viscosity at rest (visc = 100 m2/s:
+    6.67 s).  A swapped ratio, a missing cflmax or dnmax, or dt used where
+    the integer idt belongs moves them by far more than the 1 ns
+    truncation."""
+    mm, case = _uniform_case()
+    m = po.Model(mm.oracle_case(case), *mm.profiles(case))
+    m.field("u")[...] = 10.0
+    m.substep()
+    assert abs(m.dt - _cfl_dt(10.0)) <= 2e-9, m.dt
+    m.close()
+
+    mm, case = _uniform_case(diff=1, visc=100.0)
+    m = po.Model(mm.oracle_case(case), *mm.profiles(case))
+    m.substep()
+    assert abs(m.dt - _dn_dt(100.0)) <= 2e-9, m.dt
+    m.close()
