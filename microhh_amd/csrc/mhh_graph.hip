@@ -6,7 +6,10 @@
 // eddy viscosity, the fused tendency pass, and the projection with the RK3
 // update.  Each context records on its own stream; the streams fork from and
 // join into the first one during capture, so the members are independent
-// branches of the graph and run concurrently.  The kernel arguments (the sub
+// branches of the graph; whether the branches overlap is the runtime's
+// choice (measured on ROCm 7.2: they do not, and 64 small members run 2.2x
+// slower as one graph than issued to their streams with mhh_substep_exec,
+// which the ensemble driver therefore does by default).  The kernel arguments (the sub
 // time step among them) are those of the capture; mhh_graph_update records
 // the sequence again for new values and updates the instantiated graph in
 // place.
@@ -29,6 +32,7 @@ struct mhh_graph
     hipGraphExec_t exec = nullptr;
     hipEvent_t fork = nullptr;
     std::vector<hipEvent_t> join;
+    bool valid = true;      // false after a failed in-place update: record again first
 };
 
 namespace
@@ -159,11 +163,21 @@ int mhh_graph_update(mhh_graph* g, const mhh_substep_desc* desc)
 {
     if (!g || !desc) return fail(MHH_EINVAL, "mhh_graph_update: null argument");
     const int n = static_cast<int>(g->ctx.size());
+    // a failed recording leaves the graph as it was
+    const std::vector<mhh_substep_desc> old_desc = g->desc;
+    const std::vector<char> old_pend_in = g->pend_in, old_pend_out = g->pend_out;
+    const std::vector<double> old_cA_in = g->cA_in, old_cA_out = g->cA_out;
     g->desc.assign(desc, desc + n);
     // the entry state is the contexts' current one
     for (int m = 0; m < n; ++m) { g->pend_in[m] = g->ctx[m]->defer_pending; g->cA_in[m] = g->ctx[m]->defer_cA; }
     hipGraph_t graph = nullptr;
-    MHH_CHECK(record(g, &graph));
+    if (const int rc = record(g, &graph))
+    {
+        g->desc = old_desc;
+        g->pend_in = old_pend_in; g->pend_out = old_pend_out;
+        g->cA_in = old_cA_in; g->cA_out = old_cA_out;
+        return rc;
+    }
     hipGraphExecUpdateResult res;
     hipGraphNode_t bad = nullptr;
     if (hipGraphExecUpdate(g->exec, graph, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess)
@@ -172,18 +186,27 @@ int mhh_graph_update(mhh_graph* g, const mhh_substep_desc* desc)
         (void)hipGetLastError();
         hipGraphExec_t exec = nullptr;
         const int rc = check_hip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-        if (rc) { (void)hipGraphDestroy(graph); return rc; }
+        if (rc)
+        {
+            // the executable graph may be partly updated: unusable until a
+            // later update succeeds
+            (void)hipGraphDestroy(graph);
+            g->valid = false;
+            return rc;
+        }
         (void)hipGraphExecDestroy(g->exec);
         g->exec = exec;
     }
     (void)hipGraphDestroy(g->graph);
     g->graph = graph;
+    g->valid = true;
     return 0;
 }
 
 int mhh_graph_launch(mhh_graph* g)
 {
     if (!g) return fail(MHH_EINVAL, "mhh_graph_launch: null graph");
+    if (!g->valid) return fail(MHH_ESTATE, "mhh_graph_launch: the last mhh_graph_update failed; update again first");
     const int n = static_cast<int>(g->ctx.size());
     for (int m = 0; m < n; ++m)
         if (g->ctx[m]->defer_pending != bool(g->pend_in[m]) || (g->pend_in[m] && g->ctx[m]->defer_cA != g->cA_in[m]))
