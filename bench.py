@@ -61,6 +61,9 @@ PASSES = {
     # y transform + forward elimination, back substitution + inverse y
     # transform (fused): spectrum in and out = 2 each
     "ytri_fwd": 2, "ytri_bwd": 2,
+    # 2nd-order advection (same reads and writes as advec_2i5) and the
+    # constant-viscosity diffusion of one field (reads a and at, writes at)
+    "advec_2_u": 5, "advec_2_v": 5, "advec_2_w": 5, "advec_2_s": 6, "diff_2": 3,
 }
 KERNELS = list(PASSES) + ["cyclic", "ghost_cells", "mirror_walls", "rk3_zero_ghosts", "cfl", "dnmul", "pres_mean"]
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
@@ -317,6 +320,8 @@ def main():
     ap.add_argument("--itot", type=int, default=512)
     ap.add_argument("--jtot", type=int, default=512)
     ap.add_argument("--ktot", type=int, default=256)
+    ap.add_argument("--advec", default="2i5", choices=["2i5", "2"], help="[advec] swadvec")
+    ap.add_argument("--diff", default="smag2", choices=["smag2", "2"], help="[diff] swdiff")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel events in the timed steps")
@@ -352,7 +357,8 @@ def main():
     esize = 8 if prec == mm.MHH_FP64 else 4
     itot, jtot, ktot = args.itot, args.jtot, args.ktot
     case = mm.dry_cbl_case(itot=itot, jtot=jtot, ktot=ktot, xsize=50.0 * itot, ysize=50.0 * jtot,
-                           zsize=12.5 * ktot, precision=prec)
+                           zsize=12.5 * ktot, precision=prec, advec=0 if args.advec == "2i5" else 1,
+                           diff=0 if args.diff == "smag2" else 1)
     log(f"bench: creating {itot}x{jtot}x{ktot} {args.precision} model")
     m = mm.Model(case)
     L = abi.lib()
@@ -425,7 +431,8 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if prec == mm.MHH_FP64 else "f32", "data": "synthetic",
-        "config": {"workload": f"dry CBL LES {itot}x{jtot}x{ktot} {args.precision.upper()}, advec_2i5 + smag2 + "
+        "config": {"workload": f"dry CBL LES {itot}x{jtot}x{ktot} {args.precision.upper()}, advec_{args.advec} + "
+                               f"{'smag2' if args.diff == 'smag2' else 'diff_2'} + "
                                f"pres_2 (FFT + tridiagonal), RK3, adaptive dt (CFL 1.2), seeded noise fields",
                    "itot": itot, "jtot": jtot, "ktot": ktot, "substeps_per_step": 3,
                    "parallelism": "single GPU"},
