@@ -1066,22 +1066,36 @@ namespace
         // Nyquist entries of a real row are real.  The staging buffer is the row
         // buffer itself (in-place engine) or the row's second buffer (fallback).
         const int soff = plan.fast ? 0 : row_half(n);
-        for (int idx = tid; idx < nx*JB; idx += nthr)
+        // in chunks of SB loads per thread, all in flight before their LDS stores
+        constexpr int SB = 8;
+        for (int base = tid; base < nx*JB; base += SB*nthr)
         {
-            const int kx = idx / JB, jl = idx % JB;
-            const int r = jl >> 1, part = jl & 1;
-            const int j = j0 + jl;
-            T val;
-            if (j < M) val = Ck[static_cast<size_t>(kx)*M + j];
-            else { val.x = TF(0); val.y = TF(0); }
-            T* Sr = rows + r*re + soff;
-            if (kx == 0 || 2*kx == n)
+            T vals[SB];
+            #pragma unroll
+            for (int u = 0; u < SB; ++u)
             {
-                TF* d = reinterpret_cast<TF*>(&Sr[kx == 0 ? 0 : 1]);
-                d[part] = val.x;
+                const int idx = base + u*nthr;
+                const int j = j0 + idx % JB;
+                vals[u].x = TF(0); vals[u].y = TF(0);
+                if (idx < nx*JB && j < M) vals[u] = Ck[static_cast<size_t>(idx / JB)*M + j];
             }
-            else
-                Sr[lpad(stage_base(kx, n) + part)] = val;
+            #pragma unroll
+            for (int u = 0; u < SB; ++u)
+            {
+                const int idx = base + u*nthr;
+                if (idx >= nx*JB) break;
+                const int kx = idx / JB, jl = idx % JB;
+                const int r = jl >> 1, part = jl & 1;
+                const T val = vals[u];
+                T* Sr = rows + r*re + soff;
+                if (kx == 0 || 2*kx == n)
+                {
+                    TF* d = reinterpret_cast<TF*>(&Sr[kx == 0 ? 0 : 1]);
+                    d[part] = val.x;
+                }
+                else
+                    Sr[lpad(stage_base(kx, n) + part)] = val;
+            }
         }
         __syncthreads();
         T* A = rows + row*re;
