@@ -127,10 +127,10 @@ def cpu_baseline():
     return out
 
 
-def copy_calibration(nbytes=1 << 30, reps=5):
+def measure_copy(nbytes=1 << 30, reps=5):
     """Device-to-device copies of a known size through the library's plain
-    streaming copy kernel: the reference point for the HBM traffic counters
-    (a copy of B bytes reads B and writes B) and a measured copy bandwidth."""
+    streaming copy kernel (each call synchronises): ms per copy and the
+    read + write bandwidth."""
     from microhh_amd import abi
     L = abi.lib()
     src, dst = C.c_void_p(), C.c_void_p()
@@ -143,8 +143,15 @@ def copy_calibration(nbytes=1 << 30, reps=5):
     el = (time.perf_counter() - t0) / reps
     L.mhh_device_free(src)
     L.mhh_device_free(dst)
+    return el * 1e3, 2 * nbytes / el / 1e9
+
+
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """The reference point for the HBM traffic counters (a copy of B bytes
+    reads B and writes B) and a measured copy bandwidth."""
+    ms, gbs = measure_copy(nbytes, reps)
     print(json.dumps({"mode": "copy_calibration", "bytes_per_copy": nbytes, "copies": reps + 1,
-                      "ms_per_copy": round(el * 1e3, 4), "gbs_read_plus_write": round(2 * nbytes / el / 1e9, 1)}))
+                      "ms_per_copy": round(ms, 4), "gbs_read_plus_write": round(gbs, 1)}))
 
 
 def kernel_table(L, abi, ctx, npts, esize, itot, only=None):
@@ -411,6 +418,12 @@ def main():
                 "launches": live["launches"]}
     sub_ms = elapsed / nsub * 1e3
     substep_gbs = B_ALG_PASSES * esize * npts / (sub_ms * 1e-3) / 1e9
+    # the achievable streaming rate on this GPU: the library's copy kernel
+    # (1 GiB read + 1 GiB written), measured after the timed steps
+    _, copy_gbs = measure_copy()
+    if roof:
+        roof["measured_copy_gbs"] = round(copy_gbs, 1)
+        roof["frac_of_copy"] = round(roof["achieved"] / copy_gbs, 4)
     div = m.div()
     t, dt, _, it = m.time()
     mixed = None
@@ -439,7 +452,9 @@ def main():
         "roofline": roof,
         "substep_roofline": {"alg_passes": B_ALG_PASSES, "ms_per_substep": round(sub_ms, 3),
                              "achieved": round(substep_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                             "frac": round(substep_gbs / HBM_PEAK_GBS, 4)},
+                             "frac": round(substep_gbs / HBM_PEAK_GBS, 4),
+                             "measured_copy_gbs": round(copy_gbs, 1),
+                             "frac_of_copy": round(substep_gbs / copy_gbs, 4)},
         "cpu_baseline": cpu,
         "mixed": mixed,
         "kernels": kernels,

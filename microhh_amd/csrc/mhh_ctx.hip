@@ -367,7 +367,8 @@ int mhh_copy_device(void* dst, const void* src, size_t bytes)
         return mhh::fail(MHH_EINVAL, "mhh_copy_device: pointers and size must be 16-byte aligned");
     if (!bytes) return 0;
     const size_t n = bytes / 16;
-    const unsigned int nb = static_cast<unsigned int>(std::min<size_t>((n + 255)/256, 8192));
+    // one element per thread up to 2^24 blocks (4 GiB), grid-stride beyond
+    const unsigned int nb = static_cast<unsigned int>(std::min<size_t>((n + 255)/256, size_t(1) << 24));
     copy_g<<<nb, 256>>>(static_cast<double2*>(dst), static_cast<const double2*>(src), n);
     MHH_CHECK(mhh::check_launch("copy"));
     return mhh::check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
