@@ -257,7 +257,7 @@ def ensemble_bench(args):
     members per substep).  The same members stepped one model at a time
     (mhh_model_substep, member after member) are timed beside it.  The surface forcing is a
     per-member surface theta (Dirichlet, 300.2-302 K): a prescribed heat flux
-    needs the surface layer, which this build does not have."""
+    needs the surface layer, which microhh_amd does not implement."""
     import numpy as np
     from microhh_amd import abi, model as mm
     rng = np.random.default_rng(20240607)
@@ -349,14 +349,14 @@ def main():
         return
     if args.workload == "ensemble":
         if args.gpus != 1:
-            print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+            print(json.dumps({"error": "microhh_amd runs on one GPU; --gpus must be 1"}))
             sys.exit(2)
         ensemble_bench(args)
         return
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a "
-                                   "design in DESIGN.md, not a measured path)"}))
+        print(json.dumps({"error": "microhh_amd runs on one GPU; --gpus must be 1 (the x-y sharding of the "
+                                   "path across GPUs is designed, not implemented)"}))
         sys.exit(2)
 
     from microhh_amd import abi, model as mm
